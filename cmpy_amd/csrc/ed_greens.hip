@@ -1,0 +1,472 @@
+// Green's-function kernels.
+//
+// Reference counterparts:
+//   _apply_{creation,annihilation}_{up,dn}   cmpy/operators.py:652-703
+//   CreationOperator / AnnihilationOperator  cmpy/operators.py:706-791
+//   _accumulate_sum / accumulate_gf          cmpy/exactdiag.py:110-134
+//   solve_sector (np.linalg.eigh)            cmpy/exactdiag.py:46-57
+// The continued fraction has no reference counterpart (cmpy has none).
+#include "ed_internal.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <mutex>
+
+namespace ed {
+
+// ---------------------------------------------------------------------------
+// c^dagger / c as gathers over the target sector.  The reference scatters
+// source blocks into a zeroed target (operators.py:652-703); a gather writes
+// every target element exactly once, so no atomics and no separate memset.
+// ---------------------------------------------------------------------------
+
+__global__ void k_unrank_states(uint32_t* out, int64_t count, int num_sites, int n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = comb_unrank(i, num_sites, n);
+}
+
+// sigma = up: one block per target row (grid-stride), lanes over columns.
+__global__ __launch_bounds__(256) void k_ladder_up(int num_sites, int n_tgt, int pos, int create, int jw,
+                                                   const double* __restrict__ x, double* __restrict__ y,
+                                                   int64_t rows_tgt, int64_t cols) {
+  const uint32_t bit = 1u << pos;
+  for (int64_t it = blockIdx.x; it < rows_tgt; it += gridDim.x) {
+    const uint32_t st = comb_unrank(it, num_sites, n_tgt);
+    const bool has = (st & bit) != 0;
+    const bool valid = create ? has : !has;
+    int64_t is = 0;
+    double sg = 1.0;
+    if (valid) {
+      const uint32_t ss = st ^ bit;
+      is = comb_rank(ss);
+      if (jw && (popc32(ss & (bit - 1u)) & 1)) sg = -1.0;
+    }
+    for (int64_t c = threadIdx.x; c < cols; c += blockDim.x)
+      y[it * cols + c] = valid ? sg * x[is * cols + c] : 0.0;
+  }
+}
+
+// sigma = dn: lanes over target columns, grid.y over row chunks.
+__global__ __launch_bounds__(256) void k_ladder_dn(int num_sites, int n_tgt, int pos, int create, int jw,
+                                                   const double* __restrict__ x, double* __restrict__ y,
+                                                   int64_t rows, int64_t cols_src, int64_t cols_tgt,
+                                                   int64_t rows_per_block) {
+  const int64_t jt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (jt >= cols_tgt) return;
+  const uint32_t bit = 1u << pos;
+  const uint32_t st = comb_unrank(jt, num_sites, n_tgt);
+  const bool has = (st & bit) != 0;
+  const bool valid = create ? has : !has;
+  int64_t js = 0;
+  double sg = 1.0;
+  if (valid) {
+    const uint32_t ss = st ^ bit;
+    js = comb_rank(ss);
+    if (jw && (popc32(ss & (bit - 1u)) & 1)) sg = -1.0;
+  }
+  const int64_t r0 = (int64_t)blockIdx.y * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < rows ? r0 + rows_per_block : rows;
+  for (int64_t r = r0; r < r1; ++r) y[r * cols_tgt + jt] = valid ? sg * x[r * cols_src + js] : 0.0;
+}
+
+static int ladder(int num_sites, int n_up, int n_dn, int pos, int sigma, int sign_mode, int create,
+                  const double* x, double* y, int64_t nvec, hipStream_t s) {
+  ED_REQUIRE(num_sites >= 1 && num_sites <= ED_MAX_SITES, "num_sites=%d outside [1, %d]", num_sites,
+             ED_MAX_SITES);
+  ED_REQUIRE(n_up >= 0 && n_up <= num_sites && n_dn >= 0 && n_dn <= num_sites, "sector (%d, %d) invalid",
+             n_up, n_dn);
+  ED_REQUIRE(pos >= 0 && pos < num_sites, "pos=%d outside [0, %d)", pos, num_sites);
+  ED_REQUIRE(sigma == ED_UP || sigma == ED_DN, "sigma must be ED_UP or ED_DN");
+  ED_REQUIRE(sign_mode == ED_SIGN_REFERENCE || sign_mode == ED_SIGN_JORDAN_WIGNER, "bad sign_mode");
+  ED_REQUIRE(x && y, "NULL buffer");
+  ED_REQUIRE(nvec >= 0, "nvec < 0");
+  const int d = create ? 1 : -1;
+  const int nu_t = sigma == ED_UP ? n_up + d : n_up;
+  const int nd_t = sigma == ED_DN ? n_dn + d : n_dn;
+  ED_REQUIRE(nu_t >= 0 && nu_t <= num_sites && nd_t >= 0 && nd_t <= num_sites,
+             "target sector (%d, %d) does not exist", nu_t, nd_t);
+  const int64_t su = (int64_t)binom_u(num_sites, n_up), sd = (int64_t)binom_u(num_sites, n_dn);
+  const int64_t tu = (int64_t)binom_u(num_sites, nu_t), td = (int64_t)binom_u(num_sites, nd_t);
+  const int jw = sign_mode == ED_SIGN_JORDAN_WIGNER;
+  for (int64_t v = 0; v < nvec; ++v) {
+    const double* xv = x + v * su * sd;
+    double* yv = y + v * tu * td;
+    if (sigma == ED_UP) {
+      const unsigned g = (unsigned)std::min<int64_t>(tu, 65535);
+      hipLaunchKernelGGL(k_ladder_up, dim3(g), dim3(256), 0, s, num_sites, nu_t, pos, create, jw, xv, yv, tu, td);
+    } else {
+      const int64_t rpb = 64;
+      const dim3 g((unsigned)((td + 255) / 256), (unsigned)((su + rpb - 1) / rpb));
+      hipLaunchKernelGGL(k_ladder_dn, g, dim3(256), 0, s, num_sites, nd_t, pos, create, jw, xv, yv, su, sd, td, rpb);
+    }
+  }
+  ED_HIP(hipGetLastError());
+  return ED_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Continued fraction and pole sums (complex FP64, one lane per frequency).
+// ---------------------------------------------------------------------------
+struct cplx {
+  double re, im;
+};
+__device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
+  // Smith's algorithm
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double r = b.im / b.re, den = b.re + b.im * r;
+    return {(a.re + a.im * r) / den, (a.im - a.re * r) / den};
+  } else {
+    const double r = b.re / b.im, den = b.re * r + b.im;
+    return {(a.re * r + a.im) / den, (a.im * r - a.re) / den};
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cf_eval(const double* __restrict__ alpha, const double* __restrict__ b2,
+                                                 int m, double norm2, double shift, int sign,
+                                                 const double* __restrict__ z, int64_t nz, double* __restrict__ g,
+                                                 int accumulate) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nz) return;
+  const cplx w = {sign * z[2 * k] + shift, sign * z[2 * k + 1]};
+  cplx t = {0.0, 0.0};
+  for (int j = m - 1; j >= 1; --j) t = cdiv({b2[j - 1], 0.0}, {w.re - alpha[j] - t.re, w.im - t.im});
+  cplx r = cdiv({sign * norm2, 0.0}, {w.re - alpha[0] - t.re, w.im - t.im});
+  if (accumulate) {
+    g[2 * k] += r.re;
+    g[2 * k + 1] += r.im;
+  } else {
+    g[2 * k] = r.re;
+    g[2 * k + 1] = r.im;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pole_sum(const double* __restrict__ w, const double* __restrict__ p,
+                                                  int64_t m, const double* __restrict__ z, int64_t nz,
+                                                  double* __restrict__ g, int accumulate) {
+  __shared__ double sw[1024], sp[1024];
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  cplx acc = {0.0, 0.0};
+  const cplx zz = k < nz ? cplx{z[2 * k], z[2 * k + 1]} : cplx{0.0, 1.0};
+  for (int64_t base = 0; base < m; base += 1024) {
+    const int64_t cnt = m - base < 1024 ? m - base : 1024;
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+      sw[i] = w[base + i];
+      sp[i] = p[base + i];
+    }
+    __syncthreads();
+    for (int64_t i = 0; i < cnt; ++i) {
+      const cplx q = cdiv({sw[i], 0.0}, {zz.re - sp[i], zz.im});
+      acc.re += q.re;
+      acc.im += q.im;
+    }
+  }
+  if (k < nz) {
+    if (accumulate) {
+      g[2 * k] += acc.re;
+      g[2 * k + 1] += acc.im;
+    } else {
+      g[2 * k] = acc.re;
+      g[2 * k + 1] = acc.im;
+    }
+  }
+}
+
+// Finite-T pair sum, stage 1: block (z tile, m group) -> partial[mg][z].
+constexpr int kPairGroup = 16;
+__global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ ov, const double* __restrict__ en,
+                                               int64_t n, const double* __restrict__ em, int64_t m,
+                                               double beta, double emin, const double* __restrict__ z,
+                                               int64_t nz, double* __restrict__ partial) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t mg = blockIdx.y;
+  cplx acc = {0.0, 0.0};
+  if (k < nz) {
+    const cplx zz = {z[2 * k], z[2 * k + 1]};
+    const bool fin = isfinite(beta);
+    for (int64_t mi = mg * kPairGroup; mi < (mg + 1) * kPairGroup && mi < m; ++mi) {
+      const double e_m = em[mi];
+      const double xm = fin ? exp(-beta * (e_m - emin)) : 1.0;
+      for (int64_t ni = 0; ni < n; ++ni) {
+        const double o = ov[mi * n + ni];
+        const double wgt = o * o * ((fin ? exp(-beta * (en[ni] - emin)) : 1.0) + xm);
+        const cplx q = cdiv({wgt, 0.0}, {zz.re - e_m + en[ni], zz.im});
+        acc.re += q.re;
+        acc.im += q.im;
+      }
+    }
+    partial[2 * (mg * nz + k)] = acc.re;
+    partial[2 * (mg * nz + k) + 1] = acc.im;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pairs_reduce(const double* __restrict__ partial, int64_t ng, int64_t nz,
+                                                      double* __restrict__ g) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nz) return;
+  double re = 0.0, im = 0.0;
+  for (int64_t gi = 0; gi < ng; ++gi) {
+    re += partial[2 * (gi * nz + k)];
+    im += partial[2 * (gi * nz + k) + 1];
+  }
+  g[2 * k] += re;
+  g[2 * k + 1] += im;
+}
+
+// C[M x N] = A B^T with A: [M][K], B: [N][K] (vectors stored one after another),
+// 64x64 output tiles, 16-wide K slabs staged in LDS, 4x4 outputs per lane.
+__global__ __launch_bounds__(256) void k_gemm_nt(const double* __restrict__ A, const double* __restrict__ B,
+                                                 double* __restrict__ Cm, int64_t M, int64_t N, int64_t K) {
+  __shared__ double As[16][64 + 1];
+  __shared__ double Bs[16][64 + 1];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int64_t m0 = (int64_t)blockIdx.y * 64, n0 = (int64_t)blockIdx.x * 64;
+  double acc[4][4] = {};
+  for (int64_t k0 = 0; k0 < K; k0 += 16) {
+    for (int i = threadIdx.x; i < 16 * 64; i += 256) {
+      const int row = i / 16, kk = i % 16;
+      const int64_t kg = k0 + kk;
+      As[kk][row] = (kg < K && m0 + row < M) ? A[(m0 + row) * K + kg] : 0.0;
+      Bs[kk][row] = (kg < K && n0 + row < N) ? B[(n0 + row) * K + kg] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t r = m0 + ty * 4 + i, c = n0 + tx * 4 + j;
+      if (r < M && c < N) Cm[r * N + c] = acc[i][j];
+    }
+}
+
+// Occupation partials: block (c, k) sums w_k |V_k[i]|^2 over the indices i of
+// chunk c whose up/dn/both spins occupy `pos`.
+__global__ __launch_bounds__(256) void k_occ(const double* __restrict__ V, int64_t D, int64_t num_dn,
+                                             const uint32_t* __restrict__ st_up,
+                                             const uint32_t* __restrict__ st_dn, uint32_t bit, int kind,
+                                             const double* __restrict__ w, int64_t chunk, double* partial) {
+  __shared__ double sh[4];
+  const int64_t c = blockIdx.x, k = blockIdx.y;
+  const double* v = V + k * D;
+  double acc = 0.0;
+  const int64_t i1 = (c + 1) * chunk < D ? (c + 1) * chunk : D;
+  for (int64_t i = c * chunk + threadIdx.x; i < i1; i += blockDim.x) {
+    const int64_t iu = i / num_dn, id = i - iu * num_dn;
+    const bool up = (st_up[iu] & bit) != 0, dn = (st_dn[id] & bit) != 0;
+    const bool sel = kind == 0 ? up : (kind == 1 ? dn : (up && dn));
+    if (sel) acc += v[i] * v[i];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[k * gridDim.x + c] = w[k] * ((sh[0] + sh[1]) + (sh[2] + sh[3]));
+}
+
+__global__ void k_sum_serial(const double* __restrict__ partial, int64_t n, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += partial[i];
+  *out = s;
+}
+
+// rocBLAS handle per device (the solver library's own state).
+static std::mutex g_handle_mu;
+static rocblas_handle g_handles[64] = {};
+
+static int get_handle(rocblas_handle* h) {
+  int dev = 0;
+  ED_HIP(hipGetDevice(&dev));
+  ED_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_handle_mu);
+  if (!g_handles[dev]) {
+    if (rocblas_create_handle(&g_handles[dev]) != rocblas_status_success)
+      return set_error(ED_ERR_HIP, "rocblas_create_handle failed");
+  }
+  *h = g_handles[dev];
+  return ED_OK;
+}
+
+}  // namespace ed
+
+using namespace ed;
+
+extern "C" {
+
+int ed_apply_cdag(int num_sites, int n_up, int n_dn, int pos, int sigma, int sign_mode, const double* x_dev,
+                  double* y_dev, int64_t nvec, void* stream) {
+  clear_error();
+  return ladder(num_sites, n_up, n_dn, pos, sigma, sign_mode, 1, x_dev, y_dev, nvec, as_stream(stream));
+}
+
+int ed_apply_c(int num_sites, int n_up, int n_dn, int pos, int sigma, int sign_mode, const double* x_dev,
+               double* y_dev, int64_t nvec, void* stream) {
+  clear_error();
+  return ladder(num_sites, n_up, n_dn, pos, sigma, sign_mode, 0, x_dev, y_dev, nvec, as_stream(stream));
+}
+
+int ed_cf_eval(const double* alpha, const double* beta, int32_t m, double norm2, double shift, int32_t sign,
+               const double* z_dev, int64_t nz, double* g_dev, int32_t accumulate, void* stream) {
+  clear_error();
+  ED_REQUIRE(m >= 0 && m <= 1000000, "m=%d invalid", m);
+  ED_REQUIRE(sign == 1 || sign == -1, "sign must be +1 or -1");
+  ED_REQUIRE(nz >= 0 && (nz == 0 || (z_dev && g_dev)), "NULL buffer");
+  ED_REQUIRE(m == 0 || alpha, "alpha is NULL");
+  ED_REQUIRE(m <= 1 || beta, "beta is NULL");
+  hipStream_t s = as_stream(stream);
+  if (nz == 0) return ED_OK;
+  if (m == 0 || norm2 == 0.0) {
+    if (!accumulate) ED_HIP(hipMemsetAsync(g_dev, 0, sizeof(double) * 2 * nz, s));
+    return ED_OK;
+  }
+  std::vector<double> h(2 * (size_t)m);
+  for (int j = 0; j < m; ++j) h[j] = alpha[j];
+  for (int j = 0; j + 1 < m; ++j) h[m + j] = beta[j] * beta[j];
+  double* d = nullptr;
+  ED_HIP(hipMalloc(&d, sizeof(double) * 2 * m));
+  int rc = ED_OK;
+  if (hipMemcpyAsync(d, h.data(), sizeof(double) * 2 * m, hipMemcpyHostToDevice, s) != hipSuccess) {
+    rc = set_error(ED_ERR_HIP, "copying CF coefficients failed");
+  } else {
+    hipLaunchKernelGGL(k_cf_eval, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, d, d + m, m, norm2, shift,
+                       sign, z_dev, nz, g_dev, accumulate);
+    if (hipGetLastError() != hipSuccess) rc = set_error(ED_ERR_HIP, "k_cf_eval launch failed");
+  }
+  if (hipStreamSynchronize(s) != hipSuccess && rc == ED_OK) rc = set_error(ED_ERR_HIP, "stream sync failed");
+  (void)hipFree(d);
+  return rc;
+}
+
+int ed_pole_sum(const double* w_dev, const double* p_dev, int64_t m, const double* z_dev, int64_t nz,
+                double* g_dev, int32_t accumulate, void* stream) {
+  clear_error();
+  ED_REQUIRE(m >= 0 && nz >= 0, "negative size");
+  ED_REQUIRE(nz == 0 || (z_dev && g_dev), "NULL buffer");
+  ED_REQUIRE(m == 0 || (w_dev && p_dev), "NULL buffer");
+  if (nz == 0) return ED_OK;
+  hipLaunchKernelGGL(k_pole_sum, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, as_stream(stream), w_dev, p_dev,
+                     m, z_dev, nz, g_dev, accumulate);
+  ED_HIP(hipGetLastError());
+  return ED_OK;
+}
+
+int ed_lehmann_pairs(const double* overlap_dev, const double* evals_dev, int64_t n, const double* evals_p1_dev,
+                     int64_t m, double beta, double emin, const double* z_dev, int64_t nz, double* g_dev,
+                     void* stream) {
+  clear_error();
+  ED_REQUIRE(n >= 0 && m >= 0 && nz >= 0, "negative size");
+  if (n == 0 || m == 0 || nz == 0) return ED_OK;
+  ED_REQUIRE(overlap_dev && evals_dev && evals_p1_dev && z_dev && g_dev, "NULL buffer");
+  ED_REQUIRE(!std::isnan(beta) && std::isfinite(emin), "beta/emin invalid");
+  hipStream_t s = as_stream(stream);
+  const int64_t ng = (m + kPairGroup - 1) / kPairGroup;
+  double* part = nullptr;
+  ED_HIP(hipMalloc(&part, sizeof(double) * 2 * ng * nz));
+  int rc = ED_OK;
+  hipLaunchKernelGGL(k_pairs, dim3((unsigned)((nz + 255) / 256), (unsigned)ng), dim3(256), 0, s, overlap_dev,
+                     evals_dev, n, evals_p1_dev, m, beta, emin, z_dev, nz, part);
+  hipLaunchKernelGGL(k_pairs_reduce, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, part, ng, nz, g_dev);
+  if (hipGetLastError() != hipSuccess) rc = set_error(ED_ERR_HIP, "pair-sum launch failed");
+  if (hipStreamSynchronize(s) != hipSuccess && rc == ED_OK) rc = set_error(ED_ERR_HIP, "stream sync failed");
+  (void)hipFree(part);
+  return rc;
+}
+
+int ed_gemm_nt(const double* a_dev, const double* b_dev, double* c_dev, int64_t m, int64_t n, int64_t k,
+               void* stream) {
+  clear_error();
+  ED_REQUIRE(k >= 0 && m >= 0 && n >= 0, "negative size");
+  if (m == 0 || n == 0) return ED_OK;
+  ED_REQUIRE(a_dev && b_dev && c_dev, "NULL buffer");
+  const dim3 g((unsigned)((n + 63) / 64), (unsigned)((m + 63) / 64));
+  hipLaunchKernelGGL(k_gemm_nt, g, dim3(256), 0, as_stream(stream), a_dev, b_dev, c_dev, m, n, k);
+  ED_HIP(hipGetLastError());
+  return ED_OK;
+}
+
+int ed_occupation(const double* vecs_dev, int64_t nvec, int num_sites, int n_up, int n_dn, int pos, int kind,
+                  const double* w_dev, double* out_dev, void* stream) {
+  clear_error();
+  ED_REQUIRE(num_sites >= 1 && num_sites <= ED_MAX_SITES, "num_sites=%d invalid", num_sites);
+  ED_REQUIRE(n_up >= 0 && n_up <= num_sites && n_dn >= 0 && n_dn <= num_sites, "sector invalid");
+  ED_REQUIRE(pos >= 0 && pos < num_sites, "pos=%d invalid", pos);
+  ED_REQUIRE(kind >= 0 && kind <= 2, "kind must be 0 (up), 1 (dn) or 2 (double)");
+  ED_REQUIRE(nvec >= 0 && nvec <= 65535, "nvec=%lld outside [0, 65535]", (long long)nvec);
+  ED_REQUIRE(out_dev && (nvec == 0 || (vecs_dev && w_dev)), "NULL buffer");
+  hipStream_t s = as_stream(stream);
+  const int64_t nu = (int64_t)binom_u(num_sites, n_up), nd = (int64_t)binom_u(num_sites, n_dn);
+  const int64_t D = nu * nd;
+  const int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>((D + 4095) / 4096, 1024));
+  const int64_t chunk = (D + nchunk - 1) / nchunk;
+  uint32_t *su = nullptr, *sd = nullptr;
+  double* part = nullptr;
+  int rc = ED_OK;
+  if (hipMalloc(&su, sizeof(uint32_t) * std::max<int64_t>(nu, 1)) != hipSuccess ||
+      hipMalloc(&sd, sizeof(uint32_t) * std::max<int64_t>(nd, 1)) != hipSuccess ||
+      hipMalloc(&part, sizeof(double) * std::max<int64_t>(nchunk * nvec, 1)) != hipSuccess) {
+    rc = set_error(ED_ERR_NOMEM, "hipMalloc failed");
+  } else {
+    hipLaunchKernelGGL(k_unrank_states, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, s, su, nu, num_sites, n_up);
+    hipLaunchKernelGGL(k_unrank_states, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s, sd, nd, num_sites, n_dn);
+    if (nvec > 0)
+      hipLaunchKernelGGL(k_occ, dim3((unsigned)nchunk, (unsigned)nvec), dim3(256), 0, s, vecs_dev, D, nd, su, sd,
+                         1u << pos, kind, w_dev, chunk, part);
+    hipLaunchKernelGGL(k_sum_serial, dim3(1), dim3(64), 0, s, part, nchunk * nvec, out_dev);
+    if (hipGetLastError() != hipSuccess) rc = set_error(ED_ERR_HIP, "occupation launch failed");
+    if (hipStreamSynchronize(s) != hipSuccess && rc == ED_OK) rc = set_error(ED_ERR_HIP, "stream sync failed");
+  }
+  if (su) (void)hipFree(su);
+  if (sd) (void)hipFree(sd);
+  if (part) (void)hipFree(part);
+  return rc;
+}
+
+int ed_dense_eigh(double* a_dev, int64_t n, double* w_dev, void* stream) {
+  clear_error();
+  ED_REQUIRE(n >= 0 && n <= 65536, "n=%lld invalid", (long long)n);
+  if (n == 0) return ED_OK;
+  ED_REQUIRE(a_dev && w_dev, "NULL buffer");
+  rocblas_handle h;
+  ED_TRY(get_handle(&h));
+  hipStream_t s = as_stream(stream);
+  if (rocblas_set_stream(h, s) != rocblas_status_success) return set_error(ED_ERR_HIP, "rocblas_set_stream failed");
+  double* e = nullptr;
+  rocblas_int* info = nullptr;
+  ED_HIP(hipMalloc(&e, sizeof(double) * n));
+  if (hipMalloc(&info, sizeof(rocblas_int)) != hipSuccess) {
+    (void)hipFree(e);
+    return set_error(ED_ERR_NOMEM, "hipMalloc failed");
+  }
+  int rc = ED_OK;
+  const rocblas_status st = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, a_dev,
+                                             (rocblas_int)n, w_dev, e, info);
+  rocblas_int hinfo = 0;
+  if (st != rocblas_status_success) {
+    rc = set_error(ED_ERR_HIP, "rocsolver_dsyevd failed with status %d", (int)st);
+  } else if (hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+             hipStreamSynchronize(s) != hipSuccess) {
+    rc = set_error(ED_ERR_HIP, "reading rocsolver info failed");
+  } else if (hinfo != 0) {
+    rc = set_error(ED_ERR_NUMERIC, "rocsolver_dsyevd did not converge (info=%d)", (int)hinfo);
+  }
+  (void)hipFree(e);
+  (void)hipFree(info);
+  return rc;
+}
+
+}  // extern "C"

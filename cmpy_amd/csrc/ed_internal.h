@@ -1,0 +1,187 @@
+// Internal declarations shared by the libcmpy_amd translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/cmpy_amd.h"
+
+namespace ed {
+
+// ---------------------------------------------------------------------------
+// Errors: every C entry point returns a status and records a thread-local
+// message; nothing throws across the ABI.
+// ---------------------------------------------------------------------------
+int set_error(int code, const char* fmt, ...);
+void clear_error();
+
+#define ED_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return ::ed::set_error(ED_ERR_HIP, "%s failed: %s (%s:%d)", #call,          \
+                             hipGetErrorString(e_), __FILE__, __LINE__);          \
+  } while (0)
+
+#define ED_TRY(call)              \
+  do {                            \
+    int s_ = (call);              \
+    if (s_ != ED_OK) return s_;   \
+  } while (0)
+
+#define ED_REQUIRE(cond, ...)                                          \
+  do {                                                                 \
+    if (!(cond)) return ::ed::set_error(ED_ERR_INVALID, __VA_ARGS__);  \
+  } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---------------------------------------------------------------------------
+// Binomial coefficients, usable on host and device (constant memory).
+// ---------------------------------------------------------------------------
+constexpr int kMaxSites = 32;
+struct BinomTable {
+  uint64_t v[kMaxSites + 1][kMaxSites + 1];
+};
+constexpr BinomTable make_binom() {
+  BinomTable t{};
+  for (int n = 0; n <= kMaxSites; ++n) {
+    t.v[n][0] = 1;
+    for (int k = 1; k <= n; ++k) t.v[n][k] = t.v[n - 1][k - 1] + (k <= n - 1 ? t.v[n - 1][k] : 0);
+  }
+  return t;
+}
+constexpr BinomTable kHostBinom = make_binom();
+static __constant__ BinomTable kDevBinom = make_binom();
+
+__host__ __device__ inline uint64_t binom_u(int n, int k) {
+  if (n < 0 || k < 0 || k > n || n > kMaxSites) return 0;
+#ifdef __HIP_DEVICE_COMPILE__
+  return kDevBinom.v[n][k];
+#else
+  return kHostBinom.v[n][k];
+#endif
+}
+
+// Combinadic rank of a bit pattern: r(s) = sum_k C(p_k, k), k 1-based over the
+// set bits p_1 < p_2 < ...; equals the position of s in the ascending list of
+// states with the same popcount.
+__host__ __device__ inline int64_t comb_rank(uint32_t s) {
+  int64_t r = 0;
+  int k = 0;
+  while (s) {
+    const int p = __builtin_ctz(s);
+    ++k;
+    r += (int64_t)binom_u(p, k);
+    s &= s - 1;
+  }
+  return r;
+}
+
+// Inverse of comb_rank within the (num_sites, n) sector: greedy from the top.
+__host__ __device__ inline uint32_t comb_unrank(int64_t r, int num_sites, int n) {
+  uint32_t s = 0;
+  int p = num_sites - 1;
+  for (int k = n; k >= 1; --k) {
+    while (p >= 0 && (int64_t)binom_u(p, k) > r) --p;
+    s |= 1u << p;
+    r -= (int64_t)binom_u(p, k);
+    --p;
+  }
+  return s;
+}
+
+__host__ __device__ inline int popc32(uint32_t v) { return __builtin_popcount(v); }
+
+// Hopping sign (-1)^popcount(state & bits strictly between site1 and site2,
+// restricted to the low `width` bits): cmpy/operators.py:253-273, 425-433.
+__host__ __device__ inline int hop_sign(uint32_t state, int width, int site1, int site2) {
+  if (width <= 0) return 1;
+  uint32_t between = 0;
+  if (site2 - site1 > 1) between = ((site2 >= 32 ? 0u : (1u << site2)) - (1u << (site1 + 1)));
+  const uint32_t low = width >= 32 ? 0xffffffffu : ((1u << width) - 1u);
+  return (popc32(state & between & low) & 1) ? -1 : 1;
+}
+
+// ---------------------------------------------------------------------------
+// Sector operator handle.
+// ---------------------------------------------------------------------------
+struct Tables {
+  // ELL hop table for one spin species, valid entries first in bond order.
+  int32_t width = 0;         // slots per state (= number of bonds)
+  int32_t* cnt = nullptr;    // [D]
+  int32_t* col_rm = nullptr; // [D][width]
+  double* val_rm = nullptr;  // [D][width]
+  int32_t* col_cm = nullptr; // [width][D]
+  double* val_cm = nullptr;  // [width][D]
+};
+
+}  // namespace ed
+
+struct ed_op {
+  int device = 0;
+  int num_sites = 0, n_up = 0, n_dn = 0, num_bonds = 0, width = 0;
+  int64_t num_up = 0, num_dn = 0, size = 0;
+  std::vector<int32_t> bonds;
+  std::vector<double> hop, eps, u;
+  int u_mode = 0;        // 0: no interaction, 1: uniform (popcount), 2: per site
+  double u_uniform = 0.0;
+  uint32_t u_mask = 0;
+  // device data
+  uint32_t* st_up = nullptr;
+  uint32_t* st_dn = nullptr;
+  double* d_up = nullptr;
+  double* d_dn = nullptr;
+  double* u_site = nullptr;  // [num_sites]
+  ed::Tables up, dn;
+  int max_hops_up = 0, max_hops_dn = 0;
+};
+
+namespace ed {
+
+// Plain matvec arguments (device pointers from the op).
+struct HvArgs {
+  int64_t num_up, num_dn;
+  const uint32_t* st_up;
+  const uint32_t* st_dn;
+  const double* d_up;
+  const double* d_dn;
+  const int32_t* up_cnt;
+  const int32_t* up_col;  // row-major [num_up][up_w]
+  const double* up_val;
+  int32_t up_w;
+  const int32_t* dn_cnt;
+  const int32_t* dn_col;  // column-major [dn_w][num_dn]
+  const double* dn_val;
+  int32_t dn_w;
+  int32_t u_mode;
+  double u_uniform;
+  uint32_t u_mask;
+  const double* u_site;
+};
+
+HvArgs make_hv_args(const ed_op* op);
+
+// Lanczos epilogue: y = s_j * H x - c_j * xprev, partial <y, x>.
+struct HvEpilogue {
+  const double* xprev;  // may be null
+  const double* sx;     // device scalar: scale of x (null = 1)
+  const double* cprev;  // device scalar: coefficient of xprev (null = 0)
+  double* partial;      // per-block partial dots (null = none)
+};
+
+// Launch helpers implemented in ed_matvec.hip.
+int launch_matvec(const ed_op* op, const double* x, double* y, const HvEpilogue& epi,
+                  hipStream_t stream, int* num_partials);
+int matvec_num_partials(const ed_op* op);
+
+// Deterministic reductions (ed_lanczos.hip).
+int launch_reduce(const double* partial, int n, double* out, int op_kind, hipStream_t stream);
+
+}  // namespace ed

@@ -1,0 +1,761 @@
+// FP64 Lanczos on the device.
+//
+// Replaces the ARPACK ground-state call of compute_groundstate
+// (cmpy/exactdiag.py:24-43, one sector) and the unnormalised coefficient
+// recurrence iter_lanczos_coeffs / lanczos_ground_state (cmpy/exactdiag.py:324-375).
+// Here the recurrence is normalised, does one H.v per step, starts from a seeded
+// vector and keeps every scalar on the device, so a whole solve runs without a
+// host round trip except at convergence checks.
+//
+// Stored vectors are unnormalised: v_j = s_j u_j with s_j = 1/beta_j kept on the
+// device.  Step j:
+//   w      = s_j H u_j - beta_j s_{j-1} u_{j-1}         (fused into the H.v sweep,
+//   alpha_j = s_j <w, u_j>                               with the block partials)
+//   w     -= alpha_j s_j u_j,  beta_{j+1} = |w|          (one vector pass)
+//   u_{j+1} = w,  s_{j+1} = 1/beta_{j+1}
+// All reductions are two-stage with a fixed summation order, so repeated runs
+// (and the eigenvector's second pass) reproduce every vector bit for bit.
+#include "ed_internal.h"
+
+#include <algorithm>
+#include <chrono>
+
+namespace ed {
+
+enum RedKind { RED_NORM0 = 0, RED_ALPHA = 1, RED_BETA = 2, RED_SUM_TO = 3, RED_SQRT_TO = 4 };
+
+struct LzCoef {
+  double* alpha;  // [M]
+  double* beta;   // [M+1]
+  double* scale;  // [M+1]
+  double* cprev;  // [M+1]  beta_j * s_{j-1}
+  double* misc;   // [16]   0: breakdown flag, 1: one, 2..: scratch scalars
+};
+
+constexpr int kRedThreads = 1024;
+constexpr int kVecThreads = 256;
+
+__global__ __launch_bounds__(kRedThreads) void k_reduce(const double* __restrict__ partial, int n,
+                                                        int kind, LzCoef c, int j, double* out) {
+  __shared__ double sh[kRedThreads / 64];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += kRedThreads) v += partial[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double S = 0.0;
+  for (int w = 0; w < kRedThreads / 64; ++w) S += sh[w];
+  switch (kind) {
+    case RED_NORM0: {
+      if (!isfinite(S)) c.misc[3] = 1.0;
+      const double b = isfinite(S) ? sqrt(fmax(S, 0.0)) : 0.0;
+      c.beta[0] = b;
+      c.scale[0] = b > 0.0 ? 1.0 / b : 0.0;
+      c.cprev[0] = 0.0;
+      c.misc[0] = b > 0.0 ? 0.0 : 1.0;
+      break;
+    }
+    case RED_ALPHA:
+      c.alpha[j] = c.scale[j] * S;
+      break;
+    case RED_BETA: {
+      double b = sqrt(fmax(S, 0.0));
+      const double ref = fabs(c.alpha[j]) + (j > 0 ? c.beta[j] : 0.0);
+      if (c.misc[0] != 0.0 || !(b > 1e-12 * ref) || !isfinite(b)) {
+        if (!isfinite(b)) c.misc[3] = 1.0;
+        b = 0.0;
+        c.misc[0] = 1.0;
+      }
+      c.beta[j + 1] = b;
+      c.scale[j + 1] = b > 0.0 ? 1.0 / b : 0.0;
+      c.cprev[j + 1] = b * c.scale[j];
+      break;
+    }
+    case RED_SUM_TO:
+      *out = S;
+      break;
+    case RED_SQRT_TO:
+      *out = sqrt(fmax(S, 0.0));
+      break;
+  }
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += sh[w];
+  return r;
+}
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// u = v0 (or seeded uniform [-1, 1)), partial |u|^2
+__global__ __launch_bounds__(kVecThreads) void k_start(double* __restrict__ u, const double* __restrict__ v0,
+                                                       int64_t D, uint64_t seed, double* partial) {
+  __shared__ double sh[kVecThreads / 64];
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < D; k += stride) {
+    double v;
+    if (v0)
+      v = v0[k];
+    else
+      v = (double)(splitmix64(seed ^ (0xD1B54A32D192ED03ull * (uint64_t)(k + 1))) >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+    u[k] = v;
+    acc += v * v;
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// w -= alpha_j s_j u;  psi += yj s_j u (optional);  partial |w|^2 (optional)
+__global__ __launch_bounds__(kVecThreads) void k_update(double* __restrict__ w, const double* __restrict__ u,
+                                                        int64_t D, const double* alpha, const double* scale,
+                                                        double* __restrict__ psi, double yj, double* partial) {
+  __shared__ double sh[kVecThreads / 64];
+  const double a = (*alpha) * (*scale);
+  const double ys = yj * (*scale);
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < D; k += stride) {
+    const double uk = u[k];
+    const double wk = w[k] - a * uk;
+    w[k] = wk;
+    acc += wk * wk;
+    if (psi) psi[k] += ys * uk;
+  }
+  if (partial) {
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+}
+
+// dst = scale * src (scale from device), partial |dst|^2 optional
+__global__ __launch_bounds__(kVecThreads) void k_scale(double* dst, const double* src,
+                                                       int64_t D, const double* scale, int invert,
+                                                       double* partial) {
+  __shared__ double sh[kVecThreads / 64];
+  double s = *scale;
+  if (invert) s = s > 0.0 ? 1.0 / s : 0.0;
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < D; k += stride) {
+    const double v = s * src[k];
+    dst[k] = v;
+    acc += v * v;
+  }
+  if (partial) {
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  }
+}
+
+// partial |w - e psi|^2
+__global__ __launch_bounds__(kVecThreads) void k_resid(const double* __restrict__ w, const double* __restrict__ psi,
+                                                       int64_t D, double e, double* partial) {
+  __shared__ double sh[kVecThreads / 64];
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < D; k += stride) {
+    const double r = w[k] - e * psi[k];
+    acc += r * r;
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Block (c, i): partial of <V_i, w> over chunk c.
+__global__ __launch_bounds__(kVecThreads) void k_multidot(const double* __restrict__ V, int64_t D,
+                                                          const double* __restrict__ w, int64_t chunk,
+                                                          double* partial, int nchunks) {
+  __shared__ double sh[kVecThreads / 64];
+  const int c = blockIdx.x, i = blockIdx.y;
+  const double* vi = V + (int64_t)i * D;
+  const int64_t k0 = (int64_t)c * chunk, k1 = std::min<int64_t>(D, k0 + chunk);
+  double acc = 0.0;
+  for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) acc += vi[k] * w[k];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[(int64_t)i * nchunks + c] = s;
+}
+
+__global__ __launch_bounds__(kVecThreads) void k_reduce_rows(const double* __restrict__ partial, int nchunks,
+                                                             double* out) {
+  __shared__ double sh[kVecThreads / 64];
+  double acc = 0.0;
+  for (int c = threadIdx.x; c < nchunks; c += blockDim.x) acc += partial[(int64_t)blockIdx.x * nchunks + c];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// w -= sum_i coef_i V_i   (sign = -1)   or   w = sum_i coef_i V_i (sign = 0)
+__global__ __launch_bounds__(kVecThreads) void k_axpy_multi(const double* __restrict__ V, int nv, int64_t D,
+                                                            const double* __restrict__ coef,
+                                                            double* __restrict__ w, int assign) {
+  extern __shared__ double cs[];
+  for (int i = threadIdx.x; i < nv; i += blockDim.x) cs[i] = coef[i];
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < D; k += stride) {
+    double acc = 0.0;
+    for (int i = 0; i < nv; ++i) acc += cs[i] * V[(int64_t)i * D + k];
+    w[k] = assign ? acc : w[k] - acc;
+  }
+}
+
+static int vec_blocks(int64_t D) { return (int)std::max<int64_t>(1, std::min<int64_t>((D + 255) / 256, 2048)); }
+
+int launch_reduce(const double* partial, int n, double* out, int kind, hipStream_t stream) {
+  LzCoef c{};
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, stream, partial, n, kind, c, 0, out);
+  ED_HIP(hipGetLastError());
+  return ED_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Host tridiagonal solver: lowest eigenpair of T (diag a[0..m-1], off b[0..m-2])
+// by Sturm bisection and inverse iteration.
+// ---------------------------------------------------------------------------
+static int sturm_count(const double* a, const double* b, int m, double x) {
+  int cnt = 0;
+  double q = a[0] - x;
+  const double tiny = 1e-300;
+  if (q < 0) ++cnt;
+  for (int i = 1; i < m; ++i) {
+    if (q == 0.0) q = tiny;
+    q = a[i] - x - b[i - 1] * b[i - 1] / q;
+    if (q < 0) ++cnt;
+  }
+  return cnt;
+}
+
+double tri_lowest_value(const double* a, const double* b, int m) {
+  double lo = a[0], hi = a[0];
+  for (int i = 0; i < m; ++i) {
+    const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < m - 1 ? fabs(b[i]) : 0.0);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+  }
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (sturm_count(a, b, m, mid) >= 1)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// Solve (T - lam I) x = r with partial pivoting (tridiagonal Gaussian elimination).
+static void tri_solve_shifted(const double* a, const double* b, int m, double lam, std::vector<double>& x) {
+  std::vector<double> d(m), du(m, 0.0), du2(m, 0.0), dl(m, 0.0);
+  for (int i = 0; i < m; ++i) d[i] = a[i] - lam;
+  for (int i = 0; i < m - 1; ++i) {
+    du[i] = b[i];
+    dl[i] = b[i];
+  }
+  std::vector<int> piv(m, 0);
+  const double tiny = 1e-300;
+  for (int i = 0; i < m - 1; ++i) {
+    if (fabs(d[i]) >= fabs(dl[i])) {
+      if (d[i] == 0.0) d[i] = tiny;
+      const double f = dl[i] / d[i];
+      d[i + 1] -= f * du[i];
+      if (i < m - 2) du2[i] = 0.0;
+      dl[i] = f;
+      piv[i] = 0;
+    } else {
+      const double f = d[i] / dl[i];
+      d[i] = dl[i];
+      const double tmp = d[i + 1];
+      d[i + 1] = du[i] - f * tmp;
+      if (i < m - 2) {
+        du2[i] = du[i + 1];
+        du[i + 1] = -f * du2[i];
+      }
+      du[i] = tmp;
+      dl[i] = f;
+      piv[i] = 1;
+    }
+  }
+  if (d[m - 1] == 0.0) d[m - 1] = tiny;
+  // forward
+  for (int i = 0; i < m - 1; ++i) {
+    if (piv[i] == 0) {
+      x[i + 1] -= dl[i] * x[i];
+    } else {
+      const double t = x[i];
+      x[i] = x[i + 1];
+      x[i + 1] = t - dl[i] * x[i + 1];
+    }
+  }
+  // backward
+  x[m - 1] /= d[m - 1];
+  if (m > 1) x[m - 2] = (x[m - 2] - du[m - 2] * x[m - 1]) / d[m - 2];
+  for (int i = m - 3; i >= 0; --i) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / d[i];
+}
+
+void tri_lowest_vector(const double* a, const double* b, int m, double lam, std::vector<double>& y) {
+  y.assign(m, 1.0);
+  if (m == 1) return;
+  double scale = 0.0;
+  for (int i = 0; i < m; ++i) scale = std::max(scale, fabs(a[i]) + (i < m - 1 ? fabs(b[i]) : 0.0));
+  const double shift = lam - 1e-13 * std::max(scale, 1.0);
+  for (int i = 0; i < m; ++i) y[i] = 1.0 + 0.01 * (double)((i * 7919) % 13) / 13.0;
+  for (int it = 0; it < 3; ++it) {
+    tri_solve_shifted(a, b, m, shift, y);
+    double n = 0.0;
+    for (double v : y) n += v * v;
+    n = sqrt(n);
+    for (double& v : y) v /= n;
+  }
+  if (y[0] < 0)
+    for (double& v : y) v = -v;
+}
+
+// ---------------------------------------------------------------------------
+// Workspace layout
+// ---------------------------------------------------------------------------
+struct LzLayout {
+  int64_t coef_off, part_off, vec_off, total;
+  int64_t D;
+  int M;
+  int nvec;
+  int npart;
+};
+
+static int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
+
+static LzLayout lz_layout(const ed_op* op, int M, int reorth, int want_vector, int basis_vectors) {
+  LzLayout L{};
+  L.D = op->size;
+  L.M = M;
+  const int P1 = matvec_num_partials(op);
+  const int P2 = vec_blocks(L.D);
+  int64_t chunks = std::min<int64_t>((L.D + 4095) / 4096, 256);
+  int64_t P3 = reorth ? (int64_t)(M + 1) * chunks : 0;
+  L.npart = (int)std::max<int64_t>({(int64_t)P1, (int64_t)P2, P3, (int64_t)(M + 2)});
+  L.nvec = reorth ? (M + 2) : (3 + (want_vector ? 1 : 0) + basis_vectors);
+  L.coef_off = 0;
+  L.part_off = align256(L.coef_off + sizeof(double) * (4 * (int64_t)M + 4 + 16));
+  L.vec_off = align256(L.part_off + sizeof(double) * (int64_t)(L.npart + M + 2));
+  L.total = L.vec_off + align256(sizeof(double) * L.D) * L.nvec;
+  return L;
+}
+
+static LzCoef coef_ptrs(char* base, int M) {
+  double* c = reinterpret_cast<double*>(base);
+  LzCoef k;
+  k.alpha = c;
+  k.beta = c + M;
+  k.scale = c + 2 * M + 1;
+  k.cprev = c + 3 * M + 2;
+  k.misc = c + 4 * M + 3;
+  return k;
+}
+
+static double* vec_ptr(char* base, const LzLayout& L, int i) {
+  return reinterpret_cast<double*>(base + L.vec_off + align256(sizeof(double) * L.D) * i);
+}
+
+__global__ void k_init_misc(double* misc) {
+  if (threadIdx.x < 16) misc[threadIdx.x] = 0.0;
+  if (threadIdx.x == 1) misc[1] = 1.0;
+}
+
+struct EventTimer {
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  bool on = true;
+  ~EventTimer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+  hipEvent_t next() {
+    if (used == ev.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) {
+        on = false;
+        return nullptr;
+      }
+      ev.push_back(e);
+    }
+    return ev[used++];
+  }
+  double total_ms() {
+    double t = 0.0;
+    for (size_t i = 0; i + 1 < used; i += 2) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) t += ms;
+    }
+    return t;
+  }
+};
+
+static int run_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, double* u_prev, double* u_cur,
+                    double* w, int j, hipStream_t s, EventTimer* tm, double* psi, double yj, bool reduce) {
+  HvEpilogue e;
+  e.xprev = j > 0 ? u_prev : nullptr;
+  e.sx = c.scale + j;
+  e.cprev = c.cprev + j;
+  e.partial = reduce ? part : nullptr;
+  if (tm && tm->on) {
+    hipEvent_t a = tm->next();
+    if (a) ED_HIP(hipEventRecord(a, s));
+  }
+  int np = 0;
+  if (!reduce) {
+    // keep the epilogue path (sx/cprev) without partials
+    ED_TRY(launch_matvec(op, u_cur, w, e, s, &np));
+  } else {
+    ED_TRY(launch_matvec(op, u_cur, w, e, s, &np));
+  }
+  if (tm && tm->on) {
+    hipEvent_t b = tm->next();
+    if (b) ED_HIP(hipEventRecord(b, s));
+  }
+  if (reduce) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, np, (int)RED_ALPHA, c, j, nullptr);
+  const int vb = vec_blocks(L.D);
+  hipLaunchKernelGGL(k_update, dim3(vb), dim3(kVecThreads), 0, s, w, u_cur, L.D, c.alpha + j, c.scale + j, psi, yj,
+                     reduce ? part : nullptr);
+  if (reduce) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_BETA, c, j, nullptr);
+  ED_HIP(hipGetLastError());
+  return ED_OK;
+}
+
+// Full reorthogonalisation.  V[0..M] holds normalised basis vectors.
+struct ReorthCtx {
+  double* V;
+  double* w;
+  double* coefs;
+  int64_t chunks, chunk;
+};
+
+static ReorthCtx reorth_ctx(char* base, const LzLayout& L, double* part) {
+  ReorthCtx r;
+  r.V = vec_ptr(base, L, 0);
+  r.w = vec_ptr(base, L, L.M + 1);
+  r.coefs = part + L.npart;
+  r.chunks = std::min<int64_t>((L.D + 4095) / 4096, 256);
+  r.chunk = (L.D + r.chunks - 1) / r.chunks;
+  return r;
+}
+
+static int reorth_start(const LzLayout& L, LzCoef c, double* part, const ReorthCtx& r, const double* v0,
+                        uint64_t seed, hipStream_t s) {
+  const int vb = vec_blocks(L.D);
+  hipLaunchKernelGGL(k_start, dim3(vb), dim3(kVecThreads), 0, s, r.w, v0, L.D, seed, part);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_NORM0, c, 0, nullptr);
+  hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, r.V, r.w, L.D, c.scale, 0, nullptr);
+  ED_HIP(hipGetLastError());
+  return ED_OK;
+}
+
+// Step j: w = H v_j - beta_j v_{j-1}; alpha_j = <w, v_j>; w -= alpha_j v_j;
+// two classical Gram-Schmidt passes against v_0..v_j; beta_{j+1} = |w|;
+// v_{j+1} = w / beta_{j+1}.
+static int reorth_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, const ReorthCtx& r, int j,
+                       hipStream_t s, EventTimer* tm) {
+  const int64_t D = L.D;
+  const int vb = vec_blocks(D);
+  double* vj = r.V + (int64_t)j * D;
+  HvEpilogue e;
+  e.xprev = j > 0 ? r.V + (int64_t)(j - 1) * D : nullptr;
+  e.sx = c.misc + 1;
+  e.cprev = c.beta + j;
+  e.partial = part;
+  if (tm && tm->on) {
+    hipEvent_t a = tm->next();
+    if (a) ED_HIP(hipEventRecord(a, s));
+  }
+  int np = 0;
+  ED_TRY(launch_matvec(op, vj, r.w, e, s, &np));
+  if (tm && tm->on) {
+    hipEvent_t b = tm->next();
+    if (b) ED_HIP(hipEventRecord(b, s));
+  }
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, np, (int)RED_SUM_TO, c, j, c.alpha + j);
+  hipLaunchKernelGGL(k_update, dim3(vb), dim3(kVecThreads), 0, s, r.w, vj, D, c.alpha + j, c.misc + 1, nullptr, 0.0,
+                     nullptr);
+  for (int pass = 0; pass < 2; ++pass) {
+    hipLaunchKernelGGL(k_multidot, dim3((unsigned)r.chunks, (unsigned)(j + 1)), dim3(kVecThreads), 0, s, r.V, D, r.w,
+                       r.chunk, part, (int)r.chunks);
+    hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)(j + 1)), dim3(kVecThreads), 0, s, part, (int)r.chunks, r.coefs);
+    hipLaunchKernelGGL(k_axpy_multi, dim3(vb), dim3(kVecThreads), sizeof(double) * (j + 1), s, r.V, j + 1, D,
+                       r.coefs, r.w, 0);
+  }
+  hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, r.w, r.w, D, c.misc + 1, 0, part);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_BETA, c, j, nullptr);
+  hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, r.V + (int64_t)(j + 1) * D, r.w, D, c.scale + j + 1, 0,
+                     nullptr);
+  ED_HIP(hipGetLastError());
+  return ED_OK;
+}
+
+static int validate_cfg(const ed_lanczos_cfg* cfg) {
+  ED_REQUIRE(cfg != nullptr, "cfg is NULL");
+  ED_REQUIRE(cfg->max_iter >= 1 && cfg->max_iter <= 100000, "max_iter=%d outside [1, 100000]", cfg->max_iter);
+  ED_REQUIRE(cfg->check_every >= 1, "check_every must be >= 1");
+  ED_REQUIRE(cfg->tol >= 0.0 && std::isfinite(cfg->tol), "tol must be finite and >= 0");
+  // the reorthogonalisation kernels stage one coefficient per basis vector in LDS
+  ED_REQUIRE(!cfg->reorth || cfg->max_iter <= 4096, "reorth supports max_iter <= 4096 (got %d)", cfg->max_iter);
+  return ED_OK;
+}
+
+}  // namespace ed
+
+using namespace ed;
+
+extern "C" int ed_lanczos_workspace(const ed_op* op, const ed_lanczos_cfg* cfg, int64_t* bytes) {
+  clear_error();
+  ED_REQUIRE(op && bytes, "NULL argument");
+  ED_TRY(validate_cfg(cfg));
+  const int M = std::min<int64_t>(cfg->max_iter, op->size);
+  *bytes = lz_layout(op, std::max(M, 1), cfg->reorth, cfg->want_vector, 0).total;
+  return ED_OK;
+}
+
+extern "C" int ed_lanczos_groundstate(ed_op* op, const ed_lanczos_cfg* cfg, const double* v0_dev,
+                                      double* psi_dev, void* work_dev, int64_t work_bytes,
+                                      ed_lanczos_result* res, void* stream) {
+  clear_error();
+  ED_REQUIRE(op && res && work_dev, "NULL argument");
+  ED_TRY(validate_cfg(cfg));
+  ED_REQUIRE(!cfg->want_vector || psi_dev, "want_vector requires psi_dev");
+  const auto t0 = std::chrono::steady_clock::now();
+  const int M = (int)std::min<int64_t>(cfg->max_iter, op->size);
+  const LzLayout L = lz_layout(op, M, cfg->reorth, cfg->want_vector, 0);
+  ED_REQUIRE(work_bytes >= L.total, "workspace too small: %lld < %lld bytes", (long long)work_bytes,
+             (long long)L.total);
+  hipStream_t s = as_stream(stream);
+  char* base = static_cast<char*>(work_dev);
+  LzCoef c = coef_ptrs(base, M);
+  double* part = reinterpret_cast<double*>(base + L.part_off);
+  const int64_t D = L.D;
+  const int vb = vec_blocks(D);
+  *res = ed_lanczos_result{};
+
+  hipLaunchKernelGGL(k_init_misc, dim3(1), dim3(64), 0, s, c.misc);
+  EventTimer tm;
+  std::vector<double> ha(M), hb(M + 1), y;
+  int m = 0;
+  bool converged = false, breakdown = false;
+  double e0 = 0.0, ritz = 0.0;
+
+  auto check = [&](int steps) -> int {
+    ED_HIP(hipMemcpyAsync(ha.data(), c.alpha, sizeof(double) * steps, hipMemcpyDeviceToHost, s));
+    ED_HIP(hipMemcpyAsync(hb.data(), c.beta, sizeof(double) * (steps + 1), hipMemcpyDeviceToHost, s));
+    double flag[4];
+    ED_HIP(hipMemcpyAsync(flag, c.misc, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
+    ED_HIP(hipStreamSynchronize(s));
+    if (flag[3] != 0.0 || !std::isfinite(ha[steps - 1]))
+      return set_error(ED_ERR_NUMERIC, "non-finite value in the Lanczos recurrence at step %d", steps);
+    if (hb[0] == 0.0) return set_error(ED_ERR_INVALID, "the Lanczos start vector is zero");
+    int mm = steps;
+    breakdown = false;
+    for (int k = 1; k <= steps; ++k) {
+      if (hb[k] == 0.0) {
+        mm = k;
+        breakdown = true;
+        break;
+      }
+    }
+    m = mm;
+    e0 = tri_lowest_value(ha.data(), hb.data() + 1, m);
+    tri_lowest_vector(ha.data(), hb.data() + 1, m, e0, y);
+    ritz = breakdown ? 0.0 : fabs(hb[m]) * fabs(y[m - 1]);
+    return ED_OK;
+  };
+
+  if (!cfg->reorth) {
+    double* u[3] = {vec_ptr(base, L, 0), vec_ptr(base, L, 1), vec_ptr(base, L, 2)};
+    double* psi = cfg->want_vector ? vec_ptr(base, L, 3) : nullptr;
+    hipLaunchKernelGGL(k_start, dim3(vb), dim3(kVecThreads), 0, s, u[0], v0_dev, D, cfg->seed, part);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_NORM0, c, 0, nullptr);
+    ED_HIP(hipGetLastError());
+    int ip = 2, ic = 0, iw = 1;  // u_prev, u_cur, w
+    int steps = 0;
+    for (int j = 0; j < M; ++j) {
+      ED_TRY(run_step(op, L, c, part, u[ip], u[ic], u[iw], j, s, &tm, nullptr, 0.0, true));
+      steps = j + 1;
+      const int old_p = ip;
+      ip = ic;
+      ic = iw;
+      iw = old_p;
+      const bool last = (j + 1 == M);
+      const bool due = !cfg->fixed_iter && steps >= cfg->min_iter && (steps % cfg->check_every == 0);
+      if (due || last) {
+        ED_TRY(check(steps));
+        if (breakdown || ritz <= cfg->tol) {
+          converged = true;
+          break;
+        }
+      }
+    }
+    res->iterations = steps;
+    if (cfg->want_vector) {
+      // second pass: regenerate u_0..u_{m-1} and accumulate psi = sum_j y_j s_j u_j
+      ED_HIP(hipMemsetAsync(psi, 0, sizeof(double) * D, s));
+      hipLaunchKernelGGL(k_start, dim3(vb), dim3(kVecThreads), 0, s, u[0], v0_dev, D, cfg->seed, part);
+      ip = 2;
+      ic = 0;
+      iw = 1;
+      for (int j = 0; j < m; ++j) {
+        if (j + 1 < m) {
+          ED_TRY(run_step(op, L, c, part, u[ip], u[ic], u[iw], j, s, nullptr, psi, y[j], false));
+        } else {
+          // last vector: accumulate only (w is scratch)
+          hipLaunchKernelGGL(k_update, dim3(vb), dim3(kVecThreads), 0, s, u[iw], u[ic], D, c.misc + 2, c.scale + j,
+                             psi, y[j], nullptr);
+        }
+        const int old_p = ip;
+        ip = ic;
+        ic = iw;
+        iw = old_p;
+      }
+      ED_HIP(hipGetLastError());
+    }
+  } else {
+    // full reorthogonalisation: V holds normalised basis vectors
+    const ReorthCtx rc = reorth_ctx(base, L, part);
+    double* V = rc.V;
+    double* coefs = rc.coefs;
+    ED_TRY(reorth_start(L, c, part, rc, v0_dev, cfg->seed, s));
+    int steps = 0;
+    for (int j = 0; j < M; ++j) {
+      ED_TRY(reorth_step(op, L, c, part, rc, j, s, &tm));
+      steps = j + 1;
+      const bool last = (j + 1 == M);
+      const bool due = !cfg->fixed_iter && steps >= cfg->min_iter && (steps % cfg->check_every == 0);
+      if (due || last) {
+        ED_TRY(check(steps));
+        if (breakdown || ritz <= cfg->tol) {
+          converged = true;
+          break;
+        }
+      }
+    }
+    res->iterations = steps;
+    if (cfg->want_vector) {
+      ED_HIP(hipMemcpyAsync(coefs, y.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_axpy_multi, dim3(vb), dim3(kVecThreads), sizeof(double) * m, s, V, m, D, coefs,
+                         vec_ptr(base, L, M + 1), 1);
+      ED_HIP(hipGetLastError());
+    }
+  }
+
+  res->e0 = e0;
+  res->ritz_residual = ritz;
+  res->converged = converged ? 1 : 0;
+  res->breakdown = breakdown ? 1 : 0;
+
+  if (cfg->want_vector) {
+    // normalise psi into psi_dev and compute the explicit residual |H psi - e0 psi|
+    double* psi = cfg->reorth ? vec_ptr(base, L, M + 1) : vec_ptr(base, L, 3);
+    hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, psi_dev, psi, D, c.misc + 1, 0, part);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_SQRT_TO, c, 0, c.misc + 4);
+    hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, psi_dev, psi_dev, D, c.misc + 4, 1, nullptr);
+    double* w = cfg->reorth ? vec_ptr(base, L, 0) : vec_ptr(base, L, 0);
+    HvEpilogue none{nullptr, nullptr, nullptr, nullptr};
+    ED_TRY(launch_matvec(op, psi_dev, w, none, s, nullptr));
+    hipLaunchKernelGGL(k_resid, dim3(vb), dim3(kVecThreads), 0, s, w, psi_dev, D, e0, part);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_SQRT_TO, c, 0, c.misc + 5);
+    ED_HIP(hipGetLastError());
+    double r[2];
+    ED_HIP(hipMemcpyAsync(r, c.misc + 4, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+    ED_HIP(hipStreamSynchronize(s));
+    res->residual = r[1];
+    if (!std::isfinite(r[1])) return set_error(ED_ERR_NUMERIC, "non-finite ground-state residual");
+  } else {
+    res->residual = ritz;
+  }
+  ED_HIP(hipStreamSynchronize(s));
+  res->matvec_ms = tm.total_ms();
+  res->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return ED_OK;
+}
+
+extern "C" int ed_lanczos_coeffs_workspace(const ed_op* op, int32_t nsteps, int32_t reorth, int64_t* bytes) {
+  clear_error();
+  ED_REQUIRE(op && bytes, "NULL argument");
+  ED_REQUIRE(nsteps >= 1 && nsteps <= 100000, "nsteps=%d outside [1, 100000]", nsteps);
+  ED_REQUIRE(!reorth || nsteps <= 4096, "reorth supports nsteps <= 4096 (got %d)", nsteps);
+  *bytes = lz_layout(op, nsteps, reorth ? 1 : 0, 0, 0).total;
+  return ED_OK;
+}
+
+extern "C" int ed_lanczos_coeffs(ed_op* op, const double* v0_dev, int32_t nsteps, int32_t reorth, double* alpha,
+                                 double* beta, double* norm2, int32_t* steps_out, double* basis_dev, void* work_dev,
+                                 int64_t work_bytes, void* stream) {
+  clear_error();
+  ED_REQUIRE(op && v0_dev && alpha && beta && norm2 && steps_out && work_dev, "NULL argument");
+  ED_REQUIRE(nsteps >= 1 && nsteps <= 100000, "nsteps=%d outside [1, 100000]", nsteps);
+  ED_REQUIRE(!reorth || nsteps <= 4096, "reorth supports nsteps <= 4096 (got %d)", nsteps);
+  const int M = nsteps;
+  const LzLayout L = lz_layout(op, M, reorth ? 1 : 0, 0, 0);
+  ED_REQUIRE(work_bytes >= L.total, "workspace too small: %lld < %lld bytes", (long long)work_bytes,
+             (long long)L.total);
+  hipStream_t s = as_stream(stream);
+  char* base = static_cast<char*>(work_dev);
+  LzCoef c = coef_ptrs(base, M);
+  double* part = reinterpret_cast<double*>(base + L.part_off);
+  const int64_t D = L.D;
+  const int vb = vec_blocks(D);
+  hipLaunchKernelGGL(k_init_misc, dim3(1), dim3(64), 0, s, c.misc);
+  if (reorth) {
+    const ReorthCtx rc = reorth_ctx(base, L, part);
+    ED_TRY(reorth_start(L, c, part, rc, v0_dev, 0, s));
+    for (int j = 0; j < M; ++j) ED_TRY(reorth_step(op, L, c, part, rc, j, s, nullptr));
+    if (basis_dev) ED_HIP(hipMemcpyAsync(basis_dev, rc.V, sizeof(double) * D * M, hipMemcpyDeviceToDevice, s));
+  } else {
+    double* u[3] = {vec_ptr(base, L, 0), vec_ptr(base, L, 1), vec_ptr(base, L, 2)};
+    hipLaunchKernelGGL(k_start, dim3(vb), dim3(kVecThreads), 0, s, u[0], v0_dev, D, (uint64_t)0, part);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_NORM0, c, 0, nullptr);
+    ED_HIP(hipGetLastError());
+    int ip = 2, ic = 0, iw = 1;
+    for (int j = 0; j < M; ++j) {
+      if (basis_dev)
+        hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, basis_dev + (int64_t)j * D, u[ic], D,
+                           c.scale + j, 0, nullptr);
+      ED_TRY(run_step(op, L, c, part, u[ip], u[ic], u[iw], j, s, nullptr, nullptr, 0.0, true));
+      const int old_p = ip;
+      ip = ic;
+      ic = iw;
+      iw = old_p;
+    }
+  }
+  std::vector<double> ha(M), hb(M + 1);
+  double flag[4];
+  ED_HIP(hipMemcpyAsync(ha.data(), c.alpha, sizeof(double) * M, hipMemcpyDeviceToHost, s));
+  ED_HIP(hipMemcpyAsync(hb.data(), c.beta, sizeof(double) * (M + 1), hipMemcpyDeviceToHost, s));
+  ED_HIP(hipMemcpyAsync(flag, c.misc, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
+  ED_HIP(hipStreamSynchronize(s));
+  if (flag[3] != 0.0) return set_error(ED_ERR_NUMERIC, "non-finite value in the Lanczos recurrence");
+  for (int k = 0; k < M; ++k)
+    if (!std::isfinite(ha[k])) return set_error(ED_ERR_NUMERIC, "non-finite alpha at step %d", k);
+  *norm2 = hb[0] * hb[0];
+  if (hb[0] == 0.0) {
+    *steps_out = 0;
+    return ED_OK;
+  }
+  int m = M;
+  for (int k = 1; k <= M; ++k)
+    if (hb[k] == 0.0) {
+      m = k;
+      break;
+    }
+  for (int k = 0; k < m; ++k) alpha[k] = ha[k];
+  for (int k = 0; k + 1 < m; ++k) beta[k] = hb[k + 1];
+  *steps_out = m;
+  return ED_OK;
+}

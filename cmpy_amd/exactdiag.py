@@ -1,0 +1,449 @@
+"""Ground state and Green's functions on the GPU (cmpy/exactdiag.py surface).
+
+* ``groundstate``            one sector, device Lanczos (FP64, seeded start,
+                              convergence on the Ritz residual, two-pass vector)
+* ``compute_groundstate``    minimum over all sectors (cmpy/exactdiag.py:24-43)
+* ``solve_sector``           dense eigh of a sector on the device (46-57)
+* ``gf_lehmann``             grand-canonical finite-T Lehmann G with occupations
+                              (``GreensFunctionMeasurement``, 137-245)
+* ``greens_lehmann_t0``      T = 0 fixed-sector Lehmann G (particle + hole),
+                              the ``_accumulate_sum`` formula (110-129) restricted
+                              to the ground state
+* ``greens_cf``              T = 0 continued-fraction G from Lanczos
+                              coefficients (no counterpart in cmpy)
+* ``iter_lanczos_coeffs`` / ``lanczos_coeffs`` / ``lanczos_matrix`` /
+  ``lanczos_ground_state``   the coefficient API of 324-375
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import logging
+from dataclasses import dataclass
+from typing import NamedTuple, Optional, Tuple
+
+import numpy as np
+import scipy.linalg as la
+import torch
+
+from . import _lib
+from .basis import UP, DN, Sector, upper_sector, lower_sector
+from .operators import SectorHamiltonOperator, CreationOperator, AnnihilationOperator, SIGN_MODES
+
+logger = logging.getLogger(__name__)
+
+__all__ = [
+    "EigenState", "LanczosStats", "groundstate", "compute_groundstate", "solve_sector",
+    "GreensFunctionMeasurement", "gf_lehmann", "greens_lehmann_t0", "greens_cf", "CFResult",
+    "iter_lanczos_coeffs", "lanczos_coeffs", "lanczos_matrix", "lanczos_ground_state",
+]
+
+
+class EigenState(NamedTuple):
+    """cmpy/matrix.py:42-47."""
+
+    energy: float = np.inf
+    state: np.ndarray = None
+    n_up: int = None
+    n_dn: int = None
+
+
+@dataclass
+class LanczosStats:
+    iterations: int
+    converged: bool
+    breakdown: bool
+    ritz_residual: float
+    residual: float
+    matvec_ms: float
+    total_ms: float
+
+
+def _operator(model_or_op, n_up=None, n_dn=None) -> SectorHamiltonOperator:
+    if isinstance(model_or_op, SectorHamiltonOperator):
+        return model_or_op
+    if n_up is None or n_dn is None:
+        raise ValueError("a model needs the sector (n_up, n_dn)")
+    return model_or_op.hamilton_operator(n_up=n_up, n_dn=n_dn)
+
+
+def groundstate(model_or_op, n_up: int = None, n_dn: int = None, *, tol: float = 1e-10,
+                max_iter: int = 1000, min_iter: int = 0, check_every: int = 10,
+                reorth: Optional[bool] = None, want_vector: bool = True, fixed_iter: bool = False,
+                seed: int = 12345, v0=None, device: bool = False, return_stats: bool = False):
+    """Lowest eigenpair of one sector by device Lanczos.
+
+    Converged when the Ritz residual |H psi - E psi| <= tol (absolute, psi
+    normalised).  ``reorth`` defaults to full reorthogonalisation for sectors
+    up to 2e5 states and to the three-vector recurrence above.
+    Returns ``EigenState(energy, state, n_up, n_dn)``; ``state`` is a numpy array,
+    or a CUDA tensor when ``device=True``.
+    """
+    op = _operator(model_or_op, n_up, n_dn)
+    D = op.size
+    if D == 0:
+        raise ValueError("empty sector")
+    if reorth is None:
+        reorth = D <= 200_000
+    mi = int(min(max_iter, D))
+    cfg = _lib.LanczosCfg(mi, int(min_iter), int(check_every), int(bool(reorth)), int(bool(want_vector)),
+                          int(bool(fixed_iter)), float(tol), int(seed) & 0xFFFFFFFFFFFFFFFF)
+    nbytes = C.c_int64()
+    _lib.call("ed_lanczos_workspace", op.handle, C.byref(cfg), C.byref(nbytes))
+    work = torch.empty(nbytes.value, dtype=torch.uint8, device="cuda")
+    psi = torch.empty(D, dtype=torch.float64, device="cuda") if want_vector else None
+    v0t = None
+    if v0 is not None:
+        v0t = v0 if isinstance(v0, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(v0, np.float64))
+        v0t = v0t.to("cuda", torch.float64).contiguous()
+    res = _lib.LanczosResult()
+    _lib.call("ed_lanczos_groundstate", op.handle, C.byref(cfg), _lib.dptr(v0t) if v0t is not None else None,
+              _lib.dptr(psi) if psi is not None else None, _lib.dptr(work), nbytes.value, C.byref(res),
+              _lib.stream_ptr())
+    del work
+    state = None
+    if psi is not None:
+        state = psi if device else psi.cpu().numpy()
+    es = EigenState(float(res.e0), state, op.n_up, op.n_dn)
+    if return_stats:
+        st = LanczosStats(res.iterations, bool(res.converged), bool(res.breakdown), res.ritz_residual,
+                          res.residual, res.matvec_ms, res.total_ms)
+        return es, st
+    return es
+
+
+def _dense_eigh_device(op: SectorHamiltonOperator) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Eigenvalues (ascending) and eigenvectors stored as rows [k][D] on the device."""
+    D = op.size
+    h = torch.empty(D * D, dtype=torch.float64, device="cuda")
+    _lib.call("ed_op_dense", op.handle, _lib.dptr(h), _lib.stream_ptr())
+    w = torch.empty(D, dtype=torch.float64, device="cuda")
+    _lib.call("ed_dense_eigh", _lib.dptr(h), D, _lib.dptr(w), _lib.stream_ptr())
+    return w, h.view(D, D)
+
+
+def compute_groundstate(model, thresh: int = 50, tol: float = 1e-10) -> EigenState:
+    """Ground state over every (n_up, n_dn) sector (cmpy/exactdiag.py:24-43)."""
+    gs = EigenState()
+    for n_up, n_dn in model.basis.iter_fillings():
+        op = model.hamilton_operator(n_up=n_up, n_dn=n_dn)
+        if op.size <= thresh:
+            w, v = _dense_eigh_device(op)
+            energy, state = float(w[0].item()), v[0].cpu().numpy()
+        else:
+            r = groundstate(op, tol=tol)
+            energy, state = r.energy, r.state
+        if energy < gs.energy:
+            gs = EigenState(energy, state, n_up, n_dn)
+    return gs
+
+
+def solve_sector(model, sector: Sector, cache: dict = None):
+    """Dense eigendecomposition of one sector (cmpy/exactdiag.py:46-57).
+
+    Returns numpy (eigvals, eigvecs) with eigenvectors as columns, like
+    ``np.linalg.eigh``.
+    """
+    key = (sector.n_up, sector.n_dn)
+    if cache is not None and key in cache:
+        w, v = cache[key]
+    else:
+        w, v = _dense_eigh_device(model.hamilton_operator(sector=sector))
+        if cache is not None:
+            cache[key] = (w, v)
+    return w.cpu().numpy(), v.cpu().numpy().T
+
+
+# =============================================================================
+# Finite-temperature Lehmann Green's function
+# =============================================================================
+
+
+class GreensFunctionMeasurement:
+    """Accumulator of the grand-canonical Lehmann sum (cmpy/exactdiag.py:137-212).
+
+    Partition function, G(z) and occupations are accumulated relative to the
+    lowest energy seen so far; whenever a lower one appears every accumulated
+    quantity is rescaled by exp(-beta (E_old - E_new)).
+    """
+
+    def __init__(self, z, beta, pos=0, sigma=UP, dtype=None, measure_occ=True):
+        self.z = np.asarray(z)
+        self.beta = beta
+        self.pos = pos
+        self.sigma = sigma
+        self._measure_occ = measure_occ
+        self._part = 0.0
+        self._gs_energy = np.inf
+        self._gf = torch.zeros(2 * self.z.size, dtype=torch.float64, device="cuda")
+        self._z_dev = torch.from_numpy(np.ascontiguousarray(
+            np.stack([self.z.real.ravel(), self.z.imag.ravel()], axis=1).ravel())).cuda()
+        self._occ = 0.0
+        self._occ_double = 0.0
+
+    @property
+    def part(self):
+        return self._part * np.exp(-self.beta * self._gs_energy)
+
+    @property
+    def gf(self):
+        g = self._gf.view(-1, 2).cpu().numpy()
+        return ((g[:, 0] + 1j * g[:, 1]) / self._part).reshape(self.z.shape)
+
+    @property
+    def occ(self):
+        return self._occ / self._part
+
+    @property
+    def occ_double(self):
+        return self._occ_double / self._part
+
+    @property
+    def gs_energy(self):
+        return self._gs_energy
+
+    def _weights(self, w: torch.Tensor) -> torch.Tensor:
+        if np.isfinite(self.beta):
+            return torch.exp(-self.beta * (w - self._gs_energy))
+        return torch.ones_like(w)
+
+    def accumulate(self, num_sites, sector, sector_p1, w, V, w_p1, V_p1, sign_mode="reference"):
+        """Add one (sector, upper sector) pair; w/V, w_p1/V_p1 are device eigenpairs
+        (eigenvectors as rows)."""
+        min_energy = float(w[0].item())
+        factor = 1.0
+        if min_energy < self._gs_energy:
+            factor = float(np.exp(-self.beta * (self._gs_energy - min_energy)))
+            self._gs_energy = min_energy
+        boltz = self._weights(w)
+        self._part = self._part * factor + float(boltz.sum().item())
+        if factor != 1.0:
+            self._gf.mul_(factor)
+        n_up, n_dn = sector
+        N = V.shape[0]
+        # C = c+ V (rows), overlap[m][n] = <m_p1 | c+ | n>
+        Cm = torch.empty((N, V_p1.shape[1]), dtype=torch.float64, device="cuda")
+        _lib.call("ed_apply_cdag", num_sites, n_up, n_dn, self.pos, self.sigma, SIGN_MODES[sign_mode],
+                  _lib.dptr(V), _lib.dptr(Cm), N, _lib.stream_ptr())
+        M = V_p1.shape[0]
+        ov = torch.empty((M, N), dtype=torch.float64, device="cuda")
+        _lib.call("ed_gemm_nt", _lib.dptr(V_p1), _lib.dptr(Cm), _lib.dptr(ov), M, N, V_p1.shape[1],
+                  _lib.stream_ptr())
+        beta = float(self.beta) if np.isfinite(self.beta) else float("inf")
+        _lib.call("ed_lehmann_pairs", _lib.dptr(ov), _lib.dptr(w), N, _lib.dptr(w_p1), M, beta,
+                  float(self._gs_energy), _lib.dptr(self._z_dev), self.z.size, _lib.dptr(self._gf),
+                  _lib.stream_ptr())
+        if self._measure_occ:
+            out = torch.empty(2, dtype=torch.float64, device="cuda")
+            kind = 0 if self.sigma == UP else 1
+            _lib.call("ed_occupation", _lib.dptr(V), N, num_sites, n_up, n_dn, self.pos, kind,
+                      _lib.dptr(boltz), _lib.dptr(out[0:1]), _lib.stream_ptr())
+            _lib.call("ed_occupation", _lib.dptr(V), N, num_sites, n_up, n_dn, self.pos, 2,
+                      _lib.dptr(boltz), _lib.dptr(out[1:2]), _lib.stream_ptr())
+            o = out.cpu().numpy()
+            self._occ = self._occ * factor + float(o[0])
+            self._occ_double = self._occ_double * factor + float(o[1])
+
+
+def gf_lehmann(model, z, beta, pos=0, sigma=UP, eig_cache=None, occ=True) -> GreensFunctionMeasurement:
+    """Grand-canonical Lehmann Green's function (cmpy/exactdiag.py:215-245).
+
+    Visits the fillings in ``iter_fillings`` order; every sector with an upper
+    sector is diagonalised on the device (cached) and accumulated.  With
+    ``beta = inf`` the Boltzmann weights are 1, as in the reference.
+    """
+    basis = model.basis
+    ns = model.num_sites
+    data = GreensFunctionMeasurement(z, beta, pos, sigma, measure_occ=occ)
+    cache = eig_cache if eig_cache is not None else {}
+    sign_mode = "jordan_wigner" if getattr(model, "sign_mode", "reference") == "fermion" else "reference"
+
+    def solve(nu, nd):
+        if (nu, nd) not in cache:
+            cache[(nu, nd)] = _dense_eigh_device(model.hamilton_operator(n_up=nu, n_dn=nd))
+        return cache[(nu, nd)]
+
+    for n_up, n_dn in basis.iter_fillings():
+        p1 = upper_sector(n_up, n_dn, sigma, ns)
+        if p1 is None:
+            continue
+        w, V = solve(n_up, n_dn)
+        w1, V1 = solve(*p1)
+        data.accumulate(ns, (n_up, n_dn), p1, w, V, w1, V1, sign_mode)
+    return data
+
+
+# =============================================================================
+# T = 0 Green's functions in a fixed sector
+# =============================================================================
+
+
+def _z_device(z: np.ndarray) -> torch.Tensor:
+    z = np.asarray(z, dtype=np.complex128).ravel()
+    return torch.from_numpy(np.ascontiguousarray(np.stack([z.real, z.imag], axis=1).ravel())).cuda()
+
+
+def _ladder_device(psi: torch.Tensor, ns, n_up, n_dn, pos, sigma, create, sign_mode):
+    d = 1 if create else -1
+    nu = n_up + (d if sigma == UP else 0)
+    nd = n_dn + (d if sigma == DN else 0)
+    if not (0 <= nu <= ns and 0 <= nd <= ns):
+        return None, None
+    lib = _lib.lib()
+    size = int(lib.ed_binom(ns, nu)) * int(lib.ed_binom(ns, nd))
+    y = torch.empty(size, dtype=torch.float64, device="cuda")
+    _lib.call("ed_apply_cdag" if create else "ed_apply_c", ns, n_up, n_dn, pos, sigma, SIGN_MODES[sign_mode],
+              _lib.dptr(psi), _lib.dptr(y), 1, _lib.stream_ptr())
+    return y, (nu, nd)
+
+
+def _sign_mode(model, sign_mode):
+    if sign_mode is not None:
+        return sign_mode
+    return "jordan_wigner" if getattr(model, "sign_mode", "reference") == "fermion" else "reference"
+
+
+def _gs(model, n_up, n_dn, gs, tol):
+    if gs is not None:
+        psi = gs.state if isinstance(gs.state, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(gs.state))
+        return float(gs.energy), psi.to("cuda", torch.float64).contiguous()
+    r = groundstate(model, n_up, n_dn, tol=tol, device=True)
+    return r.energy, r.state
+
+
+def greens_lehmann_t0(model, z, n_up, n_dn, pos=0, sigma=UP, gs: EigenState = None, tol=1e-10,
+                      sign_mode=None) -> np.ndarray:
+    """T = 0 Lehmann G in sector (n_up, n_dn):
+
+    G(z) = sum_m |<m|c+|0>|^2/(z - E_m + E0) + sum_n |<n|c|0>|^2/(z + E_n - E0)
+
+    with the N+-1 sectors diagonalised densely on the device.
+    """
+    ns = model.num_sites
+    sign_mode = _sign_mode(model, sign_mode)
+    e0, psi = _gs(model, n_up, n_dn, gs, tol)
+    zt = _z_device(z)
+    g = torch.zeros(zt.numel(), dtype=torch.float64, device="cuda")
+    for create in (True, False):
+        phi, sec = _ladder_device(psi, ns, n_up, n_dn, pos, sigma, create, sign_mode)
+        if phi is None:
+            continue
+        op = model.hamilton_operator(n_up=sec[0], n_dn=sec[1])
+        w, V = _dense_eigh_device(op)
+        M = V.shape[0]
+        ov = torch.empty(M, dtype=torch.float64, device="cuda")
+        _lib.call("ed_gemm_nt", _lib.dptr(V), _lib.dptr(phi), _lib.dptr(ov), M, 1, op.size, _lib.stream_ptr())
+        weights = ov * ov
+        poles = (w - e0) if create else (e0 - w)
+        _lib.call("ed_pole_sum", _lib.dptr(weights), _lib.dptr(poles.contiguous()), M, _lib.dptr(zt),
+                  zt.numel() // 2, _lib.dptr(g), 1, _lib.stream_ptr())
+    gh = g.view(-1, 2).cpu().numpy()
+    return (gh[:, 0] + 1j * gh[:, 1]).reshape(np.shape(z))
+
+
+@dataclass
+class CFResult:
+    gf: np.ndarray
+    e0: float
+    alpha_p: np.ndarray
+    beta_p: np.ndarray
+    norm2_p: float
+    alpha_h: np.ndarray
+    beta_h: np.ndarray
+    norm2_h: float
+
+
+REORTH_MAX_BYTES = 16 << 30  # Krylov basis kept for full reorthogonalisation
+
+
+def _default_reorth(size: int, nsteps: int) -> bool:
+    return nsteps <= 4096 and 8 * size * (nsteps + 2) <= REORTH_MAX_BYTES
+
+
+def _cf_coeffs(op: SectorHamiltonOperator, phi: torch.Tensor, nsteps: int, reorth: Optional[bool] = None):
+    nsteps = int(max(1, min(nsteps, op.size)))
+    if reorth is None:
+        reorth = _default_reorth(op.size, nsteps)
+    nbytes = C.c_int64()
+    _lib.call("ed_lanczos_coeffs_workspace", op.handle, nsteps, int(bool(reorth)), C.byref(nbytes))
+    work = torch.empty(nbytes.value, dtype=torch.uint8, device="cuda")
+    a = np.zeros(nsteps)
+    b = np.zeros(max(nsteps, 1))
+    n2 = C.c_double()
+    m = C.c_int32()
+    _lib.call("ed_lanczos_coeffs", op.handle, _lib.dptr(phi), nsteps, int(bool(reorth)),
+              a.ctypes.data_as(C.POINTER(C.c_double)), b.ctypes.data_as(C.POINTER(C.c_double)),
+              C.byref(n2), C.byref(m), None, _lib.dptr(work), nbytes.value, _lib.stream_ptr())
+    return a[: m.value].copy(), b[: max(m.value - 1, 0)].copy(), float(n2.value)
+
+
+def greens_cf(model, z, n_up, n_dn, pos=0, sigma=UP, nsteps=200, gs: EigenState = None, tol=1e-10,
+              sign_mode=None, reorth: Optional[bool] = None) -> CFResult:
+    """T = 0 continued-fraction G from ``nsteps`` Lanczos steps started at c+psi0
+    (particle, sector N+1) and c psi0 (hole, sector N-1).  ``reorth`` defaults
+    to full reorthogonalisation whenever the Krylov basis fits in 16 GiB."""
+    ns = model.num_sites
+    sign_mode = _sign_mode(model, sign_mode)
+    e0, psi = _gs(model, n_up, n_dn, gs, tol)
+    zt = _z_device(z)
+    g = torch.zeros(zt.numel(), dtype=torch.float64, device="cuda")
+    coeffs = {}
+    for create in (True, False):
+        phi, sec = _ladder_device(psi, ns, n_up, n_dn, pos, sigma, create, sign_mode)
+        if phi is None:
+            coeffs[create] = (np.zeros(0), np.zeros(0), 0.0)
+            continue
+        op = model.hamilton_operator(n_up=sec[0], n_dn=sec[1])
+        a, b, n2 = _cf_coeffs(op, phi, nsteps, reorth)
+        coeffs[create] = (a, b, n2)
+        if len(a):
+            _lib.call("ed_cf_eval", a.ctypes.data_as(C.POINTER(C.c_double)),
+                      (b if len(b) else np.zeros(1)).ctypes.data_as(C.POINTER(C.c_double)), len(a), n2, e0,
+                      1 if create else -1, _lib.dptr(zt), zt.numel() // 2, _lib.dptr(g), 1, _lib.stream_ptr())
+    gh = g.view(-1, 2).cpu().numpy()
+    gf = (gh[:, 0] + 1j * gh[:, 1]).reshape(np.shape(z))
+    ap, bp, np_ = coeffs[True]
+    ah, bh, nh = coeffs[False]
+    return CFResult(gf, e0, ap, bp, np_, ah, bh, nh)
+
+
+# =============================================================================
+# Lanczos coefficient API (cmpy/exactdiag.py:324-375)
+# =============================================================================
+
+
+def iter_lanczos_coeffs(ham: SectorHamiltonOperator, size: int = 10, v0=None, seed: int = 0):
+    """Yields (a_n, b_n) like cmpy (first b is None), from a normalised device
+    Lanczos run.  ``v0`` defaults to a seeded uniform [0, 1) vector (cmpy draws an
+    unseeded one, exactdiag.py:326)."""
+    a, b = lanczos_coeffs(ham, size, v0=v0, seed=seed)
+    yield a[0], None
+    for k in range(1, len(a)):
+        yield a[k], b[k - 1]
+
+
+def lanczos_coeffs(ham: SectorHamiltonOperator, size: int = 10, v0=None, seed: int = 0, reorth=False):
+    """(a[size], b[size-1]) Lanczos coefficients of ``ham`` (cmpy/exactdiag.py:350-358)."""
+    if not isinstance(ham, SectorHamiltonOperator):
+        raise TypeError("lanczos_coeffs needs a SectorHamiltonOperator (model.hamilton_operator(...))")
+    if v0 is None:
+        v0 = np.random.default_rng(seed).uniform(0, 1, size=ham.size)
+    phi = torch.from_numpy(np.ascontiguousarray(v0, np.float64)).cuda()
+    a, b, _ = _cf_coeffs(ham, phi, size, reorth)
+    return list(a), list(b)
+
+
+def lanczos_matrix(a_coeffs, b_coeffs):
+    """Tridiagonal matrix of the coefficients (cmpy/exactdiag.py:361-365)."""
+    mat = np.diag(np.asarray(a_coeffs, dtype=np.float64))
+    np.fill_diagonal(mat[1:], b_coeffs)
+    np.fill_diagonal(mat[:, 1:], b_coeffs)
+    return mat
+
+
+def lanczos_ground_state(a_coeffs, b_coeffs, max_eig=3):
+    """Lowest eigenpair of the tridiagonal matrix (cmpy/exactdiag.py:368-375)."""
+    max_eig = min(max_eig, len(a_coeffs) - 1)
+    xi, vi = la.eigh_tridiagonal(a_coeffs, b_coeffs, select="i", select_range=(0, max_eig))
+    idx = np.argmin(xi)
+    return xi[idx], vi[:, idx]

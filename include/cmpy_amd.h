@@ -1,0 +1,249 @@
+/*
+ * cmpy_amd.h -- C ABI of the MI355X exact-diagonalisation engine (libcmpy_amd.so).
+ *
+ * This is the boundary the Python surface (cmpy_amd) binds with ctypes, and the
+ * one any other host language would bind.  Types are plain C: integers, doubles,
+ * device pointers and an opaque operator handle.  No torch type appears here.
+ *
+ * Conventions
+ *   - Every function returns an int status: ED_OK (0) or a negative ED_ERR_*.
+ *     ed_last_error() returns a thread-local message for the last failure.
+ *     No C++ exception crosses this boundary.
+ *   - Pointers named *_dev are device (HBM) pointers; pointers without the
+ *     suffix are host pointers.  The caller owns every buffer it passes.  The
+ *     library allocates only what sits behind an ed_op handle (state lists,
+ *     hop tables, diagonal tables) and frees it in ed_op_destroy.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *     Device work is enqueued on it; functions that return host results
+ *     (ed_lanczos_*, ed_batch_*) synchronise that stream before returning.
+ *   - Sector vectors are FP64, index k = iu * num_dn + id with iu/id the
+ *     positions of the spin-up/spin-down states in ascending integer order
+ *     (cmpy/basis.py:570-575, 655-666).  A block of nvec vectors is stored
+ *     vector after vector ([nvec][D]).
+ *   - Complex numbers are interleaved (re, im) doubles.
+ *   - Distinct handles may be used from distinct threads; one handle must not
+ *     be used by two threads at once.
+ */
+#ifndef CMPY_AMD_H
+#define CMPY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ED_OK 0
+#define ED_ERR_INVALID (-1)  /* bad argument (shape, range, bond order, ...) */
+#define ED_ERR_HIP (-2)      /* a HIP runtime call failed                     */
+#define ED_ERR_NOMEM (-3)    /* device allocation failed                      */
+#define ED_ERR_NUMERIC (-4)  /* NaN/Inf met, or a solve did not converge       */
+#define ED_ERR_LIMIT (-5)    /* size beyond what this build supports          */
+
+#define ED_UP 1 /* cmpy/basis.py:40 */
+#define ED_DN 2
+
+#define ED_SIGN_REFERENCE 0     /* no Jordan-Wigner string (cmpy/operators.py:652-703) */
+#define ED_SIGN_JORDAN_WIGNER 1 /* within-spin string (-1)^popcount(s & (2^pos - 1))  */
+
+#define ED_MAX_SITES 30
+
+typedef struct ed_op ed_op;
+
+/* ------------------------------------------------------------------------ */
+/* Library                                                                   */
+/* ------------------------------------------------------------------------ */
+
+const char* ed_last_error(void);
+int ed_version(void);
+/* Number of visible HIP devices (0 without a GPU); never fails. */
+int ed_device_count(void);
+
+/* ------------------------------------------------------------------------ */
+/* Basis -- replaces Basis.generate_states / get_states (cmpy/basis.py:655-676)
+ * and the bisect_left index lookup (cmpy/operators.py:276-299).             */
+/* ------------------------------------------------------------------------ */
+
+/* C(n, k) for 0 <= n <= 64; 0 outside the triangle. */
+int64_t ed_binom(int n, int k);
+
+/* Host: all num_sites-bit states with popcount n, ascending (Gosper's
+ * next-combination).  `out` holds ed_binom(num_sites, n) entries.           */
+int ed_sector_states_host(int num_sites, int n, uint32_t* out);
+
+/* Device: same list, state r computed by combinadic unranking of r.        */
+int ed_sector_states(int num_sites, int n, uint32_t* out_dev, void* stream);
+
+/* Device: combinadic rank r(s) = sum_k C(p_k, k) of each state (its position
+ * in the ascending list of its popcount); -1 where popcount(s) != n.        */
+int ed_sector_rank(int num_sites, int n, const uint32_t* states_dev, int64_t count,
+                   int64_t* ranks_dev, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Projectors -- replaces _hopping_sign / _compute_hopping_term
+ * (cmpy/operators.py:425-460).  One entry per state of the (num_sites, n)
+ * spin sector, in ascending state order: target = index of the hopped state
+ * (or -1 when the bond cannot hop from this state), value = sign * hop.
+ * `width` is the bit width of the sign string: num_sites for the Hubbard
+ * model (cmpy/models/hubbard.py:22), 0 for the Anderson impurity model
+ * (cmpy/models/anderson.py:149, all signs +1).                              */
+/* ------------------------------------------------------------------------ */
+int ed_hop_terms(int num_sites, int n, int width, int site1, int site2, double hop,
+                 int64_t* target_dev, double* value_dev, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Sector Hamiltonian -- replaces AbstractManyBodyModel.hamilton_operator +
+ * HamiltonOperator (cmpy/models/abc.py:242-256, cmpy/operators.py:614-646)
+ * with a matrix-free operator over the up (x) dn product structure:
+ *   H = sum_i eps_i (n_i,up + n_i,dn) + sum_i u_i n_i,up n_i,dn
+ *     + sum_b hop_b sum_sigma (c+_{j,sigma} c_{i,sigma} + h.c.) * sign       */
+/* ------------------------------------------------------------------------ */
+
+typedef struct ed_model_desc {
+  int32_t num_sites;     /* <= ED_MAX_SITES                                  */
+  int32_t n_up, n_dn;    /* sector fillings                                  */
+  int32_t num_bonds;
+  const int32_t* bonds;  /* host [num_bonds][2], bonds[b][0] < bonds[b][1]   */
+  const double* hop;     /* host [num_bonds]                                 */
+  const double* eps;     /* host [num_sites], on-site energy minus mu        */
+  const double* u;       /* host [num_sites], on-site interaction            */
+  int32_t width;         /* hopping-sign width (see ed_hop_terms)            */
+} ed_model_desc;
+
+int ed_op_create(ed_op** op, const ed_model_desc* desc, void* stream);
+int ed_op_destroy(ed_op* op);
+int ed_op_shape(const ed_op* op, int64_t* num_up, int64_t* num_dn);
+
+/* y = H x for nvec vectors (cmpy/operators.py:626-630).  x and y must not
+ * overlap.                                                                  */
+int ed_op_matvec(ed_op* op, const double* x_dev, double* y_dev, int64_t nvec, void* stream);
+
+/* y = H x for H given as triples sorted by target row (CSR by target):
+ * y[t] = sum_{k=ptr[t]}^{ptr[t+1]-1} val[k] * x[src[k]], in stored order.
+ * The COO HamiltonOperator path (cmpy/operators.py:614-630).              */
+int ed_csr_matvec(const int64_t* ptr_dev, const int64_t* src_dev, const double* val_dev, int64_t n,
+                  const double* x_dev, double* y_dev, void* stream);
+
+/* Diagonal of H (onsite + interaction, cmpy/operators.py:305-422).          */
+int ed_op_diagonal(ed_op* op, double* diag_dev, void* stream);
+
+/* Dense H, row-major D x D (HamiltonOperator.toarray, operators.py:632-635). */
+int ed_op_dense(ed_op* op, double* h_dev, void* stream);
+
+/* Per-spin hop table of the operator in its ELL form, for inspection:
+ * cnt[D_sigma], col/val[D_sigma][max_hops] (row-major, valid entries first,
+ * in bond order).  max_hops is returned through *max_hops (call with NULL
+ * buffers to query it).                                                     */
+int ed_op_hop_table(const ed_op* op, int sigma, int32_t* max_hops, int32_t* cnt_dev,
+                    int32_t* col_dev, double* val_dev, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Creation / annihilation -- replaces _apply_{creation,annihilation}_{up,dn},
+ * CreationOperator, AnnihilationOperator (cmpy/operators.py:652-791).
+ * x lives in sector (n_up, n_dn); y in the sector one particle above (cdag)
+ * or below (c).  y is fully written (zeros where nothing maps).            */
+/* ------------------------------------------------------------------------ */
+int ed_apply_cdag(int num_sites, int n_up, int n_dn, int pos, int sigma, int sign_mode,
+                  const double* x_dev, double* y_dev, int64_t nvec, void* stream);
+int ed_apply_c(int num_sites, int n_up, int n_dn, int pos, int sigma, int sign_mode,
+               const double* x_dev, double* y_dev, int64_t nvec, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Lanczos -- replaces compute_groundstate's eigsh call for one sector
+ * (cmpy/exactdiag.py:24-43) and iter_lanczos_coeffs / lanczos_ground_state
+ * (cmpy/exactdiag.py:324-375).  FP64 throughout, deterministic reductions.  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct ed_lanczos_cfg {
+  int32_t max_iter;     /* maximum Lanczos steps                              */
+  int32_t min_iter;     /* no convergence test before this many steps         */
+  int32_t check_every;  /* steps between convergence tests                    */
+  int32_t reorth;       /* 0: three-vector recurrence; 1: full reorthogonalisation */
+  int32_t want_vector;  /* 1: also return the ground-state vector             */
+  int32_t fixed_iter;   /* 1: run exactly max_iter steps (no early stop)      */
+  double tol;           /* stop when the Ritz residual |H psi - E psi| <= tol  */
+  uint64_t seed;        /* start vector seed (used when v0_dev is NULL)       */
+} ed_lanczos_cfg;
+
+typedef struct ed_lanczos_result {
+  double e0;             /* lowest Ritz value                                  */
+  double ritz_residual;  /* beta_{m+1} |s_m| of the lowest Ritz pair           */
+  double residual;       /* explicit |H psi - e0 psi| (want_vector only)        */
+  int32_t iterations;    /* Lanczos steps taken                                */
+  int32_t converged;
+  int32_t breakdown;     /* invariant subspace reached                         */
+  int32_t pad;
+  double matvec_ms;      /* summed H.v kernel time (HIP events)                */
+  double total_ms;       /* wall time of the call                              */
+} ed_lanczos_result;
+
+/* Workspace (bytes) that ed_lanczos_groundstate needs for this op/config.   */
+int ed_lanczos_workspace(const ed_op* op, const ed_lanczos_cfg* cfg, int64_t* bytes);
+
+int ed_lanczos_groundstate(ed_op* op, const ed_lanczos_cfg* cfg, const double* v0_dev,
+                           double* psi_dev, void* work_dev, int64_t work_bytes,
+                           ed_lanczos_result* res, void* stream);
+
+/* Lanczos coefficients from a given start vector (the continued-fraction
+ * input): alpha[0..m-1], beta[0..m-2] (beta[k] couples steps k and k+1),
+ * *norm2 = |v0|^2, *steps = m <= nsteps (fewer on breakdown).  reorth = 1
+ * orthogonalises every new vector against the whole basis (two Gram-Schmidt
+ * passes; nsteps <= 4096), which keeps real-axis continued fractions exact up
+ * to the Krylov dimension.  Optionally the normalised basis vectors are
+ * written to basis_dev ([nsteps][D]).                                       */
+int ed_lanczos_coeffs_workspace(const ed_op* op, int32_t nsteps, int32_t reorth, int64_t* bytes);
+int ed_lanczos_coeffs(ed_op* op, const double* v0_dev, int32_t nsteps, int32_t reorth, double* alpha,
+                      double* beta, double* norm2, int32_t* steps, double* basis_dev,
+                      void* work_dev, int64_t work_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Green's functions                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Continued fraction (no reference counterpart; cmpy has no CF):
+ *   g[k] (+)= sign * norm2 / (sign z_k + shift - a_0 - b_0^2/(sign z_k + shift - a_1 - ...))
+ * sign=+1, shift=E0: particle part from sector N+1;
+ * sign=-1, shift=E0: hole part from sector N-1.
+ * alpha/beta are host arrays (m and m-1 entries).                           */
+int ed_cf_eval(const double* alpha, const double* beta, int32_t m, double norm2, double shift,
+               int32_t sign, const double* z_dev, int64_t nz, double* g_dev, int32_t accumulate,
+               void* stream);
+
+/* Pole sum g[k] (+)= sum_m w_m / (z_k - p_m): the T = 0 Lehmann sum of
+ * _accumulate_sum (cmpy/exactdiag.py:110-129) restricted to the ground state. */
+int ed_pole_sum(const double* w_dev, const double* p_dev, int64_t m, const double* z_dev,
+                int64_t nz, double* g_dev, int32_t accumulate, void* stream);
+
+/* Finite-temperature pair sum of _accumulate_sum (cmpy/exactdiag.py:110-129):
+ *   g[k] += sum_{m,n} |ov[m,n]|^2 (e^{-beta(E_n-emin)} + e^{-beta(E_m-emin)})
+ *                      / (z_k - E_m + E_n)
+ * with ov = V_p1^T C (overlap_dev holds ov[m][n], row-major M x N).
+ * beta = +inf gives unit weights, as the reference does.                    */
+int ed_lehmann_pairs(const double* overlap_dev, const double* evals_dev, int64_t n,
+                     const double* evals_p1_dev, int64_t m, double beta, double emin,
+                     const double* z_dev, int64_t nz, double* g_dev, void* stream);
+
+/* C[M x N] = A B^T with A: [M][K], B: [N][K] (vectors stored one after
+ * another): the overlap product evecs_p1^T (c+ evecs) of cmpy/exactdiag.py:112. */
+int ed_gemm_nt(const double* a_dev, const double* b_dev, double* c_dev, int64_t m, int64_t n,
+               int64_t k, void* stream);
+
+/* Thermal occupations of occupation_up/dn and double_occupation
+ * (cmpy/exactdiag.py:60-107):  *out = sum_k w_k sum_{i in S} |V_k[i]|^2 with
+ * V: [nvec][D] sector vectors of (num_sites, n_up, n_dn) and S the indices
+ * whose up (kind 0), dn (kind 1) or both (kind 2) spins occupy `pos`.
+ * w_dev: [nvec] weights; out_dev: one double.                               */
+int ed_occupation(const double* vecs_dev, int64_t nvec, int num_sites, int n_up, int n_dn, int pos,
+                  int kind, const double* w_dev, double* out_dev, void* stream);
+
+/* Dense symmetric eigensolver (LAPACK dsyevd semantics, ascending values):
+ * replaces np.linalg.eigh of solve_sector (cmpy/exactdiag.py:46-57).
+ * a_dev (n x n symmetric) is overwritten by the eigenvectors stored vector
+ * after vector: a[k*n + i] = component i of eigenvector k.                */
+int ed_dense_eigh(double* a_dev, int64_t n, double* w_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CMPY_AMD_H */

@@ -1,0 +1,859 @@
+"""CPU oracle for the exact-diagonalisation hot path.
+
+TEST INFRASTRUCTURE ONLY.  This module restates, in numpy/scipy, the algorithm
+that cmpy (dylanljones/cmpy) uses for the Hubbard/Anderson sector Hamiltonian,
+the ground state, the Lanczos recurrence and the Lehmann Green's function.  It
+is the checker: only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import it.  The product package
+``cmpy_amd`` never imports, calls or links anything under ``oracle/``.
+
+Every function cites the reference file:line it follows (paths relative to
+the reference checkout).  How each one is pinned (checked against something
+other than a retyping of its own formula) is recorded in ``PARITY`` at the
+bottom of this file; the test-suite exercises every entry.
+
+Conventions carried over from the reference:
+  * sector index  k = iu * num_dn + id  (up-major, cmpy/basis.py:570-575)
+  * spin states in ascending integer order (cmpy/basis.py:655-666)
+  * hopping sign (-1)^popcount(state & between(site1, site2) & lowmask(width))
+    with width = num_sites for the Hubbard model and width = 0 (all signs +1)
+    for the Anderson impurity model (cmpy/models/anderson.py:149)
+  * creation/annihilation operators copy blocks without a Jordan-Wigner sign
+    (cmpy/operators.py:652-703); ``sign_mode="jordan_wigner"`` adds the
+    within-spin string as an opt-in
+"""
+
+from __future__ import annotations
+
+import itertools
+import math
+from typing import Iterable, List, Sequence, Tuple
+
+import numpy as np
+import scipy.linalg as la
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+UP, DN = 1, 2  # cmpy/basis.py:40
+
+
+# =============================================================================
+# Basis
+# =============================================================================
+
+
+def binom(n: int, k: int) -> int:
+    if k < 0 or k > n or n < 0:
+        return 0
+    return math.comb(n, k)
+
+
+def generate_states_permutations(num_sites: int, n):
+    """Literal restatement of ``Basis.generate_states`` (cmpy/basis.py:655-666).
+
+    Enumerates permutations of a bit list, O(num_sites!) -- only usable for
+    num_sites <= 9.  Kept to pin :func:`sector_states` against the reference's
+    own enumeration.
+    """
+    total = num_sites
+    if n is None:
+        return list(range(2**total))
+    elif n == 0:
+        return [0]
+    elif n == 1:
+        return list(2**b for b in range(total))
+    bitvals = ["0" for _ in range(total - n)] + ["1" for _ in range(n)]
+    states = set(int("".join(bits), 2) for bits in itertools.permutations(bitvals))
+    return np.asarray(sorted(states))
+
+
+def popcount(a) -> np.ndarray:
+    a = np.asarray(a, dtype=np.uint64)
+    c = np.zeros(a.shape, dtype=np.int64)
+    x = a.copy()
+    while np.any(x):
+        c += (x & np.uint64(1)).astype(np.int64)
+        x >>= np.uint64(1)
+    return c
+
+
+def sector_states(num_sites: int, n) -> np.ndarray:
+    """All num_sites-bit integers with popcount n, ascending.
+
+    Same output contract as ``Basis.get_states`` (cmpy/basis.py:655-676), computed
+    by filtering ``range(2**num_sites)`` instead of enumerating permutations.
+    """
+    if n is None:
+        return np.arange(2**num_sites, dtype=np.int64)
+    if n < 0 or n > num_sites:
+        return np.zeros(0, dtype=np.int64)
+    allst = np.arange(2**num_sites, dtype=np.int64)
+    return allst[popcount(allst) == n]
+
+
+def combinadic_rank(states: Sequence[int]) -> np.ndarray:
+    """r(s) = sum_k C(p_k, k) over set bits p_1 < p_2 < ... (k is 1-based).
+
+    Equals the position of s in the ascending list of states with the same
+    popcount; used as the index lookup that replaces ``bisect_left``
+    (cmpy/operators.py:276-299).
+    """
+    out = np.zeros(len(states), dtype=np.int64)
+    for idx, s in enumerate(np.asarray(states, dtype=np.int64)):
+        s = int(s)
+        r, k, p = 0, 0, 0
+        while s:
+            if s & 1:
+                k += 1
+                r += binom(p, k)
+            s >>= 1
+            p += 1
+        out[idx] = r
+    return out
+
+
+def upper_sector(n_up, n_dn, sigma, num_sites):
+    """cmpy/basis.py:504-526."""
+    if sigma == UP and n_up < num_sites:
+        return n_up + 1, n_dn
+    elif sigma == DN and n_dn < num_sites:
+        return n_up, n_dn + 1
+    return None
+
+
+def lower_sector(n_up, n_dn, sigma):
+    """cmpy/basis.py:529-553."""
+    if sigma == UP and n_up > 0:
+        return n_up - 1, n_dn
+    elif sigma == DN and n_dn > 0:
+        return n_up, n_dn - 1
+    return None
+
+
+def state_label(up_num: int, dn_num: int, digits: int = None) -> str:
+    """cmpy/basis.py:49-74 (one character per site, site 0 first)."""
+    chars_map = {0: ".", 1: "↑", 2: "↓", 3: "⇅"}
+    digits = digits if digits is not None else 0
+    min_digits = max(len(bin(up_num)[2:]), len(bin(dn_num)[2:]))
+    num_chars = max(min_digits, digits)
+    chars = []
+    for i in range(num_chars):
+        u = up_num >> i & 1
+        d = dn_num >> i & 1
+        chars.append(chars_map[u + (d << 1)])
+    return "".join(chars[::1])
+
+
+# =============================================================================
+# Projectors (integer parts and FP64 matrix elements)
+# =============================================================================
+
+
+def bit_count_width(number: int, width: int) -> int:
+    """cmpy/operators.py:253-273: popcount of the low ``width`` bits."""
+    count = 0
+    for i in range(width):
+        if number & (1 << i):
+            count += 1
+    return count
+
+
+def hopping_sign(initial_state: int, width: int, site1: int, site2: int) -> int:
+    """cmpy/operators.py:425-433."""
+    mask = 0
+    for i in range(site1 + 1, site2):
+        mask += 1 << i
+    jump_overs = bit_count_width(initial_state & mask, width)
+    return (-1) ** jump_overs
+
+
+def hopping_terms(states, width: int, site1: int, site2: int, hop: float):
+    """Vectorised ``_compute_hopping_term`` (cmpy/operators.py:436-460).
+
+    Returns (origin, target, value) arrays in the reference's yield order
+    (ascending origin).  ``target`` is found with a left bisection of the sorted
+    state list, as the reference does (operators.py:276-299, 454/459).
+    """
+    assert site1 < site2
+    st = np.asarray(states, dtype=np.int64)
+    op1 = np.int64(1 << site1)
+    op2 = np.int64(1 << site2)
+    occ1 = (st & op1) != 0
+    occ2 = (st & op2) != 0
+    new = st ^ op1 ^ op2
+    ok = occ1 ^ occ2
+    mask = 0
+    for i in range(site1 + 1, site2):
+        mask += 1 << i
+    mask &= (1 << max(width, 0)) - 1 if width > 0 else 0
+    jumps = popcount(st & np.int64(mask))
+    sign = np.where(jumps % 2 == 0, 1, -1)
+    origin = np.nonzero(ok)[0].astype(np.int64)
+    target = np.searchsorted(st, new[ok], side="left").astype(np.int64)
+    value = (sign[ok] * hop).astype(np.float64)
+    return origin, target, value
+
+
+def weighted_element(states, values) -> np.ndarray:
+    """cmpy/operators.py:226-250, summed site by site in the reference's order."""
+    st = np.asarray(states, dtype=np.int64)
+    acc = np.zeros(st.shape, dtype=np.float64)
+    for i in range(len(values)):
+        bit = (st & np.int64(1 << i)) != 0
+        acc = np.where(bit, acc + float(values[i]), acc)
+    return acc
+
+
+def onsite_energy_triples(up_states, dn_states, eps):
+    """cmpy/operators.py:359-422.  Only nonzero energies are emitted (409, 419)."""
+    up = np.asarray(up_states, dtype=np.int64)
+    dn = np.asarray(dn_states, dtype=np.int64)
+    num_up, num_dn = len(up), len(dn)
+    e_up = weighted_element(up, eps)
+    e_dn = weighted_element(dn, eps)
+    rows = []
+    vals = []
+    iu = np.nonzero(e_up)[0]
+    r = (iu[:, None] * num_dn + np.arange(num_dn)[None, :]).ravel()
+    rows.append(r)
+    vals.append(np.repeat(e_up[iu], num_dn))
+    idn = np.nonzero(e_dn)[0]
+    r = (np.arange(num_up)[None, :] * num_dn + idn[:, None]).ravel()
+    rows.append(r)
+    vals.append(np.repeat(e_dn[idn], num_up))
+    rows = np.concatenate(rows).astype(np.int64)
+    return rows, rows.copy(), np.concatenate(vals)
+
+
+def hubbard_inter_triples(up_states, dn_states, u):
+    """cmpy/operators.py:305-356.  Only nonzero energies are emitted (354)."""
+    up = np.asarray(up_states, dtype=np.int64)
+    dn = np.asarray(dn_states, dtype=np.int64)
+    num_dn = len(dn)
+    both = (up[:, None] & dn[None, :]).ravel()
+    energy = weighted_element(both, u)
+    idx = np.nonzero(energy)[0].astype(np.int64)
+    return idx, idx.copy(), energy[idx]
+
+
+def hopping_triples(up_states, dn_states, num_sites, site1, site2, hop):
+    """cmpy/operators.py:463-527 (up block T_up x I, then dn block I x T_dn)."""
+    up = np.asarray(up_states, dtype=np.int64)
+    dn = np.asarray(dn_states, dtype=np.int64)
+    num_up, num_dn = len(up), len(dn)
+    all_up, all_dn = np.arange(num_up), np.arange(num_dn)
+    o, t, a = hopping_terms(up, num_sites, site1, site2, hop)
+    r1 = (o[:, None] * num_dn + all_dn[None, :]).ravel()
+    c1 = (t[:, None] * num_dn + all_dn[None, :]).ravel()
+    v1 = np.repeat(a, num_dn)
+    o, t, a = hopping_terms(dn, num_sites, site1, site2, hop)
+    r2 = (all_up[None, :] * num_dn + o[:, None]).ravel()
+    c2 = (all_up[None, :] * num_dn + t[:, None]).ravel()
+    v2 = np.repeat(a, num_up)
+    return (
+        np.concatenate([r1, r2]).astype(np.int64),
+        np.concatenate([c1, c2]).astype(np.int64),
+        np.concatenate([v1, v2]).astype(np.float64),
+    )
+
+
+def _concat(parts):
+    rows = np.concatenate([p[0] for p in parts]) if parts else np.zeros(0, np.int64)
+    cols = np.concatenate([p[1] for p in parts]) if parts else np.zeros(0, np.int64)
+    vals = np.concatenate([p[2] for p in parts]) if parts else np.zeros(0)
+    return rows.astype(np.int64), cols.astype(np.int64), vals.astype(np.float64)
+
+
+def hubbard_triples(up_states, dn_states, num_sites, neighbors, inter, eps, hop, mu=0.0):
+    """``HubbardModel._hamiltonian_data`` (cmpy/models/hubbard.py:13-22, 75-81)."""
+    energy = np.full(num_sites, eps - mu, dtype=np.float64)
+    interaction = np.full(num_sites, inter, dtype=np.float64)
+    parts = [
+        onsite_energy_triples(up_states, dn_states, energy),
+        hubbard_inter_triples(up_states, dn_states, interaction),
+    ]
+    for i, j in neighbors:
+        if i < j:
+            parts.append(hopping_triples(up_states, dn_states, num_sites, i, j, hop))
+    return _concat(parts)
+
+
+def siam_triples(up_states, dn_states, u, eps_imp, eps_bath, v, mu):
+    """``SingleImpurityAndersonModel._hamiltonian_data`` (anderson.py:147-158).
+
+    Note ``nsites = 0`` (anderson.py:149): the hopping-sign width is zero, so
+    every bath hopping carries sign +1.
+    """
+    eps_bath = np.atleast_1d(np.asarray(eps_bath, dtype=np.float64))
+    v = np.atleast_1d(np.asarray(v, dtype=np.float64))
+    num_bath = len(eps_bath)
+    nsites = 0
+    uu = np.append(np.float64(u), np.zeros(num_bath))
+    eps = np.append(np.float64(eps_imp) - mu, eps_bath)
+    parts = [
+        onsite_energy_triples(up_states, dn_states, eps),
+        hubbard_inter_triples(up_states, dn_states, uu),
+    ]
+    for j in range(num_bath):
+        parts.append(hopping_triples(up_states, dn_states, nsites, 0, j + 1, v[j]))
+    return _concat(parts)
+
+
+# =============================================================================
+# COO Hamilton operator (reference semantics)
+# =============================================================================
+
+
+def coo_matvec(size, rows, cols, vals, x):
+    """``HamiltonOperator._matvec`` (cmpy/operators.py:626-630):
+    ``matvec[col] += val * x[row]`` for every stored triple."""
+    x = np.asarray(x)
+    y = np.zeros_like(x)
+    np.add.at(y, cols, vals * x[rows])
+    return y
+
+
+def coo_dense(size, rows, cols, vals):
+    """``HamiltonOperator.toarray`` (cmpy/operators.py:632-635): csr sums duplicates."""
+    return sp.csr_matrix((vals, (rows, cols)), shape=(size, size)).toarray()
+
+
+def coo_csr(size, rows, cols, vals):
+    return sp.csr_matrix((vals, (rows, cols)), shape=(size, size))
+
+
+def coo_trace(rows, cols, vals):
+    """cmpy/operators.py:641-646."""
+    return float(np.sum(vals[rows == cols]))
+
+
+# =============================================================================
+# Matrix-free factorised form used for larger sectors
+# =============================================================================
+
+
+class SectorOperator:
+    """H = diag + T_up (x) I + I (x) T_dn for one (n_up, n_dn) sector.
+
+    Built from the same projector restatements as the COO assembly above
+    (``hopping_terms``, ``weighted_element``); pinned to it at small sizes by the
+    test-suite.  ``matvec`` is exact up to floating-point summation order.
+    """
+
+    def __init__(self, num_sites, n_up, n_dn, bonds, hops, eps, u, width):
+        self.num_sites = num_sites
+        self.n_up, self.n_dn = n_up, n_dn
+        self.up = sector_states(num_sites, n_up)
+        self.dn = sector_states(num_sites, n_dn)
+        self.num_up, self.num_dn = len(self.up), len(self.dn)
+        self.size = self.num_up * self.num_dn
+        eps = np.asarray(eps, dtype=np.float64)
+        u = np.asarray(u, dtype=np.float64)
+        self.d_up = weighted_element(self.up, eps)
+        self.d_dn = weighted_element(self.dn, eps)
+        self.u = u
+        self.t_up = self._spin_matrix(self.up, bonds, hops, width)
+        self.t_dn = self._spin_matrix(self.dn, bonds, hops, width)
+
+    @staticmethod
+    def _spin_matrix(states, bonds, hops, width):
+        n = len(states)
+        rows, cols, vals = [], [], []
+        for (i, j), t in zip(bonds, hops):
+            o, tg, a = hopping_terms(states, width, int(i), int(j), float(t))
+            rows.append(o)
+            cols.append(tg)
+            vals.append(a)
+        if rows:
+            rows = np.concatenate(rows)
+            cols = np.concatenate(cols)
+            vals = np.concatenate(vals)
+        else:
+            rows = cols = np.zeros(0, np.int64)
+            vals = np.zeros(0)
+        # y[target] += val * x[origin]  ->  matrix element M[target, origin]
+        return sp.csr_matrix((vals, (cols, rows)), shape=(n, n))
+
+    def diagonal(self) -> np.ndarray:
+        both = (self.up[:, None] & self.dn[None, :]).ravel()
+        ud = weighted_element(both, self.u)
+        return (self.d_up[:, None] + self.d_dn[None, :]).ravel() + ud
+
+    def matvec(self, x):
+        x = np.asarray(x)
+        X = x.reshape(self.num_up, self.num_dn)
+        Y = self.t_up @ X + (self.t_dn @ X.T).T
+        return self.diagonal() * x + Y.ravel()
+
+    def csr(self):
+        n_up, n_dn = self.num_up, self.num_dn
+        h = sp.kron(self.t_up, sp.identity(n_dn)) + sp.kron(sp.identity(n_up), self.t_dn)
+        return (h + sp.diags(self.diagonal())).tocsr()
+
+
+def hubbard_operator(num_sites, n_up, n_dn, neighbors, inter, eps, hop, mu=0.0):
+    bonds = [(int(i), int(j)) for i, j in neighbors if i < j]
+    return SectorOperator(
+        num_sites, n_up, n_dn, bonds, [hop] * len(bonds),
+        np.full(num_sites, eps - mu), np.full(num_sites, inter), num_sites,
+    )
+
+
+def siam_operator(n_up, n_dn, u, eps_imp, eps_bath, v, mu, width=0):
+    eps_bath = np.atleast_1d(eps_bath).astype(np.float64)
+    v = np.atleast_1d(v).astype(np.float64)
+    nb = len(eps_bath)
+    bonds = [(0, j + 1) for j in range(nb)]
+    eps = np.append(eps_imp - mu, eps_bath)
+    uu = np.append(u, np.zeros(nb))
+    return SectorOperator(nb + 1, n_up, n_dn, bonds, list(v), eps, uu, width)
+
+
+# =============================================================================
+# Creation / annihilation operators
+# =============================================================================
+
+
+def _jw_sign(states, pos):
+    low = (1 << pos) - 1
+    return np.where(popcount(np.asarray(states, np.int64) & low) % 2 == 0, 1.0, -1.0)
+
+
+def apply_cdag(x, num_sites, n_up, n_dn, pos, sigma, sign_mode="reference"):
+    """c^dagger_{pos,sigma} from sector (n_up, n_dn) to the upper sector.
+
+    cmpy/operators.py:652-676 (block copies, no sign) and 706-747.  With
+    ``sign_mode="jordan_wigner"`` the within-spin string (-1)^popcount(s & (2^pos-1))
+    is applied (not part of the reference; identical at pos = 0).
+    """
+    x = np.asarray(x)
+    up = sector_states(num_sites, n_up)
+    dn = sector_states(num_sites, n_dn)
+    num_up, num_dn = len(up), len(dn)
+    X = x.reshape(num_up, num_dn)
+    op = 1 << pos
+    if sigma == UP:
+        up1 = sector_states(num_sites, n_up + 1)
+        Y = np.zeros((len(up1), num_dn), dtype=x.dtype)
+        src = np.nonzero((up & op) == 0)[0]
+        tgt = np.searchsorted(up1, up[src] ^ op)
+        s = _jw_sign(up[src], pos) if sign_mode == "jordan_wigner" else np.ones(len(src))
+        Y[tgt, :] = s[:, None] * X[src, :]
+    else:
+        dn1 = sector_states(num_sites, n_dn + 1)
+        Y = np.zeros((num_up, len(dn1)), dtype=x.dtype)
+        src = np.nonzero((dn & op) == 0)[0]
+        tgt = np.searchsorted(dn1, dn[src] ^ op)
+        s = _jw_sign(dn[src], pos) if sign_mode == "jordan_wigner" else np.ones(len(src))
+        Y[:, tgt] = X[:, src] * s[None, :]
+    return Y.ravel()
+
+
+def apply_c(x, num_sites, n_up, n_dn, pos, sigma, sign_mode="reference"):
+    """c_{pos,sigma} from sector (n_up, n_dn) to the lower sector.
+
+    cmpy/operators.py:679-703 and 750-791.
+    """
+    x = np.asarray(x)
+    up = sector_states(num_sites, n_up)
+    dn = sector_states(num_sites, n_dn)
+    num_up, num_dn = len(up), len(dn)
+    X = x.reshape(num_up, num_dn)
+    op = 1 << pos
+    if sigma == UP:
+        up1 = sector_states(num_sites, n_up - 1)
+        Y = np.zeros((len(up1), num_dn), dtype=x.dtype)
+        src = np.nonzero((up & op) != 0)[0]
+        tgt = np.searchsorted(up1, up[src] ^ op)
+        s = _jw_sign(up[src], pos) if sign_mode == "jordan_wigner" else np.ones(len(src))
+        Y[tgt, :] = s[:, None] * X[src, :]
+    else:
+        dn1 = sector_states(num_sites, n_dn - 1)
+        Y = np.zeros((num_up, len(dn1)), dtype=x.dtype)
+        src = np.nonzero((dn & op) != 0)[0]
+        tgt = np.searchsorted(dn1, dn[src] ^ op)
+        s = _jw_sign(dn[src], pos) if sign_mode == "jordan_wigner" else np.ones(len(src))
+        Y[:, tgt] = X[:, src] * s[None, :]
+    return Y.ravel()
+
+
+# =============================================================================
+# Ground state and Lanczos
+# =============================================================================
+
+
+def groundstate(op, tol=0.0, v0=None):
+    """One-sector ``compute_groundstate`` (cmpy/exactdiag.py:24-43).
+
+    Dense ``eigh`` for small sectors (<= 50, exactdiag.py:31-35; extended here to
+    2000 for accuracy), otherwise ARPACK ``eigsh(k=1, which="SA")`` (37).
+    """
+    if op.size <= 2000:
+        h = op.csr().toarray() if hasattr(op, "csr") else np.asarray(op)
+        w, v = np.linalg.eigh(h)
+        return float(w[0]), v[:, 0]
+    lin = sla.LinearOperator((op.size, op.size), matvec=op.matvec, dtype=np.float64)
+    w, v = sla.eigsh(lin, k=1, which="SA", tol=tol, v0=v0)
+    return float(w[0]), v[:, 0]
+
+
+def lanczos_coeffs_reference(matvec, psi, size):
+    """The reference's unnormalised recurrence (cmpy/exactdiag.py:324-358) with a
+    caller-supplied start vector instead of the unseeded ``np.random`` draw (326).
+
+    a_n = <p|Hp>/<p|p>,  b_n^2 = |p_n|^2/|p_{n-1}|^2,
+    p_{n+1} = H p_n - a_n p_n - b_n^2 p_{n-1}.  Returns (a[size], b[size-1]).
+    """
+    psi = np.asarray(psi, dtype=np.float64)
+    a_list, b_list = [], []
+    hp = matvec(psi)
+    a = np.dot(psi, hp) / np.dot(psi, psi)
+    a_list.append(a)
+    psi_new = hp - a * psi
+    psi_prev, psi = psi, psi_new
+    for _ in range(1, size):
+        hp = matvec(psi)
+        a = np.dot(psi, hp) / np.dot(psi, psi)
+        b2 = np.dot(psi, psi) / np.dot(psi_prev, psi_prev)
+        psi_new = hp - a * psi - b2 * psi_prev
+        a_list.append(a)
+        b_list.append(np.sqrt(b2))
+        psi_prev, psi = psi, psi_new
+    return np.array(a_list), np.array(b_list)
+
+
+def lanczos_ground_state(a_coeffs, b_coeffs, max_eig=3):
+    """cmpy/exactdiag.py:368-375 (``eigh_tridiagonal`` with ``select="i"``)."""
+    max_eig = min(max_eig, len(a_coeffs) - 1)
+    xi, vi = la.eigh_tridiagonal(a_coeffs, b_coeffs, select="i", select_range=(0, max_eig))
+    idx = np.argmin(xi)
+    return xi[idx], vi[:, idx]
+
+
+def lanczos_normalized(matvec, v0, nsteps, reorth=False):
+    """Normalised Lanczos, one matvec per step.  Same coefficients as
+    :func:`lanczos_coeffs_reference` in exact arithmetic.
+
+    Returns (alpha[m], beta[m-1], norm2, V) where V holds the m basis vectors
+    (columns) and norm2 = |v0|^2.  Stops early on an invariant subspace.
+    """
+    v0 = np.asarray(v0, dtype=np.float64)
+    norm2 = float(np.dot(v0, v0))
+    v = v0 / np.sqrt(norm2)
+    vs = [v]
+    alpha, beta = [], []
+    v_prev = np.zeros_like(v)
+    b = 0.0
+    for j in range(nsteps):
+        w = matvec(v) - b * v_prev
+        a = float(np.dot(w, v))
+        w = w - a * v
+        if reorth:
+            V = np.array(vs)
+            w = w - V.T @ (V @ w)
+            w = w - V.T @ (V @ w)
+        alpha.append(a)
+        b = float(np.linalg.norm(w))
+        if j == nsteps - 1 or b < 1e-14 * max(1.0, abs(a)):
+            break
+        beta.append(b)
+        v_prev, v = v, w / b
+        vs.append(v)
+    return np.array(alpha), np.array(beta), norm2, np.array(vs).T
+
+
+def cf_greens(alpha, beta, norm2, z, shift, sign=+1):
+    """Continued fraction  s * norm2 / (s z + shift - a_0 - b_1^2/(s z + shift - a_1 - ...)).
+
+    Particle part: sign=+1, shift=E0 (sector N+1).  Hole part: sign=-1, shift=E0
+    (sector N-1), which equals  norm2/(z - E0 + a_0 - b_1^2/(z - E0 + a_1 - ...)).
+    No reference function: the reference has no continued fraction (pinned by
+    equality with :func:`lehmann_t0` in the test-suite).
+    """
+    z = np.asarray(z, dtype=np.complex128)
+    w = sign * z + shift
+    m = len(alpha)
+    g = np.zeros_like(w)
+    for k in range(m - 1, 0, -1):
+        g = beta[k - 1] ** 2 / (w - alpha[k] - g)
+    return sign * norm2 / (w - alpha[0] - g)
+
+
+# =============================================================================
+# Green's functions
+# =============================================================================
+
+
+def lehmann_t0(op_n, op_p1, op_m1, psi0, e0, cdag, c, z):
+    """T = 0, fixed-sector Lehmann sum from the reference's own pieces.
+
+    ``_accumulate_sum`` (cmpy/exactdiag.py:110-129) restricted to n = ground state:
+    G(z) = sum_m |<m|c^+|0>|^2/(z - E_m + E0) + sum_n |<n|c|0>|^2/(z + E_n - E0).
+    ``cdag``/``c`` map psi0 to the upper/lower sector (None if that sector does
+    not exist); the sectors are diagonalised densely (exactdiag.py:46-57).
+    """
+    z = np.asarray(z, dtype=np.complex128)
+    g = np.zeros_like(z)
+    if op_p1 is not None:
+        h = op_p1.csr().toarray()
+        w, v = np.linalg.eigh(h)
+        ov = np.abs(v.T @ cdag) ** 2
+        g += (ov[None, :] / (z[:, None] - w[None, :] + e0)).sum(axis=1)
+    if op_m1 is not None:
+        h = op_m1.csr().toarray()
+        w, v = np.linalg.eigh(h)
+        ov = np.abs(v.T @ c) ** 2
+        g += (ov[None, :] / (z[:, None] + w[None, :] - e0)).sum(axis=1)
+    return g
+
+
+def occupation(up_states, dn_states, evals, evecs, beta, emin=0.0, pos=0, sigma=UP):
+    """cmpy/exactdiag.py:60-92."""
+    up = np.asarray(up_states, np.int64)
+    dn = np.asarray(dn_states, np.int64)
+    num_dn = len(dn)
+    boltz = np.exp(-beta * (evals - emin))
+    occ = 0.0
+    if sigma == UP:
+        for up_idx in range(len(up)):
+            if up[up_idx] & (1 << pos):
+                idx = up_idx * num_dn + np.arange(num_dn)
+                ov = np.sum(np.abs(evecs[idx, :]) ** 2, axis=0)
+                occ += np.sum(boltz * ov)
+    else:
+        for dn_idx in range(num_dn):
+            if dn[dn_idx] & (1 << pos):
+                idx = np.arange(len(up)) * num_dn + dn_idx
+                ov = np.sum(np.abs(evecs[idx, :]) ** 2, axis=0)
+                occ += np.sum(boltz * ov)
+    return occ
+
+
+def double_occupation(up_states, dn_states, evals, evecs, beta, emin=0.0, pos=0):
+    """cmpy/exactdiag.py:95-107, with the flat index computed as
+    up_idx * num_dn + dn_idx (the reference's running ``idx`` is only correct
+    when the prange loop runs serially)."""
+    up = np.asarray(up_states, np.int64)
+    dn = np.asarray(dn_states, np.int64)
+    num_dn = len(dn)
+    boltz = np.exp(-beta * (evals - emin))
+    occ = 0.0
+    for up_idx in range(len(up)):
+        for dn_idx in range(num_dn):
+            if up[up_idx] & dn[dn_idx] & (1 << pos):
+                ov = np.abs(evecs[up_idx * num_dn + dn_idx, :]) ** 2
+                occ += np.sum(boltz * ov)
+    return occ
+
+
+def _accumulate_sum(gf, z, evals, evals_p1, evecs_p1, cdag_evec, beta, emin):
+    """cmpy/exactdiag.py:110-129 (vectorised over m and n)."""
+    overlap = np.abs(evecs_p1.T.conj() @ cdag_evec) ** 2
+    if np.isfinite(beta):
+        exp_evals = np.exp(-beta * (evals - emin))
+        exp_evals_p1 = np.exp(-beta * (evals_p1 - emin))
+    else:
+        exp_evals = np.ones_like(evals)
+        exp_evals_p1 = np.ones_like(evals_p1)
+    weights = overlap * (exp_evals[None, :] + exp_evals_p1[:, None])
+    poles = evals_p1[:, None] - evals[None, :]
+    gf += (weights[None, :, :] / (z[:, None, None] - poles[None, :, :])).sum(axis=(1, 2))
+
+
+class GreensFunctionMeasurement:
+    """cmpy/exactdiag.py:137-212."""
+
+    def __init__(self, z, beta, pos=0, sigma=UP, measure_occ=True):
+        self.z = np.asarray(z)
+        self.beta = beta
+        self.pos = pos
+        self.sigma = sigma
+        self._measure_occ = measure_occ
+        self._part = 0
+        self._gs_energy = np.inf
+        self._gf = np.zeros_like(self.z, dtype=np.complex128)
+        self._occ = 0.0
+        self._occ_double = 0.0
+
+    @property
+    def part(self):
+        return self._part * np.exp(-self.beta * self._gs_energy)
+
+    @property
+    def gf(self):
+        return self._gf / self._part
+
+    @property
+    def occ(self):
+        return self._occ / self._part
+
+    @property
+    def occ_double(self):
+        return self._occ_double / self._part
+
+    @property
+    def gs_energy(self):
+        return self._gs_energy
+
+    def accumulate(self, num_sites, sector, sector_p1, evals, evecs, evals_p1, evecs_p1):
+        n_up, n_dn = sector
+        min_energy = min(evals)
+        factor = 1.0
+        if min_energy < self._gs_energy:
+            factor = np.exp(-self.beta * (self._gs_energy - min_energy))
+            self._gs_energy = min_energy
+        up = sector_states(num_sites, n_up)
+        dn = sector_states(num_sites, n_dn)
+        # _acc_part (171-173)
+        self._part *= factor
+        self._part += np.sum(np.exp(-self.beta * (evals - self._gs_energy)))
+        # _acc_gf (175-183)
+        if factor != 1.0:
+            self._gf *= factor
+        cdag_evec = np.column_stack([
+            apply_cdag(evecs[:, k], num_sites, n_up, n_dn, self.pos, self.sigma)
+            for k in range(evecs.shape[1])
+        ])
+        _accumulate_sum(self._gf, self.z, evals, evals_p1, evecs_p1, cdag_evec,
+                        self.beta, self._gs_energy)
+        if self._measure_occ:
+            self._occ *= factor
+            self._occ += occupation(up, dn, evals, evecs, self.beta, self._gs_energy,
+                                    self.pos, self.sigma)
+            self._occ_double *= factor
+            self._occ_double += double_occupation(up, dn, evals, evecs, self.beta,
+                                                  self._gs_energy, self.pos)
+
+
+def gf_lehmann(make_op, num_sites, z, beta, pos=0, sigma=UP, occ=True):
+    """Grand-canonical finite-T Lehmann Green's function (cmpy/exactdiag.py:215-245).
+
+    ``make_op(n_up, n_dn)`` returns a :class:`SectorOperator` of the model.
+    Fillings are visited in ``product(range(N+1), repeat=2)`` order
+    (cmpy/basis.py:683-684); sectors with no upper sector are skipped.
+    """
+    data = GreensFunctionMeasurement(z, beta, pos, sigma, measure_occ=occ)
+    cache = {}
+
+    def solve(nu, nd):
+        if (nu, nd) not in cache:
+            h = make_op(nu, nd).csr().toarray()
+            cache[(nu, nd)] = np.linalg.eigh(h)
+        return cache[(nu, nd)]
+
+    for n_up, n_dn in itertools.product(range(num_sites + 1), repeat=2):
+        p1 = upper_sector(n_up, n_dn, sigma, num_sites)
+        if p1 is None:
+            continue
+        evals, evecs = solve(n_up, n_dn)
+        evals_p1, evecs_p1 = solve(*p1)
+        data.accumulate(num_sites, (n_up, n_dn), p1, evals, evecs, evals_p1, evecs_p1)
+    return data
+
+
+def gf0_lehmann(ham, z, mu=0.0, mode="diag"):
+    """cmpy/greens.py:18-64 (einsum ``"ij,...j,ji->...i"`` for mode="diag").
+
+    As written, the diag mode sums |V[j, i]|^2/(z - lambda_j); this equals the
+    diagonal Green's function whenever |V| is symmetric (tight-binding chains),
+    which is where the reference's test uses it.
+    """
+    eigvals, eigvecs = np.linalg.eigh(ham)
+    z = np.atleast_1d(z)
+    eigvecs_adj = np.conj(eigvecs).T
+    sub = {"full": "ik,...k,kj->...ij", "diag": "ij,...j,ji->...i", "total": "ij,...j,ji->..."}[mode]
+    arg = np.subtract.outer(z + mu, eigvals)
+    return np.einsum(sub, eigvecs_adj, 1 / arg, eigvecs)
+
+
+def hybridization_func(z, v, eps_bath):
+    """cmpy/models/anderson.py:127-145."""
+    x = np.asarray(z)[..., np.newaxis]
+    return np.sum(np.square(np.abs(np.atleast_1d(v))) / (x - np.atleast_1d(eps_bath)), axis=-1)
+
+
+def impurity_gf0(z, u, eps_imp, v, eps_bath, mu):
+    """cmpy/models/anderson.py:160-161."""
+    return 1 / (z + mu + eps_imp - hybridization_func(z, v, eps_bath))
+
+
+def impurity_params_ref(u, v):
+    """Two-site SIAM poles/residues at half filling, T=0 (cmpy/dmft/twosite.py:25-60)."""
+    sqrt16 = np.sqrt(u**2 + 16 * v**2)
+    sqrt64 = np.sqrt(u**2 + 64 * v**2)
+    a1 = 0.25 * (1 - (u**2 - 32 * v**2) / np.sqrt((u**2 + 64 * v**2) * (u**2 + 16 * v**2)))
+    a2 = 0.5 - a1
+    e1 = 0.25 * (sqrt64 - sqrt16)
+    e2 = 0.25 * (sqrt64 + sqrt16)
+    return float(a1), float(a2), float(e1), float(e2)
+
+
+def impurity_gf_ref(z, u, v):
+    """cmpy/dmft/twosite.py:63-81."""
+    a1, a2, e1, e2 = impurity_params_ref(u, v)
+    return (a1 / (z - e1) + a1 / (z + e1)) + (a2 / (z - e2) + a2 / (z + e2))
+
+
+# =============================================================================
+# Lattices (replacing lattpy's neighbour lists used by the reference tests)
+# =============================================================================
+
+
+def chain_bonds(num_sites, periodic=False):
+    bonds = [(i, i + 1) for i in range(num_sites - 1)]
+    if periodic and num_sites > 2:
+        bonds.append((0, num_sites - 1))
+    return bonds
+
+
+def square_bonds(nx, ny, periodic=True):
+    """Nearest-neighbour bonds (i < j) of an nx x ny square lattice, site = x + nx*y."""
+    bonds = set()
+    for y in range(ny):
+        for x in range(nx):
+            i = x + nx * y
+            for dx, dy in ((1, 0), (0, 1)):
+                xx, yy = x + dx, y + dy
+                if periodic:
+                    xx %= nx
+                    yy %= ny
+                elif xx >= nx or yy >= ny:
+                    continue
+                j = xx + nx * yy
+                if i != j:
+                    bonds.add((min(i, j), max(i, j)))
+    return sorted(bonds)
+
+
+def free_fermion_energy(num_sites, bonds, hop, eps, n_up, n_dn):
+    """U = 0 closed form: sum of the lowest n_up and n_dn single-particle levels."""
+    h = np.diag(np.full(num_sites, float(eps)))
+    for i, j in bonds:
+        h[i, j] += hop
+        h[j, i] += hop
+    w = np.linalg.eigvalsh(h)
+    return float(np.sum(w[:n_up]) + np.sum(w[:n_dn]))
+
+
+# Which check pins each function (besides retyping it):
+PARITY = {
+    "generate_states_permutations": "test_basis.py:124-141 state tables",
+    "sector_states": "generate_states_permutations for ns<=8; test_basis.py:124-141",
+    "combinadic_rank": "position in sector_states for every ns<=12",
+    "upper_sector/lower_sector": "test_basis.py:91-111 tables",
+    "state_label": "basis.py:613-631 docstring label lists",
+    "hopping_sign/hopping_terms": "operators.py:503-507 matrix; U=0 free-fermion "
+    "closed form on periodic chains and the 4x4 torus (wrong mask range or width "
+    "fails it); hand-computed sign cases",
+    "weighted_element/onsite/inter": "operators.py:337-341, 391-395 docstrings; "
+    "test_models_hubbard.py:14-25",
+    "coo_matvec/coo_dense": "test_models_hubbard.py:14-25; Hermiticity",
+    "SectorOperator": "COO assembly at small sizes; free-fermion closed form",
+    "apply_cdag/apply_c": "test_operator.py:31-42 adjointness; Lange two-site KAT",
+    "lanczos_normalized": "lanczos_coeffs_reference coefficients; dense eigh",
+    "cf_greens": "lehmann_t0 equality at small size; Lange KAT",
+    "lehmann_t0": "Lange two-site KAT (dmft/twosite.py:25-81); sum rule",
+    "gf_lehmann": "test_exactdiag.py:24-96 (U=0 vs gf0_lehmann, atomic peaks, <n>=1/2)",
+    "gf0_lehmann": "chain tight-binding resolvent computed by direct inversion",
+    "impurity_gf_ref": "lehmann_t0 on the two-site SIAM",
+}

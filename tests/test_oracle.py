@@ -1,0 +1,354 @@
+"""Pins the CPU oracle (oracle/ed_oracle.py) before anything is checked against it.
+
+Each check is independent of the oracle's own formulas: reference-held fixtures,
+closed forms, brute force and invariants chosen so that a dropped term, a wrong
+sign/index or a transposed operand fails at least one of them.
+"""
+
+import itertools
+import math
+
+import numpy as np
+import pytest
+from numpy.testing import assert_allclose, assert_array_equal
+
+from oracle import ed_oracle as O
+
+
+# -- basis ------------------------------------------------------------------------
+
+
+def test_state_tables(ref_fixtures):
+    for case in ref_fixtures["basis_get_states"]["cases"]:
+        ns, n = case["num_sites"], case["n"]
+        got = O.generate_states_permutations(ns, n)
+        assert [format(int(s), f"0{ns}b") for s in got] == case["states"]
+        got2 = O.sector_states(ns, n)
+        assert [format(int(s), f"0{ns}b") for s in got2] == case["states"]
+
+
+@pytest.mark.parametrize("ns", range(1, 9))
+def test_sector_states_match_permutation_enumeration(ns):
+    for n in range(ns + 1):
+        assert_array_equal(np.asarray(O.generate_states_permutations(ns, n)), O.sector_states(ns, n))
+
+
+@pytest.mark.parametrize("ns", range(1, 13))
+def test_combinadic_rank_is_position(ns):
+    for n in range(ns + 1):
+        st = O.sector_states(ns, n)
+        assert len(st) == math.comb(ns, n)
+        assert_array_equal(O.combinadic_rank(st), np.arange(len(st)))
+
+
+def test_upper_lower_sector_tables():
+    # cmpy/tests/test_basis.py:91-111
+    ns = 15
+    for n_up, n_dn in itertools.product(range(15), range(15)):
+        assert O.upper_sector(n_up, n_dn, O.UP, ns) == (None if n_up == ns else (n_up + 1, n_dn))
+        assert O.upper_sector(n_up, n_dn, O.DN, ns) == (None if n_dn == ns else (n_up, n_dn + 1))
+        assert O.lower_sector(n_up, n_dn, O.UP) == (None if n_up == 0 else (n_up - 1, n_dn))
+        assert O.lower_sector(n_up, n_dn, O.DN) == (None if n_dn == 0 else (n_up, n_dn - 1))
+
+
+def test_state_labels(ref_fixtures):
+    for case in ref_fixtures["state_labels"]["cases"]:
+        ns = case["num_sites"]
+        up = O.sector_states(ns, case["n_up"])
+        dn = O.sector_states(ns, case["n_dn"])
+        labels = [O.state_label(int(u), int(d), ns) for u in up for d in dn]
+        assert labels == case["labels"]
+
+
+# -- signs and projectors ---------------------------------------------------------
+
+
+def test_hopping_sign_hand_cases():
+    # (state, width, site1, site2) -> sign, counted by hand
+    cases = [
+        (0b0000, 4, 0, 3, 1), (0b0110, 4, 0, 3, 1), (0b0010, 4, 0, 3, -1), (0b0100, 4, 0, 3, -1),
+        (0b1111, 4, 0, 1, 1), (0b0111, 4, 0, 2, -1), (0b0111, 0, 0, 2, 1), (0b0111, 1, 0, 2, 1),
+        (0b0111, 2, 0, 2, -1), (0b11110, 5, 0, 4, -1), (0b01010, 5, 0, 4, 1),
+        (0b10110, 5, 1, 4, -1), (0b10110, 5, 0, 3, 1), (0b10110, 3, 0, 4, 1),
+        (0b11110, 3, 0, 4, 1), (0b11110, 4, 0, 4, -1),
+    ]
+    for s, w, i, j, sign in cases:
+        assert O.hopping_sign(s, w, i, j) == sign, (bin(s), w, i, j)
+
+
+def test_hopping_terms_vectorised_matches_loop():
+    # the vectorised hopping_terms against a direct loop over cmpy's per-state rule
+    for ns, n, width in [(5, 2, 5), (6, 3, 6), (6, 3, 0), (7, 3, 4)]:
+        st = O.sector_states(ns, n)
+        for i, j in itertools.combinations(range(ns), 2):
+            o, t, a = O.hopping_terms(st, width, i, j, 0.7)
+            ref = []
+            for k, s in enumerate(st):
+                s = int(s)
+                if bool(s >> i & 1) != bool(s >> j & 1):
+                    new = s ^ (1 << i) ^ (1 << j)
+                    ref.append((k, int(np.searchsorted(st, new)), O.hopping_sign(s, width, i, j) * 0.7))
+            assert [(int(x), int(y), float(z)) for x, y, z in zip(o, t, a)] == ref
+
+
+def _dense(triples, size):
+    return O.coo_dense(size, *triples)
+
+
+def test_docstring_matrices(ref_fixtures):
+    f = ref_fixtures["hubbard_inter_2site"]
+    up = dn = O.sector_states(2, 1)
+    assert_array_equal(_dense(O.hubbard_inter_triples(up, dn, f["u"]), 4), f["matrix"])
+    f = ref_fixtures["onsite_energy_2site"]
+    assert_array_equal(_dense(O.onsite_energy_triples(up, dn, f["eps"]), 4), f["matrix"])
+    f = ref_fixtures["hopping_2site"]
+    assert_array_equal(_dense(O.hopping_triples(up, dn, 2, 0, 1, f["hop"]), 4), f["matrix"])
+
+
+def test_hubbard_sector_1_1(ref_fixtures):
+    f = ref_fixtures["hubbard_sector_1_1"]
+    up = dn = O.sector_states(2, 1)
+    tr = O.hubbard_triples(up, dn, 2, f["neighbors"], f["inter"], f["eps"], f["hop"])
+    assert_array_equal(_dense(tr, 4), f["matrix"])
+
+
+def test_project_up_dn(ref_fixtures):
+    f = ref_fixtures["project_up"]
+    for up_idx, exp in f["cases"]:
+        assert_array_equal(up_idx * f["num_dn"] + np.asarray(f["dn_indices"]), exp)
+    f = ref_fixtures["project_dn"]
+    for dn_idx, exp in f["cases"]:
+        assert_array_equal(np.asarray(f["up_indices"]) * f["num_dn"] + dn_idx, exp)
+
+
+@pytest.mark.parametrize("ns", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("u", [0.0, 1.0, 2.0])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_hubbard_hermitian(ns, u, periodic):
+    # cmpy/tests/test_models_hubbard.py:28-48
+    bonds = O.chain_bonds(ns, periodic)
+    for n_up, n_dn in itertools.product(range(ns + 1), repeat=2):
+        up, dn = O.sector_states(ns, n_up), O.sector_states(ns, n_dn)
+        h = _dense(O.hubbard_triples(up, dn, ns, bonds, u, 0.0, 1.0, mu=u / 2), len(up) * len(dn))
+        assert_allclose(h, h.T, atol=0)
+
+
+@pytest.mark.parametrize("geom", ["chain6p", "ring5", "sq4x2", "sq2x2"])
+@pytest.mark.parametrize("filling", [(1, 2), (2, 2), (3, 2)])
+def test_free_fermion_closed_form(geom, filling):
+    """U = 0: E0 = sum of the lowest single-particle levels.  Only correct hopping
+    signs (mask between the sites, width = num_sites) reproduce it on rings/tori."""
+    ns, bonds = {
+        "chain6p": (6, O.chain_bonds(6, True)), "ring5": (5, O.chain_bonds(5, True)),
+        "sq4x2": (8, O.square_bonds(4, 2, True)), "sq2x2": (4, O.square_bonds(2, 2, False)),
+    }[geom]
+    n_up, n_dn = filling
+    op = O.hubbard_operator(ns, n_up, n_dn, bonds, 0.0, 0.3, 1.0)
+    e0, _ = O.groundstate(op)
+    assert abs(e0 - O.free_fermion_energy(ns, bonds, 1.0, 0.3, n_up, n_dn)) < 1e-10
+
+
+def test_sign_width_matters_on_a_ring():
+    # Dropping the sign (width 0) breaks the closed form on the 4x2 torus at (3,3).
+    ns, bonds = 8, O.square_bonds(4, 2, True)
+    bad = O.SectorOperator(ns, 3, 3, bonds, [1.0] * len(bonds), np.zeros(ns), np.zeros(ns), 0)
+    e_bad, _ = O.groundstate(bad)
+    assert abs(e_bad - O.free_fermion_energy(ns, bonds, 1.0, 0.0, 3, 3)) > 1e-3
+
+
+def test_atomic_limit_closed_form():
+    # t = 0: E0 = min over configurations; at half filling with mu = U/2 it is -U/2 * N
+    ns, u = 4, 3.0
+    op = O.hubbard_operator(ns, 2, 2, [], u, 0.0, 0.0, mu=u / 2)
+    e0, _ = O.groundstate(op)
+    assert abs(e0 - (-u / 2 * ns)) < 1e-12
+
+
+@pytest.mark.parametrize("case", [("hub", 5, 2, 3), ("hub", 4, 2, 2), ("siam", 0, 3, 3), ("siam", 0, 2, 3)])
+def test_factorised_operator_matches_coo(case):
+    kind, ns, n_up, n_dn = case
+    rng = np.random.default_rng(7)
+    if kind == "hub":
+        bonds = O.chain_bonds(ns, True)
+        op = O.hubbard_operator(ns, n_up, n_dn, bonds, 2.0, 0.3, 1.0, mu=1.0)
+        up, dn = op.up, op.dn
+        tr = O.hubbard_triples(up, dn, ns, bonds, 2.0, 0.3, 1.0, mu=1.0)
+    else:
+        eps, v = rng.uniform(-2, 2, 5), rng.uniform(0.2, 1.0, 5)
+        op = O.siam_operator(n_up, n_dn, 4.0, -2.0, eps, v, 0.0)
+        tr = O.siam_triples(op.up, op.dn, 4.0, -2.0, eps, v, 0.0)
+    h = _dense(tr, op.size)
+    assert_allclose(op.csr().toarray(), h, atol=1e-14)
+    x = rng.standard_normal(op.size)
+    assert_allclose(op.matvec(x), O.coo_matvec(op.size, *tr, x), atol=1e-12)
+    assert_allclose(op.diagonal().sum(), O.coo_trace(*tr), atol=1e-12)
+
+
+# -- ground state and Lanczos ------------------------------------------------------
+
+
+def _cfg1(seed):
+    rng = np.random.default_rng(seed)
+    eps = rng.uniform(-2, 2, 5)
+    v = rng.uniform(0.2, 1.0, 5)
+    return eps, v
+
+
+def test_siam_reference_convention_energies():
+    """Both sign conventions of the 1+5-site SIAM at cfg-1 parameters (seed 0):
+    the reference's width-0 signs and fermionic signs give different E0."""
+    eps, v = _cfg1(0)
+    e_ref, _ = O.groundstate(O.siam_operator(3, 3, 4.0, -2.0, eps, v, 0.0, width=0))
+    e_fer, _ = O.groundstate(O.siam_operator(3, 3, 4.0, -2.0, eps, v, 0.0, width=6))
+    assert abs(e_ref - (-12.287544399997)) < 1e-9
+    assert abs(e_fer - (-12.234494553501)) < 1e-9
+
+
+def test_siam_u0_fermion_signs_closed_form():
+    eps, v = _cfg1(3)
+    op = O.siam_operator(3, 3, 0.0, 0.0, eps, v, 0.0, width=6)
+    e0, _ = O.groundstate(op)
+    h = np.diag(np.append(0.0, eps))
+    h[0, 1:] = h[1:, 0] = v
+    assert abs(e0 - 2 * np.linalg.eigvalsh(h)[:3].sum()) < 1e-10
+
+
+def test_lanczos_normalized_matches_reference_recurrence():
+    eps, v = _cfg1(1)
+    op = O.siam_operator(3, 3, 4.0, -2.0, eps, v, 0.0)
+    psi = np.random.default_rng(2).uniform(0, 1, op.size)
+    a_ref, b_ref = O.lanczos_coeffs_reference(op.matvec, psi, 25)
+    a, b, n2, _ = O.lanczos_normalized(op.matvec, psi, 25)
+    assert_allclose(a, a_ref, rtol=1e-9, atol=1e-9)
+    assert_allclose(b, b_ref, rtol=1e-9, atol=1e-9)
+    assert abs(n2 - psi @ psi) < 1e-9 * n2
+
+
+def test_lanczos_tridiagonal_gives_e0():
+    eps, v = _cfg1(2)
+    op = O.siam_operator(3, 3, 4.0, -2.0, eps, v, 0.0)
+    e_exact = np.linalg.eigvalsh(op.csr().toarray())[0]
+    a, b, _, _ = O.lanczos_normalized(op.matvec, np.ones(op.size), 120, reorth=True)
+    e, _ = O.lanczos_ground_state(a, b)
+    assert abs(e - e_exact) < 1e-10
+
+
+# -- Green's functions ---------------------------------------------------------------
+
+
+@pytest.mark.parametrize("u,v", [(2.0, 1.0), (4.0, 0.5), (6.0, 0.25)])
+def test_lange_two_site_kat(u, v):
+    """T = 0 Lehmann of the 2-site SIAM equals the closed form (dmft/twosite.py:25-81)."""
+    opn = O.siam_operator(1, 1, u, 0.0, [0.0], [v], u / 2)
+    e0, psi = O.groundstate(opn)
+    opp = O.siam_operator(2, 1, u, 0.0, [0.0], [v], u / 2)
+    opm = O.siam_operator(0, 1, u, 0.0, [0.0], [v], u / 2)
+    cd = O.apply_cdag(psi, 2, 1, 1, 0, O.UP)
+    cc = O.apply_c(psi, 2, 1, 1, 0, O.UP)
+    z = np.linspace(-5, 5, 201) + 0.05j
+    g = O.lehmann_t0(opn, opp, opm, psi, e0, cd, cc, z)
+    assert_allclose(g, O.impurity_gf_ref(z, u, v), atol=1e-12)
+
+
+def _siam_g_pieces(seed, width=0, nsteps=300):
+    eps, v = _cfg1(seed)
+    mk = lambda nu, nd: O.siam_operator(nu, nd, 4.0, -2.0, eps, v, 0.0, width=width)
+    op = mk(3, 3)
+    e0, psi = O.groundstate(op)
+    mode = "jordan_wigner" if width else "reference"
+    cd = O.apply_cdag(psi, 6, 3, 3, 0, O.UP, mode)
+    cc = O.apply_c(psi, 6, 3, 3, 0, O.UP, mode)
+    return mk, op, e0, psi, cd, cc
+
+
+@pytest.mark.parametrize("width", [0, 6])
+def test_cf_equals_lehmann_small(width):
+    mk, op, e0, psi, cd, cc = _siam_g_pieces(4, width)
+    z = np.linspace(-8, 8, 1000) + 0.05j
+    g_l = O.lehmann_t0(op, mk(4, 3), mk(2, 3), psi, e0, cd, cc, z)
+    ap, bp, np_, _ = O.lanczos_normalized(mk(4, 3).matvec, cd, 300, reorth=True)
+    ah, bh, nh, _ = O.lanczos_normalized(mk(2, 3).matvec, cc, 300, reorth=True)
+    g_cf = O.cf_greens(ap, bp, np_, z, e0, +1) + O.cf_greens(ah, bh, nh, z, e0, -1)
+    assert_allclose(g_cf, g_l, atol=1e-8)
+
+
+def test_lehmann_sum_rule_and_tail():
+    mk, op, e0, psi, cd, cc = _siam_g_pieces(5)
+    # total weight of c and c+ is <{c, c+}> = 1 at pos 0
+    assert abs(cd @ cd + cc @ cc - 1.0) < 1e-12
+    z = np.array([1e6j])
+    g = O.lehmann_t0(op, mk(4, 3), mk(2, 3), psi, e0, cd, cc, z)
+    assert abs(g[0] * z[0] - 1.0) < 1e-6
+
+
+@pytest.mark.parametrize("ns", [2, 3, 4])
+def test_cdag_c_adjoint(ns):
+    # cmpy/tests/test_operator.py:31-42
+    for n_up, n_dn in itertools.product(range(ns + 1), repeat=2):
+        for sigma in (O.UP, O.DN):
+            p1 = O.upper_sector(n_up, n_dn, sigma, ns)
+            if p1 is None:
+                continue
+            D = math.comb(ns, n_up) * math.comb(ns, n_dn)
+            D1 = math.comb(ns, p1[0]) * math.comb(ns, p1[1])
+            for pos in range(ns):
+                for mode in ("reference", "jordan_wigner"):
+                    cd = np.column_stack([O.apply_cdag(e, ns, n_up, n_dn, pos, sigma, mode) for e in np.eye(D)])
+                    c = np.column_stack([O.apply_c(e, ns, *p1, pos, sigma, mode) for e in np.eye(D1)])
+                    assert_array_equal(c, cd.T)
+
+
+def test_jordan_wigner_anticommutation():
+    """With JW signs {c_i, c+_j} = delta_ij on the up species (fails without them)."""
+    ns = 3
+    for i, j in itertools.product(range(ns), repeat=2):
+        for n_up in range(1, ns):
+            n_dn = 1
+            D = math.comb(ns, n_up) * math.comb(ns, n_dn)
+            M = np.zeros((D, D))
+            for k, e in enumerate(np.eye(D)):
+                a = O.apply_c(O.apply_cdag(e, ns, n_up, n_dn, j, O.UP, "jordan_wigner"), ns, n_up + 1, n_dn, i, O.UP, "jordan_wigner")
+                b = O.apply_cdag(O.apply_c(e, ns, n_up, n_dn, i, O.UP, "jordan_wigner"), ns, n_up - 1, n_dn, j, O.UP, "jordan_wigner")
+                M[:, k] = a + b
+            assert_allclose(M, np.eye(D) * (i == j), atol=0)
+
+
+# -- the reference's Lehmann tests (test_exactdiag.py), restated on the oracle ---------
+
+
+def _chain_tb(ns, eps=0.0, hop=1.0):
+    h = np.diag(np.full(ns, eps))
+    for i, j in O.chain_bonds(ns):
+        h[i, j] = h[j, i] = hop
+    return h
+
+
+@pytest.mark.parametrize("ns", [2, 3, 4])
+def test_gf_lehmann_non_interacting(ns, ref_fixtures):
+    f = ref_fixtures["gf_lehmann_tests"]["non_interacting"]
+    z = np.linspace(f["z_min"], f["z_max"], f["nz"]) + 1j * f["eta"]
+    mk = lambda nu, nd: O.hubbard_operator(ns, nu, nd, O.chain_bonds(ns), 0.0, 0.0, 1.0)
+    g = O.gf_lehmann(mk, ns, z, f["beta"], pos=0, occ=False).gf
+    g0 = O.gf0_lehmann(_chain_tb(ns), z)[:, 0]
+    assert_allclose(g, g0, rtol=f["rtol"], atol=f["atol"])
+    # gf0_lehmann equals the directly inverted resolvent on chains
+    h = _chain_tb(ns)
+    direct = np.array([np.linalg.inv(zz * np.eye(ns) - h)[0, 0] for zz in z[::50]])
+    assert_allclose(g0[::50], direct, atol=1e-12)
+
+
+@pytest.mark.parametrize("ns", [2, 3])
+@pytest.mark.parametrize("u", [1.0, 4.0])
+def test_gf_lehmann_atomic_and_occupation(ns, u, ref_fixtures):
+    f = ref_fixtures["gf_lehmann_tests"]["atomic_limit"]
+    z = np.linspace(f["z_min"], f["z_max"], f["nz"]) + 1j * f["eta"]
+    mk = lambda nu, nd: O.hubbard_operator(ns, nu, nd, O.chain_bonds(ns), u, 0.0, 0.0, mu=u / 2)
+    for pos in range(ns):
+        g = O.gf_lehmann(mk, ns, z, f["beta"], pos=pos, occ=False).gf
+        c = len(z) // 2
+        e0 = z.real[:c][np.argmax(-g.imag[:c])]
+        e1 = z.real[c:][np.argmax(-g.imag[c:])]
+        assert abs(e0 + u / 2) < f["peak_tol"] and abs(e1 - u / 2) < f["peak_tol"]
+    mk = lambda nu, nd: O.hubbard_operator(ns, nu, nd, O.chain_bonds(ns), u, 0.0, 1.0, mu=u / 2)
+    m = O.gf_lehmann(mk, ns, z[:5], 10.0, pos=0, occ=True)
+    assert abs(m.occ - 0.5) < 1e-3
