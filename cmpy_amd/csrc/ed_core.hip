@@ -9,6 +9,7 @@
 //   project_hubbard_inter                     cmpy/operators.py:305-356
 //   AbstractManyBodyModel.hamilton_operator   cmpy/models/abc.py:242-256
 #include "ed_internal.h"
+#include "ed_colouring.h"
 
 #include <algorithm>
 #include <cstring>
@@ -68,7 +69,8 @@ __global__ void k_hop_terms(int64_t count, int num_sites, int n, int width, int 
 // ELL hop table of one spin sector over all bonds; both layouts.
 __global__ void k_hop_table(const uint32_t* states, int64_t count, int width, const int32_t* bonds,
                             const double* hops, int nb, int32_t* cnt, int32_t* col_rm,
-                            double* val_rm, int32_t* col_cm, double* val_cm) {
+                            double* val_rm, int32_t* col_cm, double* val_cm, uint32_t* pk_rm,
+                            uint32_t* pk_cm) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const uint32_t s = states[i];
@@ -79,11 +81,16 @@ __global__ void k_hop_table(const uint32_t* states, int64_t count, int width, co
     const bool occ1 = (s & op1) != 0, occ2 = (s & op2) != 0;
     if (occ1 == occ2) continue;
     const int32_t t = (int32_t)comb_rank(s ^ op1 ^ op2);
-    const double v = (double)hop_sign(s, width, s1, s2) * hops[b];
+    const int sg = hop_sign(s, width, s1, s2);
+    const double v = (double)sg * hops[b];
     col_rm[i * nb + k] = t;
     val_rm[i * nb + k] = v;
     col_cm[(int64_t)k * count + i] = t;
     val_cm[(int64_t)k * count + i] = v;
+    if (pk_rm) {
+      pk_rm[i * nb + k] = pack_hop(t, b, sg);
+      pk_cm[(int64_t)k * count + i] = pack_hop(t, b, sg);
+    }
     ++k;
   }
   for (int kk = k; kk < nb; ++kk) {
@@ -91,8 +98,79 @@ __global__ void k_hop_table(const uint32_t* states, int64_t count, int width, co
     val_rm[i * nb + kk] = 0.0;
     col_cm[(int64_t)kk * count + i] = 0;
     val_cm[(int64_t)kk * count + i] = 0.0;
+    if (pk_rm) {
+      pk_rm[i * nb + kk] = 0;
+      pk_cm[(int64_t)kk * count + i] = 0;
+    }
   }
   cnt[i] = k;
+}
+
+// Counts of each state's hops with sign +1 and -1 (packed column-major table).
+__global__ void k_sign_split(const uint32_t* pk_cm, const int32_t* cnt, int64_t count, int32_t* np,
+                             int32_t* nn) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  int p = 0, m = 0;
+  for (int k = 0; k < cnt[i]; ++k) {
+    if (pk_cm[(int64_t)k * count + i] & 0x80000000u)
+      ++m;
+    else
+      ++p;
+  }
+  np[i] = p;
+  nn[i] = m;
+}
+
+// Per 64-state chunk: the largest positive and negative count.
+__global__ void k_chunk_max(const int32_t* np, const int32_t* nn, int64_t count, int32_t* cmp,
+                            int32_t* cmn) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (count + 63) / 64) return;
+  int mp = 0, mn = 0;
+  for (int64_t i = c * 64; i < count && i < (c + 1) * 64; ++i) {
+    mp = max(mp, (int)np[i]);
+    mn = max(mn, (int)nn[i]);
+  }
+  cmp[c] = mp;
+  cmn[c] = mn;
+}
+
+// Sign-split byte-offset tables (see Tables::off_pos), bond order kept in each.
+__global__ void k_offset_tables(const uint32_t* pk_cm, const int32_t* cnt, int64_t count, int hp, int hn,
+                                uint32_t* off_pos, uint32_t* off_neg) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t pad = (uint32_t)(count * 8);
+  int p = 0, m = 0;
+  for (int k = 0; k < cnt[i]; ++k) {
+    const uint32_t e = pk_cm[(int64_t)k * count + i];
+    const uint32_t off = (uint32_t)hop_target(e) * 8u;
+    if (e & 0x80000000u)
+      off_neg[(int64_t)(m++) * count + i] = off;
+    else
+      off_pos[(int64_t)(p++) * count + i] = off;
+  }
+  for (; p < hp; ++p) off_pos[(int64_t)p * count + i] = pad;
+  for (; m < hn; ++m) off_neg[(int64_t)m * count + i] = pad;
+}
+
+// Sign-sorted target lists (see Tables::tgt_sorted), rows of `stride` entries.
+__global__ void k_sorted_targets(const uint32_t* pk_rm, const int32_t* cnt, int64_t count, int width, int stride,
+                                 int32_t* tgt, uint32_t* cnt_pn) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t* row = pk_rm + i * width;
+  int32_t* out = tgt + i * stride;
+  const int c = cnt[i];
+  int k = 0;
+  for (int j = 0; j < c; ++j)
+    if (!(row[j] & 0x80000000u)) out[k++] = hop_target(row[j]);
+  const int p = k;
+  for (int j = 0; j < c; ++j)
+    if (row[j] & 0x80000000u) out[k++] = hop_target(row[j]);
+  for (; k < stride; ++k) out[k] = 0;
+  cnt_pn[i] = (uint32_t)p | ((uint32_t)(c - p) << 16);
 }
 
 // Per-spin diagonal: sum_i eps_i over set bits, in ascending site order
@@ -183,6 +261,16 @@ static void free_tables(Tables& t) {
   if (t.val_rm) (void)hipFree(t.val_rm);
   if (t.col_cm) (void)hipFree(t.col_cm);
   if (t.val_cm) (void)hipFree(t.val_cm);
+  if (t.pk_rm) (void)hipFree(t.pk_rm);
+  if (t.pk_cm) (void)hipFree(t.pk_cm);
+  if (t.off_pos) (void)hipFree(t.off_pos);
+  if (t.off_neg) (void)hipFree(t.off_neg);
+  if (t.h16) (void)hipFree(t.h16);
+  if (t.hwords_chunk) (void)hipFree(t.hwords_chunk);
+  if (t.cmax_pos) (void)hipFree(t.cmax_pos);
+  if (t.cmax_neg) (void)hipFree(t.cmax_neg);
+  if (t.tgt_sorted) (void)hipFree(t.tgt_sorted);
+  if (t.cnt_pn) (void)hipFree(t.cnt_pn);
   t = Tables{};
 }
 
@@ -193,6 +281,7 @@ static void free_op(ed_op* op) {
   if (op->d_up) (void)hipFree(op->d_up);
   if (op->d_dn) (void)hipFree(op->d_dn);
   if (op->u_site) (void)hipFree(op->u_site);
+  if (op->hop_dev) (void)hipFree(op->hop_dev);
   free_tables(op->up);
   free_tables(op->dn);
   delete op;
@@ -211,6 +300,70 @@ static int dev_alloc(T** p, int64_t count) {
   return ED_OK;
 }
 
+
+// Builds the half-width table (see Tables::h16) on the host from the packed
+// column-major table and uploads it.
+static int build_half_tables(Tables& tab, int64_t count, int width, hipStream_t s) {
+  constexpr int L = kColourLanes;
+  std::vector<uint32_t> pk((size_t)count * width);
+  std::vector<int32_t> cnt(count);
+  ED_HIP(hipMemcpyAsync(pk.data(), tab.pk_cm, sizeof(uint32_t) * pk.size(), hipMemcpyDeviceToHost, s));
+  ED_HIP(hipMemcpyAsync(cnt.data(), tab.cnt, sizeof(int32_t) * count, hipMemcpyDeviceToHost, s));
+  ED_HIP(hipStreamSynchronize(s));
+  const int64_t nchunk = (count + 63) / 64;
+  // colourings per sign and half-wave: [chunk][half]
+  std::vector<ChunkColouring> cp(2 * nchunk), cn(2 * nchunk);
+  std::vector<int32_t> words(2 * nchunk);  // per chunk: positive words, total words
+  int maxw = 0;
+  for (int64_t c = 0; c < nchunk; ++c) {
+    for (int h = 0; h < 2; ++h) {
+      std::vector<std::vector<int32_t>> lp(L), ln(L);
+      for (int l = 0; l < L; ++l) {
+        const int64_t i = c * 64 + h * L + l;
+        if (i >= count) break;
+        for (int k = 0; k < cnt[i]; ++k) {
+          const uint32_t e = pk[(size_t)k * count + i];
+          ((e & 0x80000000u) ? ln : lp)[l].push_back(hop_target(e));
+        }
+      }
+      cp[2 * c + h] = colour_chunk(lp, count);
+      cn[2 * c + h] = colour_chunk(ln, count);
+      if (cp[2 * c + h].colours < 0 || cn[2 * c + h].colours < 0)
+        return set_error(ED_ERR_INVALID, "hop table entry out of range while ordering the row-pass slots");
+    }
+    const int kp = std::max(cp[2 * c].colours, cp[2 * c + 1].colours);
+    const int kn = std::max(cn[2 * c].colours, cn[2 * c + 1].colours);
+    words[2 * c] = (kp + 1) / 2;
+    words[2 * c + 1] = (kp + 1) / 2 + (kn + 1) / 2;
+    maxw = std::max(maxw, words[2 * c + 1]);
+  }
+  if (maxw > kHalfMaxWords) return ED_OK;  // row pass uses the 32-bit tables
+  tab.hwords = maxw;
+  std::vector<uint32_t> out((size_t)std::max(maxw, 1) * count, 0);
+  for (int64_t i = 0; i < count; ++i) {
+    const int64_t c = i / 64;
+    const int half = (int)((i % 64) / L), l = (int)(i % L);
+    const uint32_t lane_pad = (uint32_t)(count + ((l - count % L + L) % L));  // zero pad in bank pair l
+    const int wp = words[2 * c];
+    for (int w = 0; w < maxw; ++w) {
+      const ChunkColouring& col = w < wp ? cp[2 * c + half] : cn[2 * c + half];
+      const int k0 = 2 * (w < wp ? w : w - wp);
+      uint32_t v[2];
+      for (int j = 0; j < 2; ++j) {
+        const int k = k0 + j;
+        v[j] = k < col.colours ? (uint32_t)col.target[(size_t)k * L + l] : lane_pad;
+      }
+      out[(size_t)w * count + i] = v[0] | (v[1] << 16);
+    }
+  }
+  ED_TRY(dev_alloc(&tab.h16, (int64_t)out.size()));
+  ED_TRY(dev_alloc(&tab.hwords_chunk, 2 * nchunk));
+  ED_HIP(hipMemcpyAsync(tab.h16, out.data(), sizeof(uint32_t) * out.size(), hipMemcpyHostToDevice, s));
+  ED_HIP(hipMemcpyAsync(tab.hwords_chunk, words.data(), sizeof(int32_t) * words.size(), hipMemcpyHostToDevice, s));
+  ED_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
+  return ED_OK;
+}
+
 static int build_spin(ed_op* op, int n, uint32_t** states, double** diag, Tables& tab,
                       const int32_t* d_bonds, const double* d_hops, const double* d_eps,
                       hipStream_t s, int* max_hops) {
@@ -224,6 +377,10 @@ static int build_spin(ed_op* op, int n, uint32_t** states, double** diag, Tables
   ED_TRY(dev_alloc(&tab.val_rm, count * nb));
   ED_TRY(dev_alloc(&tab.col_cm, count * nb));
   ED_TRY(dev_alloc(&tab.val_cm, count * nb));
+  if (op->packed) {
+    ED_TRY(dev_alloc(&tab.pk_rm, count * nb));
+    ED_TRY(dev_alloc(&tab.pk_cm, count * nb));
+  }
   hipLaunchKernelGGL(k_unrank, dim3(grid_for(count, 256)), dim3(256), 0, s, *states, count,
                      op->num_sites, n);
   hipLaunchKernelGGL(k_weighted, dim3(grid_for(count, 256)), dim3(256), 0, s, *states, count, d_eps,
@@ -231,13 +388,60 @@ static int build_spin(ed_op* op, int n, uint32_t** states, double** diag, Tables
   if (op->num_bonds > 0) {
     hipLaunchKernelGGL(k_hop_table, dim3(grid_for(count, 256)), dim3(256), 0, s, *states, count,
                        op->width, d_bonds, d_hops, op->num_bonds, tab.cnt, tab.col_rm, tab.val_rm,
-                       tab.col_cm, tab.val_cm);
+                       tab.col_cm, tab.val_cm, tab.pk_rm, tab.pk_cm);
   } else {
+    if (op->packed) {
+      ED_HIP(hipMemsetAsync(tab.pk_rm, 0, sizeof(uint32_t) * count, s));
+      ED_HIP(hipMemsetAsync(tab.pk_cm, 0, sizeof(uint32_t) * count, s));
+    }
     ED_HIP(hipMemsetAsync(tab.cnt, 0, sizeof(int32_t) * count, s));
     ED_HIP(hipMemsetAsync(tab.col_rm, 0, sizeof(int32_t) * count, s));
     ED_HIP(hipMemsetAsync(tab.val_rm, 0, sizeof(double) * count, s));
     ED_HIP(hipMemsetAsync(tab.col_cm, 0, sizeof(int32_t) * count, s));
     ED_HIP(hipMemsetAsync(tab.val_cm, 0, sizeof(double) * count, s));
+  }
+  // sign-split tables for uniform hopping; the row pass addresses a row of
+  // doubles plus its zero pad with 32-bit byte offsets
+  if (op->packed && op->uniform_hop && op->num_bonds > 0 && count * 8 < (1ll << 31)) {
+    const int64_t nchunk = (count + 63) / 64;
+    int32_t *np = nullptr, *nn = nullptr;
+    int rc = dev_alloc(&np, count);
+    if (rc == ED_OK) rc = dev_alloc(&nn, count);
+    if (rc == ED_OK) rc = dev_alloc(&tab.cmax_pos, nchunk);
+    if (rc == ED_OK) rc = dev_alloc(&tab.cmax_neg, nchunk);
+    std::vector<int32_t> cmp(nchunk), cmn(nchunk);
+    if (rc == ED_OK) {
+      hipLaunchKernelGGL(k_sign_split, dim3(grid_for(count, 256)), dim3(256), 0, s, tab.pk_cm, tab.cnt, count, np,
+                         nn);
+      hipLaunchKernelGGL(k_chunk_max, dim3(grid_for(nchunk, 256)), dim3(256), 0, s, np, nn, count, tab.cmax_pos,
+                         tab.cmax_neg);
+      if (hipMemcpyAsync(cmp.data(), tab.cmax_pos, sizeof(int32_t) * nchunk, hipMemcpyDeviceToHost, s) !=
+              hipSuccess ||
+          hipMemcpyAsync(cmn.data(), tab.cmax_neg, sizeof(int32_t) * nchunk, hipMemcpyDeviceToHost, s) !=
+              hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        rc = set_error(ED_ERR_HIP, "copy of hop-count maxima failed");
+    }
+    if (np) (void)hipFree(np);
+    if (nn) (void)hipFree(nn);
+    ED_TRY(rc);
+    for (int64_t c = 0; c < nchunk; ++c) {
+      tab.wpos = std::max(tab.wpos, cmp[c]);
+      tab.wneg = std::max(tab.wneg, cmn[c]);
+    }
+    const int hp8 = (tab.wpos + 7) & ~7, hn8 = (tab.wneg + 7) & ~7;
+    ED_TRY(dev_alloc(&tab.off_pos, hp8 * count));
+    ED_TRY(dev_alloc(&tab.off_neg, hn8 * count));
+    tab.tgt_stride = (nb + 7) & ~7;
+    ED_TRY(dev_alloc(&tab.tgt_sorted, count * tab.tgt_stride));
+    ED_TRY(dev_alloc(&tab.cnt_pn, count));
+    hipLaunchKernelGGL(k_offset_tables, dim3(grid_for(count, 256)), dim3(256), 0, s, tab.pk_cm, tab.cnt, count,
+                       hp8, hn8, tab.off_pos, tab.off_neg);
+    // half-width tables where one row (with its pads) fits an LDS plane; sizes
+    // from 4096 states up, where rows are too long for several per block
+    if (count <= kHalfMaxStates && count >= 4096) ED_TRY(build_half_tables(tab, count, nb, s));
+    hipLaunchKernelGGL(k_sorted_targets, dim3(grid_for(count, 256)), dim3(256), 0, s, tab.pk_rm, tab.cnt, count, nb,
+                       tab.tgt_stride, tab.tgt_sorted, tab.cnt_pn);
   }
   ED_HIP(hipGetLastError());
   // host-side maximum hop count (used by kernels that size per-lane lists)
@@ -393,6 +597,10 @@ int ed_op_create(ed_op** out, const ed_model_desc* d, void* stream) {
   op->u_mode = !any ? 0 : (uniform ? 1 : 2);
   op->u_uniform = d->u[0];
   (void)hipGetDevice(&op->device);
+  op->packed = nu < (1ll << 24) && nd < (1ll << 24) && d->num_bonds <= 127;
+  op->uniform_hop = true;
+  for (int b = 1; b < d->num_bonds; ++b)
+    if (d->hop[b] != d->hop[0]) op->uniform_hop = false;
 
   hipStream_t s = as_stream(stream);
   int32_t* d_bonds = nullptr;
@@ -428,7 +636,7 @@ int ed_op_create(ed_op** out, const ed_model_desc* d, void* stream) {
       break;
   } while (0);
   if (d_bonds) (void)hipFree(d_bonds);
-  if (d_hops) (void)hipFree(d_hops);
+  op->hop_dev = d_hops;  // kept: packed hop entries index it
   if (d_eps) (void)hipFree(d_eps);
   if (rc != ED_OK) {
     free_op(op);

@@ -120,7 +120,55 @@ struct Tables {
   double* val_rm = nullptr;  // [D][width]
   int32_t* col_cm = nullptr; // [width][D]
   double* val_cm = nullptr;  // [width][D]
+  // Packed entries: target (bits 0-23) | bond (bits 24-30) | sign (bit 31);
+  // the value is sign * hop[bond].  Present when D < 2^24 and bonds <= 127.
+  uint32_t* pk_rm = nullptr; // [D][width]
+  uint32_t* pk_cm = nullptr; // [width][D]
+  // Sign-split offset tables (uniform hopping, row pass): state i's hops with
+  // sign +1 are off_pos[k][i] = target * 8 (a byte offset into a row of
+  // doubles), k < its positive count; the remaining slots up to wpos hold D * 8,
+  // the offset of a zero pad stored after the row.  off_neg likewise for the
+  // hops with sign -1.  cmax_pos/cmax_neg: per 64-state chunk, the largest
+  // count; wpos/wneg: the largest over all chunks.
+  uint32_t* off_pos = nullptr;  // [roundup(wpos, 8)][D]; slots past wpos are pads
+  uint32_t* off_neg = nullptr;  // [roundup(wneg, 8)][D]
+  int32_t* cmax_pos = nullptr;  // [ceil(D / 64)]
+  int32_t* cmax_neg = nullptr;
+  int32_t wpos = 0, wneg = 0;
+  // Half-width form of the same lists for D <= kHalfMaxStates: 16-bit entries
+  // p, the index of a double in a staged row, where p is a target or one of 32
+  // zero pads D .. D+31; slots 2w and 2w+1 sit in the low and high half of
+  // word w.  In each 64-state chunk the first hwords_chunk[2c] words hold the
+  // hops of sign +1 and the words up to hwords_chunk[2c+1] those of sign -1;
+  // within each 32-state half-wave the slots of either list are ordered so
+  // that every slot reads 32 distinct LDS bank pairs (p mod 32).  [hwords][D].
+  uint32_t* h16 = nullptr;
+  int32_t* hwords_chunk = nullptr;  // [2 * ceil(D / 64)]
+  int32_t hwords = 0;
+  // Sign-sorted target lists (uniform hopping, column pass): row i holds its
+  // targets with sign +1 first, then those with sign -1; cnt_pn[i] = number
+  // positive | number negative << 16.  [D][tgt_stride], tgt_stride = width
+  // rounded up to 8, unused entries 0.
+  int32_t* tgt_sorted = nullptr;
+  uint32_t* cnt_pn = nullptr;
+  int32_t tgt_stride = 0;
 };
+
+// Largest spin-sector size with half-width row-pass tables: entries and the 32
+// pads D .. D+31 must fit 16 bits, and the staged row with its pads must fit
+// the LDS of one workgroup (checked when the kernel is chosen).
+constexpr int64_t kHalfMaxStates = 65535 - 32;
+constexpr int kHalfMaxWords = 24;  // longest per-chunk word list the row pass unrolls
+
+__host__ __device__ inline uint32_t pack_hop(int32_t target, int bond, int sign) {
+  return (uint32_t)target | ((uint32_t)bond << 24) | (sign < 0 ? 0x80000000u : 0u);
+}
+__host__ __device__ inline int32_t hop_target(uint32_t e) { return (int32_t)(e & 0x00ffffffu); }
+__host__ __device__ inline int hop_bond(uint32_t e) { return (int)((e >> 24) & 0x7fu); }
+__host__ __device__ inline double hop_value(uint32_t e, const double* hop) {
+  const double v = hop[hop_bond(e)];
+  return (e & 0x80000000u) ? -v : v;
+}
 
 }  // namespace ed
 
@@ -141,6 +189,10 @@ struct ed_op {
   double* u_site = nullptr;  // [num_sites]
   ed::Tables up, dn;
   int max_hops_up = 0, max_hops_dn = 0;
+  double* hop_dev = nullptr;  // [num_bonds]
+  bool packed = false;        // packed tables present
+  bool uniform_hop = false;   // every bond has the same hop value
+  int hv_mode = -1;           // H.v kernel choice (-1 = automatic)
 };
 
 namespace ed {

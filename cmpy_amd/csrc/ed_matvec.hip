@@ -12,6 +12,9 @@
 // fused into the same pass (no extra traffic: x is already in registers).
 #include "ed_internal.h"
 
+#include <cstdlib>
+#include <mutex>
+
 namespace ed {
 
 __device__ __forceinline__ double u_term_dev(const HvArgs& a, uint32_t both) {
@@ -122,6 +125,715 @@ __global__ __launch_bounds__(256) void k_hv(HvArgs a, const double* __restrict__
   }
 }
 
+// ---------------------------------------------------------------------------
+// Two-pass sweep for large sectors.
+//
+// Pass R (rows): a workgroup stages R whole rows of x in LDS and writes
+//   y = s*(diag + dn hops) - c*xprev   (+ block partial of <y, x>)
+// so every dn-hop gather is an LDS read; x and y each cross HBM once.
+// Pass C (column slabs): y += s*(up hops).  Column tiles of TW columns are
+// assigned to XCDs round-robin and each XCD sweeps every row of a tile before
+// moving on, so the tile's slab of x (num_up x TW doubles) stays in that
+// XCD's L2 while ~17 source rows per output row are read from it.
+// ---------------------------------------------------------------------------
+struct Hv2Args {
+  int64_t nu, nd;
+  const uint32_t* st_up;
+  const uint32_t* st_dn;
+  const double* d_up;
+  const double* d_dn;
+  const int32_t* up_cnt;
+  const uint32_t* up_pk;  // [nu][up_w]
+  int32_t up_w;
+  const int32_t* dn_cnt;
+  const uint32_t* dn_pk;  // [dn_w][nd]
+  int32_t dn_w;
+  // sign-split tables (uniform hopping; see Tables)
+  const uint32_t* dn_off_pos;   // [wpos][nd]
+  const uint32_t* dn_off_neg;   // [wneg][nd]
+  const int32_t* dn_cmax_pos;   // [ceil(nd / 64)]
+  const int32_t* dn_cmax_neg;
+  const uint32_t* dn_h16;       // [hwords][nd] half-width index pairs
+  const int32_t* dn_hwords;     // [2 * ceil(nd / 64)] positive and total words per chunk
+  const int32_t* up_tgt;        // [nu][up_w]
+  const uint32_t* up_cnt_pn;    // [nu]
+  const double* hop;
+  int32_t nb;
+  int32_t u_mode;
+  double u_uniform;
+  uint32_t u_mask;
+  const double* u_site;
+};
+
+__device__ __forceinline__ double u_term2(const Hv2Args& a, uint32_t both) {
+  if (a.u_mode == 1) return a.u_uniform * (double)popc32(both);
+  if (a.u_mode == 2) {
+    double acc = 0.0;
+    uint32_t m = both & a.u_mask;
+    while (m) {
+      const int p = __builtin_ctz(m);
+      acc += a.u_site[p];
+      m &= m - 1;
+    }
+    return acc;
+  }
+  return 0.0;
+}
+
+template <int NT, typename P>
+__device__ __forceinline__ double block_sum_n(double v, P red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < NT / 64; ++w) r += red[w];
+  return r;
+}
+
+// Load from an address that is the same for every lane of the wave, through
+// the scalar cache (the tables it reads are never written by these kernels).
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)p;
+}
+
+constexpr int kRowThreads = 1024;
+
+// Hop value of a packed entry: uniform models (every bond the same hop t) need
+// only the sign bit; otherwise the bond's value comes from the LDS copy.
+template <bool UNI>
+__device__ __forceinline__ double hv_val(uint32_t ent, double t, const double* hv) {
+  if (UNI) return (ent & 0x80000000u) ? -t : t;
+  return hop_value(ent, hv);
+}
+
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Copies n doubles into LDS with every thread holding U loads in flight
+// (one HBM round trip per NT*U elements instead of one per element).
+template <int NT, int U, typename P>
+__device__ __forceinline__ void stage_rows(P lds, const double* __restrict__ src, int64_t n) {
+  for (int64_t base = 0; base < n; base += (int64_t)NT * U) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t k = base + (int64_t)u * NT + threadIdx.x;
+      v[u] = __builtin_nontemporal_load(src + (k < n ? k : 0));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t k = base + (int64_t)u * NT + threadIdx.x;
+      if (k < n) lds[k] = v[u];
+    }
+  }
+}
+
+constexpr int kHopBatch = 16;  // hop terms whose loads are issued together (column pass)
+constexpr int kRowBatch = 8;   // same for the row pass
+
+// Row pass, general tables.  Hops run in batches of kRowBatch: entries first,
+// then independent LDS gathers, then FMAs; slots past a column's hop count
+// read index 0 with value 0, so no branch separates the loads of a batch.
+template <bool UNI>
+__global__ __launch_bounds__(kRowThreads) void k_hv_rows(Hv2Args a, int rows_per_block,
+                                                         const double* __restrict__ x,
+                                                         double* __restrict__ y, HvEpilogue e) {
+  extern __shared__ double lds[];
+  __shared__ double red[kRowThreads / 64];
+  const int64_t nd = a.nd;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int R = (int)(a.nu - r0 < rows_per_block ? a.nu - r0 : rows_per_block);
+  double* hv = lds + (int64_t)rows_per_block * nd;
+  stage_rows<kRowThreads, 8>(lds, x + r0 * nd, (int64_t)R * nd);
+  for (int b = threadIdx.x; b < a.nb; b += kRowThreads) hv[b] = a.hop[b];
+  __syncthreads();
+  const double sx = e.sx ? *e.sx : 1.0;
+  const double cp = e.cprev ? *e.cprev : 0.0;
+  const double t = a.nb > 0 ? hv[0] : 0.0;
+  double dot = 0.0;
+  const int64_t nd_pad = (nd + 63) & ~int64_t(63);
+  for (int64_t id = threadIdx.x; id < nd_pad; id += kRowThreads) {
+    const bool act = id < nd;
+    const int64_t ids = act ? id : 0;
+    const int cnt0 = a.dn_cnt[ids];
+    const uint32_t sdn = a.st_dn[ids];
+    const double ddn = a.d_dn[ids];
+    const int cnt = act ? cnt0 : 0;
+    const int cmax = wave_max_int(cnt);
+    for (int r = 0; r < R; ++r) {
+      const int64_t iu = r0 + r;
+      const int64_t gi = iu * nd + id;
+      const double xp = (e.xprev && act) ? e.xprev[gi] : 0.0;
+      const double* row = lds + (int64_t)r * nd;
+      const double xo = act ? row[ids] : 0.0;
+      double acc = (a.d_up[iu] + ddn + u_term2(a, a.st_up[iu] & sdn)) * xo;
+      for (int k0 = 0; k0 < cmax; k0 += kRowBatch) {
+        uint32_t tg[kRowBatch];
+        double vl[kRowBatch];
+        {
+          uint32_t en[kRowBatch];
+#pragma unroll
+          for (int q = 0; q < kRowBatch; ++q) {
+            const int kk = k0 + q;
+            en[q] = a.dn_pk[(kk < cnt) ? (int64_t)kk * nd + ids : 0];
+          }
+#pragma unroll
+          for (int q = 0; q < kRowBatch; ++q) {
+            const bool ok = k0 + q < cnt;
+            tg[q] = ok ? (uint32_t)hop_target(en[q]) : 0u;
+            vl[q] = ok ? hv_val<UNI>(en[q], t, hv) : 0.0;
+          }
+        }
+        double g[kRowBatch];
+#pragma unroll
+        for (int q = 0; q < kRowBatch; ++q) g[q] = row[tg[q]];
+#pragma unroll
+        for (int q = 0; q < kRowBatch; ++q) acc += vl[q] * g[q];
+      }
+      if (act) {
+        const double v = sx * acc - cp * xp;
+        __builtin_nontemporal_store(v, y + gi);
+        dot += v * xo;
+      }
+    }
+  }
+  if (e.partial) {
+    const double s = block_sum_n<kRowThreads>(dot, red);
+    if (threadIdx.x == 0) e.partial[blockIdx.x] = s;
+  }
+}
+
+// Sums one sign list of the row pass over RB staged rows: 8 table words per
+// batch (coalesced over the wave's 64 columns), reused for every row of the
+// group; padding slots address the row's zero pad, so the loop runs the
+// chunk's slot count, rounded up to 8, with no per-lane test.
+template <int RB>
+__device__ __forceinline__ void gather_list(const uint32_t* __restrict__ tab, int kmax, int64_t nd, int64_t ids,
+                                            int rg, int R, uint32_t row_bytes, const double* lds, double* acc) {
+  const char* base = (const char*)lds;
+  for (int k0 = 0; k0 < kmax; k0 += 8) {
+    uint32_t w[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) w[q] = tab[(int64_t)(k0 + q) * nd + ids];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      if (rg + r < R) {
+        const uint32_t rb = (uint32_t)(rg + r) * row_bytes;
+        double g[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) g[q] = *(const double*)(base + (rb + w[q]));
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[r] += g[q];
+      }
+    }
+  }
+}
+
+// Row pass for uniform hopping t: R rows of x staged in LDS, each followed by
+// a zero pad; y = s*(diag*x + t*(P - N)) - c*xprev with P (N) the sum of the
+// dn-hop sources of sign +1 (-1), gathered through the sign-split offset
+// tables.  The loop over a 64-column chunk runs that chunk's largest count.
+template <int RB>
+__global__ __launch_bounds__(kRowThreads) void k_hv_rows_off(Hv2Args a, int rows_per_block,
+                                                             const double* __restrict__ x,
+                                                             double* __restrict__ y, HvEpilogue e) {
+  extern __shared__ double lds[];
+  __shared__ double red[kRowThreads / 64];
+  const int64_t nd = a.nd;
+  const uint32_t stride = (uint32_t)nd + 1u;  // doubles per staged row
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int R = (int)(a.nu - r0 < rows_per_block ? a.nu - r0 : rows_per_block);
+  for (int r = 0; r < R; ++r) stage_rows<kRowThreads, 8>(lds + (int64_t)r * stride, x + (r0 + r) * nd, nd);
+  if ((int)threadIdx.x < R) lds[(int64_t)threadIdx.x * stride + nd] = 0.0;
+  __syncthreads();
+  const double sx = e.sx ? *e.sx : 1.0;
+  const double cp = e.cprev ? *e.cprev : 0.0;
+  const double t = a.hop[0];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  double dot = 0.0;
+  for (int64_t c0 = (int64_t)wave * 64; c0 < nd; c0 += kRowThreads) {
+    const int64_t id = c0 + lane;
+    const bool act = id < nd;
+    const int64_t ids = act ? id : c0;
+    const int64_t chunk = c0 >> 6;
+    const int kp = (uniform_load(a.dn_cmax_pos + chunk) + 7) & ~7;
+    const int kn = (uniform_load(a.dn_cmax_neg + chunk) + 7) & ~7;
+    const uint32_t sdn = a.st_dn[ids];
+    const double ddn = a.d_dn[ids];
+    for (int rg = 0; rg < R; rg += RB) {
+      double xp[RB];
+#pragma unroll
+      for (int r = 0; r < RB; ++r)
+        xp[r] = (e.xprev && act && rg + r < R) ? e.xprev[(r0 + rg + r) * nd + id] : 0.0;
+      double accp[RB], accn[RB];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) accp[r] = accn[r] = 0.0;
+      gather_list<RB>(a.dn_off_pos, kp, nd, ids, rg, R, stride * 8u, lds, accp);
+      gather_list<RB>(a.dn_off_neg, kn, nd, ids, rg, R, stride * 8u, lds, accn);
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        if (act && rg + r < R) {
+          const int64_t iu = r0 + rg + r;
+          const double xo = lds[(int64_t)(rg + r) * stride + id];
+          const double dg = a.d_up[iu] + ddn + u_term2(a, a.st_up[iu] & sdn);
+          const double v = sx * (dg * xo + t * (accp[r] - accn[r])) - cp * xp[r];
+          __builtin_nontemporal_store(v, y + iu * nd + id);
+          dot += v * xo;
+        }
+      }
+    }
+  }
+  if (e.partial) {
+    const double s = block_sum_n<kRowThreads>(dot, red);
+    if (threadIdx.x == 0) e.partial[blockIdx.x] = s;
+  }
+}
+
+// P - N over one chunk's half-width word list of one column: NW words, word
+// k at byte tb + k * nd4 + lane4 (uniform base plus the lane's offset), the
+// first nwp of sign +1.  Straight-line code: every word is requested before
+// the first gather, so a chunk costs one table round trip and one LDS round
+// trip.
+// Workgroup LDS addressed from offset 0 (for kernels without static LDS, whose
+// dynamic LDS therefore starts at 0).
+using lds_f64 = __attribute__((address_space(3))) double;
+
+template <int NW>
+__device__ __forceinline__ double sum_words(const char* tb, uint32_t nd4, uint32_t lane4, int nwp,
+                                            const lds_f64* row) {
+  uint32_t w[NW];
+#pragma unroll
+  for (int k = 0; k < NW; ++k) w[k] = *(const uint32_t*)(tb + (size_t)k * nd4 + lane4);
+  double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    const double sg = k < nwp ? 1.0 : -1.0;
+    a0 = fma(sg, row[w[k] & 0xffffu], a0);
+    a1 = fma(sg, row[w[k] >> 16], a1);
+  }
+  return a0 + a1;
+}
+
+__device__ __forceinline__ double sum_words_n(int nw, const char* tb, uint32_t nd4, uint32_t lane4, int nwp,
+                                              const lds_f64* row) {
+  switch (nw) {
+#define ED_SUM_CASE(n) \
+  case n: return sum_words<n>(tb, nd4, lane4, nwp, row);
+    ED_SUM_CASE(1) ED_SUM_CASE(2) ED_SUM_CASE(3) ED_SUM_CASE(4) ED_SUM_CASE(5) ED_SUM_CASE(6)
+    ED_SUM_CASE(7) ED_SUM_CASE(8) ED_SUM_CASE(9) ED_SUM_CASE(10) ED_SUM_CASE(11) ED_SUM_CASE(12)
+    ED_SUM_CASE(13) ED_SUM_CASE(14) ED_SUM_CASE(15) ED_SUM_CASE(16) ED_SUM_CASE(17) ED_SUM_CASE(18)
+    ED_SUM_CASE(19) ED_SUM_CASE(20) ED_SUM_CASE(21) ED_SUM_CASE(22) ED_SUM_CASE(23) ED_SUM_CASE(24)
+#undef ED_SUM_CASE
+    default: return 0.0;  // nw == 0 (kHalfMaxWords bounds it above)
+  }
+}
+
+// Row pass for uniform hopping t and num_dn <= kHalfMaxStates, one row per
+// block: the row is staged in LDS with 32 zero pads after it (indices nd ..
+// nd+31), and the dn hops gather through the bank-ordered half-width table,
+// whose slots each read 32 distinct bank pairs per half-wave.
+__global__ __launch_bounds__(kRowThreads) void k_hv_rows_half(Hv2Args a, int rows_per_block,
+                                                              const double* __restrict__ x,
+                                                              double* __restrict__ y, HvEpilogue e) {
+  // all LDS is dynamic and addressed from offset 0 (a gather address is
+  // index * 8): row [nd], pads [32], reduction slots [kRowThreads / 64]
+  extern __shared__ double dyn_lds[];
+  (void)dyn_lds;
+  lds_f64* row = (lds_f64*)(uintptr_t)0;
+  const int64_t nd = a.nd;
+  lds_f64* red = row + nd + 32;
+  const int64_t iu = blockIdx.x;
+  stage_rows<kRowThreads, 16>(row, x + iu * nd, nd);
+  if (threadIdx.x < 32) row[nd + threadIdx.x] = 0.0;
+  __syncthreads();
+  const double sx = e.sx ? *e.sx : 1.0;
+  const double cp = e.cprev ? *e.cprev : 0.0;
+  const double t = a.hop[0];
+  const double dup = a.d_up[iu];
+  const uint32_t sup = a.st_up[iu];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  double dot = 0.0;
+  for (int64_t c0 = (int64_t)wave * 64; c0 < nd; c0 += kRowThreads) {
+    const int64_t id = c0 + lane;
+    const bool act = id < nd;
+    const int64_t ids = act ? id : c0;
+    const int64_t chunk = c0 >> 6;
+    const int nwp = uniform_load(a.dn_hwords + 2 * chunk);
+    const int nw = uniform_load(a.dn_hwords + 2 * chunk + 1);
+    const int64_t gi = iu * nd + ids;
+    const double xp = e.xprev ? e.xprev[gi] : 0.0;
+    const uint32_t sdn = a.st_dn[ids];
+    const double ddn = a.d_dn[ids];
+    const double pn = sum_words_n(nw, (const char*)a.dn_h16, (uint32_t)nd * 4u, (uint32_t)ids * 4u, nwp, row);
+    if (act) {
+      const double xo = row[id];
+      const double dg = dup + ddn + u_term2(a, sup & sdn);
+      const double v = sx * (dg * xo + t * pn) - cp * xp;
+      __builtin_nontemporal_store(v, y + gi);
+      dot += v * xo;
+    }
+  }
+  if (e.partial) {
+    const double s = block_sum_n<kRowThreads>(dot, red);
+    if (threadIdx.x == 0) e.partial[blockIdx.x] = s;
+  }
+}
+
+constexpr int kColMaxRows = 64;
+
+// Column pass, general tables: the block stages its rows' up-hop entries in
+// LDS; each row's hops run in batches of independent slab loads.
+template <int TW, int MAXH, bool UNI>
+__global__ __launch_bounds__(256) void k_hv_cols(Hv2Args a, const double* __restrict__ x,
+                                                 double* __restrict__ y, HvEpilogue e, int64_t ntiles,
+                                                 int64_t nrc, int rch, double* partial) {
+  constexpr int RPI = 256 / TW;
+  __shared__ double hv[128];
+  __shared__ double red[4];
+  __shared__ uint32_t sent[kColMaxRows * MAXH];
+  __shared__ int scnt[kColMaxRows];
+  const int64_t b = blockIdx.x;
+  const int64_t xcd = b & 7, j = b >> 3;
+  const int64_t tile = (j / nrc) * 8 + xcd;
+  const int64_t rc = j % nrc;
+  const int64_t nd = a.nd;
+  const int64_t row0 = rc * rch;
+  for (int i = threadIdx.x; i < a.nb; i += 256) hv[i] = a.hop[i];
+  for (int i = threadIdx.x; i < rch * MAXH; i += 256) {
+    const int r = i / MAXH, k = i - r * MAXH;
+    const int64_t iu = row0 + r;
+    uint32_t v = 0;
+    if (iu < a.nu && k < a.up_cnt[iu]) v = a.up_pk[iu * a.up_w + k];
+    sent[i] = v;
+  }
+  for (int r = threadIdx.x; r < rch; r += 256) scnt[r] = row0 + r < a.nu ? a.up_cnt[row0 + r] : 0;
+  __syncthreads();
+  const double t0 = a.nb > 0 ? hv[0] : 0.0;
+  double dot = 0.0;
+  if (tile < ntiles) {
+    const int64_t id = tile * TW + (threadIdx.x % TW);
+    const bool act = id < nd;
+    const int64_t ids = act ? id : 0;
+    const int rl = threadIdx.x / TW;
+    const double sx = e.sx ? *e.sx : 1.0;
+    for (int it = rl; it < rch; it += RPI) {
+      const int64_t iu = row0 + it;
+      const int cnt = scnt[it];
+      const int cmax = wave_max_int(cnt);
+      const uint32_t* ent = sent + it * MAXH;
+      // the output's read-modify-write operands are requested before the gathers
+      const bool wr = act && iu < a.nu;
+      const int64_t gw = wr ? iu * nd + id : ids;
+      const double yold = __builtin_nontemporal_load(y + gw);
+      const double xown = x[gw];
+      double acc = 0.0;
+      // B divides MAXH (both powers of two) and cmax <= MAXH, so a batch never
+      // reads past the row's MAXH staged slots; slots at or past cnt gather
+      // from the lane's own column of row 0 with value 0.
+      constexpr int B = kHopBatch < MAXH ? kHopBatch : MAXH;
+      for (int k0 = 0; k0 < cmax; k0 += B) {
+        uint32_t en[B];
+#pragma unroll
+        for (int q = 0; q < B; ++q) en[q] = ent[k0 + q];
+        double g[B];
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+          const bool ok = k0 + q < cnt;
+          g[q] = x[ok ? (int64_t)hop_target(en[q]) * nd + ids : ids];
+        }
+#pragma unroll
+        for (int q = 0; q < B; ++q) acc += (k0 + q < cnt ? hv_val<UNI>(en[q], t0, hv) : 0.0) * g[q];
+      }
+      if (wr) {
+        const double up = sx * acc;
+        __builtin_nontemporal_store(yold + up, y + gw);
+        dot += up * xown;
+      }
+    }
+  }
+  if (partial) {
+    const double s = block_sum_256(dot, red);
+    if (threadIdx.x == 0) partial[b] = s;
+  }
+}
+
+// Column pass for uniform hopping t: y += s*t*(P - N) over the up hops, one
+// wave per output row of a TW-column tile.  The row's sign-sorted target list
+// (positive first; MAXL entries, unused ones 0) comes in one scalar batch;
+// lane group q (lanes q*TW .. q*TW+TW-1) of load j takes entry j*HP + q, so
+// a load instruction covers HP hops, and all of a row's loads are issued
+// before the first is consumed.  Tiles go to XCDs round-robin and each XCD
+// sweeps every row of its tile, so the tile's slab of x (nu x TW doubles)
+// stays in that XCD's L2.  Requires D * 8 <= 2^32 (32-bit offsets).
+template <int TW, int MAXL>
+__global__ __launch_bounds__(256) void k_hv_cols_split(Hv2Args a, const double* __restrict__ x,
+                                                       double* __restrict__ y, HvEpilogue e, int64_t ntiles,
+                                                       int64_t nrc, int rch, double* partial) {
+  constexpr int HP = 64 / TW;
+  constexpr int NJ = MAXL / HP;
+  __shared__ double red[4];
+  const int64_t b = blockIdx.x;
+  const int64_t xcd = b & 7, j = b >> 3;
+  const int64_t tile = (j / nrc) * 8 + xcd;
+  const int64_t rc = j % nrc;
+  const int64_t nd = a.nd;
+  const int64_t row0 = rc * rch;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int q = lane / TW;
+  double dot = 0.0;
+  if (tile < ntiles) {
+    const int64_t col = tile * TW + (lane % TW);
+    const bool act = col < nd;
+    const int64_t colc = act ? col : tile * TW;
+    const uint32_t laneoff = (uint32_t)colc * 8u;
+    const uint32_t nd8 = (uint32_t)nd * 8u;
+    const char* xb = (const char*)x;
+    const double sx = e.sx ? *e.sx : 1.0;
+    const double st = sx * a.hop[0];
+    const bool wr = act && q == 0;
+    for (int it = wave; it < rch; it += 4) {
+      const int64_t iu = row0 + it;
+      if (iu >= a.nu) break;
+      const int64_t gw = iu * nd + colc;
+      double yold = 0.0, xown = 0.0;
+      if (wr) {
+        yold = __builtin_nontemporal_load(y + gw);
+        xown = x[gw];
+      }
+      const uint32_t cw = uniform_load(a.up_cnt_pn + iu);
+      const int np = (int)(cw & 0xffffu), n = np + (int)(cw >> 16);
+      const int32_t* tg = a.up_tgt + iu * MAXL;
+      uint32_t tv[MAXL];
+#pragma unroll
+      for (int k = 0; k < MAXL; ++k) tv[k] = (uint32_t)uniform_load(tg + k) * nd8;
+      double g[NJ];
+#pragma unroll
+      for (int jj = 0; jj < NJ; ++jj) {
+        if (jj * HP < n) {
+          uint32_t o = tv[jj * HP];
+#pragma unroll
+          for (int h = 1; h < HP; ++h) o = q == h ? tv[jj * HP + h] : o;
+          g[jj] = *(const double*)(xb + (laneoff + o));
+        }
+      }
+      double acc = 0.0;
+#pragma unroll
+      for (int jj = 0; jj < NJ; ++jj) {
+        if (jj * HP < n) {
+          const int sl = jj * HP + q;
+          const double v = sl < np ? g[jj] : -g[jj];
+          acc += sl < n ? v : 0.0;
+        }
+      }
+#pragma unroll
+      for (int off = TW; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
+      if (wr) {
+        const double up = st * acc;
+        __builtin_nontemporal_store(yold + up, y + gw);
+        dot += up * xown;
+      }
+    }
+  }
+  if (partial) {
+    const double s = block_sum_256(dot, red);
+    if (threadIdx.x == 0) partial[b] = s;
+  }
+}
+
+static int env_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return v && *v ? atoi(v) : dflt;
+}
+
+static Hv2Args make_hv2_args(const ed_op* op) {
+  Hv2Args a;
+  a.nu = op->num_up;
+  a.nd = op->num_dn;
+  a.st_up = op->st_up;
+  a.st_dn = op->st_dn;
+  a.d_up = op->d_up;
+  a.d_dn = op->d_dn;
+  a.up_cnt = op->up.cnt;
+  a.up_pk = op->up.pk_rm;
+  a.up_w = op->up.width;
+  a.dn_cnt = op->dn.cnt;
+  a.dn_pk = op->dn.pk_cm;
+  a.dn_w = op->dn.width;
+  a.dn_off_pos = op->dn.off_pos;
+  a.dn_off_neg = op->dn.off_neg;
+  a.dn_cmax_pos = op->dn.cmax_pos;
+  a.dn_cmax_neg = op->dn.cmax_neg;
+  a.dn_h16 = op->dn.h16;
+  a.dn_hwords = op->dn.hwords_chunk;
+  a.up_tgt = op->up.tgt_sorted;
+  a.up_cnt_pn = op->up.cnt_pn;
+  a.hop = op->hop_dev;
+  a.nb = op->num_bonds;
+  a.u_mode = op->u_mode;
+  a.u_uniform = op->u_uniform;
+  a.u_mask = op->u_mask;
+  a.u_site = op->u_site;
+  return a;
+}
+
+using RowKernel = void (*)(Hv2Args, int, const double*, double*, HvEpilogue);
+using ColKernel = void (*)(Hv2Args, const double*, double*, HvEpilogue, int64_t, int64_t, int, double*);
+
+struct TwoPassPlan {
+  bool ok;
+  int rows_per_block;
+  size_t lds_bytes;
+  int64_t grid_r;
+  int tw, rch;
+  int64_t ntiles, nrc, grid_c;
+  RowKernel rk;
+  ColKernel ck;
+};
+
+// Smallest compiled hop-list length >= m (0 if none fits).
+static int maxh_for(int m) {
+  if (m <= 8) return 8;
+  if (m <= 16) return 16;
+  if (m <= 32) return 32;
+  if (m <= 64) return 64;
+  return 0;
+}
+
+// Grants a row kernel `bytes` of dynamic LDS (needed above 64 KiB); false if refused.
+static bool ensure_row_lds(RowKernel k, size_t bytes) {
+  static std::mutex mu;
+  static std::vector<std::pair<const void*, size_t>> granted;
+  if (bytes <= 65536) return true;
+  std::lock_guard<std::mutex> lk(mu);
+  for (auto& g : granted)
+    if (g.first == (const void*)k && g.second >= bytes) return true;
+  const hipError_t err = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (err != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  granted.emplace_back((const void*)k, bytes);
+  return true;
+}
+
+template <int TW>
+static ColKernel col_kernel_tw(int maxh, bool uni) {
+  switch (maxh) {
+    case 8: return uni ? k_hv_cols<TW, 8, true> : k_hv_cols<TW, 8, false>;
+    case 16: return uni ? k_hv_cols<TW, 16, true> : k_hv_cols<TW, 16, false>;
+    case 32: return uni ? k_hv_cols<TW, 32, true> : k_hv_cols<TW, 32, false>;
+    default: return uni ? k_hv_cols<TW, 64, true> : k_hv_cols<TW, 64, false>;
+  }
+}
+
+template <int TW>
+static ColKernel col_kernel_split_tw(int stride) {
+  switch (stride) {
+    case 8: return k_hv_cols_split<TW, 8>;
+    case 16: return k_hv_cols_split<TW, 16>;
+    case 24: return k_hv_cols_split<TW, 24>;
+    default: return k_hv_cols_split<TW, 32>;
+  }
+}
+
+static ColKernel col_kernel_split(int tw, int maxl) {
+  if (tw == 16) return col_kernel_split_tw<16>(maxl);
+  if (tw == 64) return col_kernel_split_tw<64>(maxl);
+  return col_kernel_split_tw<32>(maxl);
+}
+
+// Two-pass plan.  Kernels: sign-split tables when hopping is uniform and the
+// tables exist (CMPY_AMD_HV_SPLIT=0 turns them off), general packed tables
+// otherwise.  CMPY_AMD_HV_MODE: 0 single fused pass, 1 two passes, unset =
+// two passes from D = 65536 up.
+static TwoPassPlan plan_two_pass(const ed_op* op) {
+  TwoPassPlan p{};
+  p.ok = false;
+  const int mode = op->hv_mode >= 0 ? op->hv_mode : env_int("CMPY_AMD_HV_MODE", -1);
+  if (!op->packed || mode == 0) return p;
+  if (mode < 0 && op->size < 65536) return p;
+  const int maxh_up = maxh_for(op->max_hops_up);
+  if (!maxh_up || !maxh_for(op->max_hops_dn)) return p;
+  bool uni = op->num_bonds > 0;
+  for (double h : op->hop)
+    if (h != op->hop[0]) uni = false;
+  const bool split = uni && op->dn.off_pos && op->up.tgt_sorted && op->size * 8 <= (1ll << 32) &&
+                     op->up.tgt_stride <= 32 && env_int("CMPY_AMD_HV_SPLIT", 1) != 0;
+  int dev = 0, max_lds = 65536;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) max_lds = 65536;
+  // leave room for the kernel's static LDS and the hop-value slots
+  const size_t budget = (size_t)max_lds - 8 * 128 - 4096;
+  const size_t row_bytes = sizeof(double) * (size_t)(op->num_dn + (split ? 1 : 0));
+  int R = (int)std::min<size_t>(budget / row_bytes, 32);
+  if (R < 1) return p;
+  R = (int)std::min<int64_t>(R, op->num_up);
+  p.rows_per_block = R;
+  // one-row blocks with the half-width tables where at most two rows fit
+  const bool half = split && R <= 2 && op->dn.h16 &&
+                    sizeof(double) * (size_t)(op->num_dn + 32 + kRowThreads / 64) <= budget &&
+                    env_int("CMPY_AMD_HV_HALF", 1) != 0;
+  if (half) {
+    p.rows_per_block = R = 1;
+    p.lds_bytes = sizeof(double) * (size_t)(op->num_dn + 32 + kRowThreads / 64);
+    p.rk = k_hv_rows_half;
+  } else if (split) {
+    p.lds_bytes = row_bytes * R;
+    p.rk = R == 1 ? k_hv_rows_off<1> : k_hv_rows_off<4>;
+  } else {
+    p.lds_bytes = row_bytes * R + 8 * 128;
+    p.rk = uni ? k_hv_rows<true> : k_hv_rows<false>;
+  }
+  if (!ensure_row_lds(p.rk, p.lds_bytes)) return p;
+  p.grid_r = (op->num_up + R - 1) / R;
+  if (split) {
+    // 64-column tiles: one hop per load instruction, so the source row and
+    // the sign are wave-uniform
+    p.tw = env_int("CMPY_AMD_HV_TW", 64);
+    if (p.tw != 16 && p.tw != 32) p.tw = 64;
+    p.rch = env_int("CMPY_AMD_HV_RCH", 32);
+    p.rch = std::max(4, std::min(256, p.rch)) / 4 * 4;
+    p.ck = col_kernel_split(p.tw, op->up.tgt_stride);
+  } else {
+    p.tw = env_int("CMPY_AMD_HV_TW", 32);
+    if (p.tw != 16 && p.tw != 32) p.tw = 32;
+    // rows per column-pass block: a multiple of the rows one pass of the block
+    // covers (256 / TW), so every wave runs the same number of row iterations
+    const int rpi = 256 / p.tw;
+    p.rch = env_int("CMPY_AMD_HV_RCH", 32);
+    p.rch = std::max(rpi, std::min(kColMaxRows, p.rch));
+    p.rch = (p.rch / rpi) * rpi;
+    p.ck = p.tw == 16 ? col_kernel_tw<16>(maxh_up, uni) : col_kernel_tw<32>(maxh_up, uni);
+  }
+  p.ntiles = (op->num_dn + p.tw - 1) / p.tw;
+  p.nrc = (op->num_up + p.rch - 1) / p.rch;
+  p.grid_c = 8 * ((p.ntiles + 7) / 8) * p.nrc;
+  p.ok = p.grid_c + p.grid_r < (1ll << 31);
+  return p;
+}
+
+static int launch_two_pass(const ed_op* op, const TwoPassPlan& p, const double* x, double* y,
+                           const HvEpilogue& epi, hipStream_t s, int* num_partials) {
+  const Hv2Args a = make_hv2_args(op);
+  hipLaunchKernelGGL(p.rk, dim3((unsigned)p.grid_r), dim3(kRowThreads), p.lds_bytes, s, a,
+                     p.rows_per_block, x, y, epi);
+  double* part_c = epi.partial ? epi.partial + p.grid_r : nullptr;
+  hipLaunchKernelGGL(p.ck, dim3((unsigned)p.grid_c), dim3(256), 0, s, a, x, y, epi, p.ntiles, p.nrc, p.rch, part_c);
+  ED_HIP(hipGetLastError());
+  if (num_partials) *num_partials = (int)(p.grid_r + p.grid_c);
+  return ED_OK;
+}
+
 static int pick_rb(const ed_op* op) {
   if (op->num_up >= 4096) return 4;
   if (op->num_up >= 64) return 2;
@@ -134,6 +846,8 @@ static void grid_shape(const ed_op* op, int rb, int64_t* nrb, int64_t* nct) {
 }
 
 int matvec_num_partials(const ed_op* op) {
+  const TwoPassPlan p = plan_two_pass(op);
+  if (p.ok) return (int)(p.grid_r + p.grid_c);
   int64_t nrb, nct;
   grid_shape(op, pick_rb(op), &nrb, &nct);
   return (int)(nrb * nct);
@@ -154,6 +868,8 @@ static void launch_rb(int rb, const HvArgs& a, const double* x, double* y, const
 
 int launch_matvec(const ed_op* op, const double* x, double* y, const HvEpilogue& epi, hipStream_t s,
                   int* num_partials) {
+  const TwoPassPlan p = plan_two_pass(op);
+  if (p.ok) return launch_two_pass(op, p, x, y, epi, s, num_partials);
   const HvArgs a = make_hv_args(op);
   const int rb = pick_rb(op);
   int64_t nrb, nct;
