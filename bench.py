@@ -39,6 +39,9 @@ def parse():
     p.add_argument("--no-lanczos", action="store_true")
     p.add_argument("--no-secondary", action="store_true")
     p.add_argument("--size", choices=["4x4", "4x3"], default="4x4")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run device copies of one sector vector (the reference kernel for "
+                        "scaling the FETCH_SIZE / WRITE_SIZE counters to bytes)")
     return p.parse_args()
 
 
@@ -70,6 +73,18 @@ def main():
     from cmpy_amd import SectorHamiltonOperator, groundstate
 
     ns, n, bonds = model_spec(args.size)
+    if args.copy_calibration:
+        from math import comb
+
+        D = comb(ns, n) ** 2
+        x = torch.rand(D, dtype=torch.float64, device="cuda") - 0.5
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"dim": D, "bytes_read": 8 * D, "bytes_written": 8 * D,
+                                               "copies": 3}}), flush=True)
+        return
     U, t = 4.0, 1.0
     log(f"building the {args.size} operator (sector ({n},{n}))")
     t0 = time.perf_counter()
