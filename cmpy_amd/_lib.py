@@ -77,14 +77,17 @@ class LanczosResult(C.Structure):
 
 class BatchDesc(C.Structure):
     _fields_ = [
-        ("num_models", C.c_int32), ("num_bath", C.POINTER(C.c_int32)),
+        ("num_models", C.c_int64), ("num_bath", C.POINTER(C.c_int32)),
         ("n_up", C.POINTER(C.c_int32)), ("n_dn", C.POINTER(C.c_int32)),
         ("u", C.POINTER(C.c_double)), ("eps_imp", C.POINTER(C.c_double)),
         ("eps_bath", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)),
         ("max_bath", C.c_int32), ("sign_width", C.c_int32), ("max_iter", C.c_int32),
-        ("cf_steps", C.c_int32), ("tol", C.c_double), ("block", C.c_int32),
-        ("keep_basis", C.c_int32),
+        ("cf_steps", C.c_int32), ("check_every", C.c_int32), ("pad", C.c_int32),
+        ("tol", C.c_double), ("seed", C.c_uint64), ("cf_tol", C.c_double),
     ]
+
+
+BATCH_MAX_BATH = 15  # ED_BATCH_MAX_BATH
 
 
 vp = C.c_void_p
@@ -121,6 +124,11 @@ SIGNATURES = {
     "ed_gemm_nt": (C.c_int, [vp, vp, vp, i64, i64, i64, vp]),
     "ed_occupation": (C.c_int, [vp, i64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp]),
     "ed_dense_eigh": (C.c_int, [vp, i64, vp, vp]),
+    "ed_batch_create": (C.c_int, [P(vp), P(BatchDesc), vp]),
+    "ed_batch_destroy": (C.c_int, [vp]),
+    "ed_batch_config": (C.c_int, [vp, i64, i32, i32, vp]),
+    "ed_batch_info": (C.c_int, [vp, P(i64), P(i32), P(i32), P(dbl)]),
+    "ed_batch_solve": (C.c_int, [vp, P(dbl), i32, P(dbl), P(dbl), P(i32), P(dbl), P(dbl), vp]),
 }
 
 _lib = None

@@ -1,0 +1,1048 @@
+// Batched Anderson-impurity solver: many SingleImpurityAndersonModel instances
+// of uneven bath size in one persistent launch.
+//
+// No reference counterpart: cmpy solves one model per call
+// (cmpy/models/anderson.py:163-164 -> cmpy/exactdiag.py:137-245).  Per model
+// this computes what the single-model path does -- ground state of the
+// (n_up, n_dn) sector by Lanczos (cmpy/exactdiag.py:24-43), then the impurity
+// Green's function G(i w_n) = particle + hole continued fraction from
+// c+_{0,up} psi0 and c_{0,up} psi0 (the T = 0 limit of
+// cmpy/exactdiag.py:110-134) -- with the model's Hamiltonian built as
+// SingleImpurityAndersonModel does (cmpy/models/anderson.py:147-158): site 0
+// is the impurity (eps_imp, U), sites 1..nb the bath (eps_k), bonds (0, k)
+// with hopping V_k, hopping signs of width `sign_width` (0 = the reference's
+// all-positive signs).
+//
+// Work layout.  Models are split by sector size into two queues, both sorted
+// by decreasing cost.  Workgroups [0, wave_blocks) run "wave mode": each wave
+// takes one small model at a time; the other workgroups run "block mode": the
+// whole workgroup takes one large model.  Every group pops models from its
+// queue with one atomic add until the queue is empty, so every wave reaches
+// the kernel's end.  All vectors and spin tables of a group live in its slot
+// of a global workspace (L1/L2-resident for these sizes); Lanczos scalars are
+// reduced in a fixed order, so results are deterministic for a given
+// configuration.
+#include "ed_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+
+namespace ed {
+namespace batch {
+
+constexpr int kMaxBath = ED_BATCH_MAX_BATH;
+constexpr int kMaxSites = kMaxBath + 1;
+constexpr int kBlock = 1024;  // threads per workgroup
+
+struct ModelParams {
+  int32_t nb, n_up, n_dn, pad;
+  double u, eps_imp;  // eps_imp already minus mu
+  double eps[kMaxBath];
+  double v[kMaxBath];
+};
+
+struct Config {
+  int64_t num_models;
+  const ModelParams* params;
+  const int32_t* queue_wave;  // model indices, decreasing cost
+  const int32_t* queue_block;
+  int32_t n_wave, n_block;
+  int32_t* counters;          // [2]: next position in each queue
+  int32_t wave_blocks;        // workgroups [0, wave_blocks) run wave mode
+  // slot workspace
+  char* ws_wave;
+  char* ws_block;
+  int64_t slot_bytes_wave, slot_bytes_block;
+  int32_t list_cap_wave, list_cap_block;   // spin-list capacity (states)
+  int64_t vec_cap_wave, vec_cap_block;     // vector capacity (doubles)
+  // algorithm
+  int32_t max_iter, cf_steps, check_every, sign_width;
+  double tol, cf_tol;
+  uint64_t seed;
+  const double* wn;  // [nw] Matsubara frequencies (imaginary parts)
+  int32_t nw;
+  double wmin;       // smallest |w_n|: where the continued fraction converges last
+  // outputs
+  double* e0;     // [num_models]
+  double* g;      // [num_models][nw][2]
+  int32_t* iters; // [num_models]
+  double* resid;  // [num_models][2]: Ritz residual, explicit residual
+};
+
+// ---------------------------------------------------------------------------
+// Group: one wave (wave mode) or the whole workgroup (block mode).
+// ---------------------------------------------------------------------------
+template <bool BLOCK>
+struct Group {
+  int rank, size;
+  double* red;  // LDS: 2 * 16 doubles (block mode)
+
+  __device__ void sync() const {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    if (BLOCK)
+      __syncthreads();
+    else
+      __builtin_amdgcn_wave_barrier();
+  }
+  // Sum over the group, the same value on every member, fixed order.
+  __device__ double sum(double v) const {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (!BLOCK) return v;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+    __syncthreads();
+    return s;
+  }
+  __device__ double max(double v) const {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if (!BLOCK) return v;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) s = fmax(s, red[w]);
+    __syncthreads();
+    return s;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Per-group slot of the workspace: spin lists of the three up sectors
+// (n_up - 1, n_up, n_up + 1) and the dn sector, four vectors and the
+// coefficient arrays.  Only the base and the capacities are kept; every
+// pointer is recomputed from them when used (cheap scalar arithmetic, and it
+// keeps the solver's live register set small).
+// ---------------------------------------------------------------------------
+struct SpinList {
+  const uint32_t* st;
+  const double* dg;
+  const int32_t* cnt;
+  const int32_t* tgt;  // [n][kMaxBath] hop targets
+  const double* val;   // [n][kMaxBath] hop values sign * V_k
+  int32_t n;
+};
+
+__host__ __device__ inline int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+struct SlotRef {
+  char* base;
+  int32_t lcap;   // spin-list capacity (states)
+  int64_t vcap;   // vector capacity (doubles)
+  int32_t miter;  // max_iter
+  int32_t csteps; // cf_steps
+
+  // byte sizes of the pieces, each rounded to 256
+  __host__ __device__ int64_t list_bytes() const {
+    return align_up(4ll * lcap, 256) + align_up(8ll * lcap, 256) + align_up(4ll * lcap, 256) +
+           align_up(4ll * lcap * kMaxBath, 256) + align_up(8ll * lcap * kMaxBath, 256);
+  }
+  __host__ __device__ int64_t vec_bytes() const { return align_up(8 * vcap, 256); }
+  __host__ __device__ int64_t coef_bytes(int32_t n) const { return align_up(8ll * (n + 2), 256); }
+  __host__ __device__ int64_t bytes() const {
+    return 4 * list_bytes() + 4 * vec_bytes() + 4 * coef_bytes(miter) + 4 * coef_bytes(csteps) + 256;
+  }
+  // list l: 0..2 up sectors n_up-1, n_up, n_up+1; 3 dn sector
+  __device__ char* list_base(int l) const { return base + l * list_bytes(); }
+  __device__ uint32_t* st(int l) const { return (uint32_t*)list_base(l); }
+  __device__ double* dg(int l) const { return (double*)(list_base(l) + align_up(4ll * lcap, 256)); }
+  __device__ int32_t* cnt(int l) const {
+    return (int32_t*)(list_base(l) + align_up(4ll * lcap, 256) + align_up(8ll * lcap, 256));
+  }
+  __device__ int32_t* tgt(int l) const { return (int32_t*)((char*)cnt(l) + align_up(4ll * lcap, 256)); }
+  __device__ double* val(int l) const { return (double*)((char*)tgt(l) + align_up(4ll * lcap * kMaxBath, 256)); }
+  // vectors 0..3: uprev, u, w, psi
+  __device__ double* vec(int i) const { return (double*)(base + 4 * list_bytes() + i * vec_bytes()); }
+  __device__ char* coef_base() const { return base + 4 * list_bytes() + 4 * vec_bytes(); }
+  __device__ double* alpha() const { return (double*)coef_base(); }
+  __device__ double* beta() const { return (double*)(coef_base() + coef_bytes(miter)); }
+  __device__ double* sv() const { return (double*)(coef_base() + 2 * coef_bytes(miter)); }
+  __device__ double* tc() const { return (double*)(coef_base() + 3 * coef_bytes(miter)); }
+  // continued-fraction coefficients: 0 ap, 1 bp, 2 am, 3 bm
+  __device__ double* cfc(int i) const { return (double*)(coef_base() + 4 * coef_bytes(miter) + i * coef_bytes(csteps)); }
+  __device__ double* scal() const { return (double*)(coef_base() + 4 * coef_bytes(miter) + 4 * coef_bytes(csteps)); }
+  __device__ SpinList list(int l, int32_t n) const { return SpinList{st(l), dg(l), cnt(l), tgt(l), val(l), n}; }
+};
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Spin list of the sector (ns, n): states, diagonal sum_i eps_i n_i (ascending
+// site order), hops over the bonds (0, k+1).
+template <bool BLOCK>
+__device__ int build_list(const Group<BLOCK>& g, const ModelParams& p, int ns, int n, int width, const SlotRef& sl,
+                          int l) {
+  const int count = (n < 0 || n > ns) ? 0 : (int)binom_u(ns, n);
+  uint32_t* st = sl.st(l);
+  double* dg = sl.dg(l);
+  int32_t* cnt = sl.cnt(l);
+  int32_t* tgt = sl.tgt(l);
+  double* val = sl.val(l);
+  for (int i = g.rank; i < count; i += g.size) {
+    const uint32_t s = comb_unrank(i, ns, n);
+    st[i] = s;
+    double d = (s & 1u) ? p.eps_imp : 0.0;
+    for (int k = 0; k < p.nb; ++k)
+      if (s & (2u << k)) d += p.eps[k];
+    dg[i] = d;
+    int c = 0;
+    for (int k = 0; k < p.nb; ++k) {
+      const uint32_t m = 1u | (2u << k);
+      const uint32_t b = s & m;
+      if (b == 0 || b == m) continue;
+      tgt[i * kMaxBath + c] = (int32_t)comb_rank(s ^ m);
+      val[i * kMaxBath + c] = hop_sign(s, width, 0, k + 1) < 0 ? -p.v[k] : p.v[k];
+      ++c;
+    }
+    cnt[i] = c;
+  }
+  return count;
+}
+
+// One Lanczos step on sector (U, Dn), lazily scaled: v_j = sj * u, v_{j-1} =
+// sp * uprev.  w = sj H u - bj * sp * uprev; then (if alpha < 0 on entry
+// coefficient unknown) a = <w, v_j>; w -= a v_j; returns |w|^2 through *nrm2.
+// With known coefficients (regeneration passes) a_in is used as given.
+template <bool BLOCK>
+__device__ void lanczos_step(const Group<BLOCK>& g, const ModelParams& p, const SpinList& U, const SpinList& Dn,
+                             const double* __restrict__ u, const double* __restrict__ uprev, double* __restrict__ w,
+                             double sj, double sp, double bj, bool known, double a_in, double* a_out, double* nrm2) {
+  const int nd = Dn.n;
+  const int64_t D = (int64_t)U.n * nd;
+  double dot = 0.0;
+  // sweep 1: w = sj H u - bj sp uprev, dot <w, u>
+  {
+    int64_t k = g.rank;
+    int iu = (int)(k / (nd > 0 ? nd : 1)), id = (int)(k - (int64_t)iu * nd);
+    for (; k < D; k += g.size) {
+      const double xk = u[k];
+      double acc = (U.dg[iu] + Dn.dg[id] + ((U.st[iu] & Dn.st[id] & 1u) ? p.u : 0.0)) * xk;
+      const int cu = U.cnt[iu];
+      for (int e = 0; e < cu; ++e)
+        acc += U.val[iu * kMaxBath + e] * u[(int64_t)U.tgt[iu * kMaxBath + e] * nd + id];
+      const int cd = Dn.cnt[id];
+      for (int e = 0; e < cd; ++e) acc += Dn.val[id * kMaxBath + e] * u[(int64_t)iu * nd + Dn.tgt[id * kMaxBath + e]];
+      double wk = sj * acc;
+      if (bj != 0.0) wk -= bj * sp * uprev[k];
+      w[k] = wk;
+      dot += wk * xk;
+      id += g.size;
+      while (id >= nd) {
+        id -= nd;
+        ++iu;
+      }
+    }
+  }
+  double a = a_in;
+  if (!known) a = sj * g.sum(dot);
+  *a_out = a;
+  g.sync();
+  // sweep 2: w -= a sj u, |w|^2
+  double n2 = 0.0;
+  for (int64_t k = g.rank; k < D; k += g.size) {
+    const double wk = w[k] - a * sj * u[k];
+    w[k] = wk;
+    n2 += wk * wk;
+  }
+  *nrm2 = known ? 0.0 : g.sum(n2);
+  g.sync();
+}
+
+// Lowest eigenpair of the m x m tridiagonal (alpha, beta): multisection over
+// the group's lanes on the Sturm count, then inverse iteration at a shift just
+// below the eigenvalue (T - shift is positive definite there, so the Thomas
+// solve needs no pivoting).  Writes the unit eigenvector to sv (tc is scratch)
+// and returns theta and |last component| through theta / slast.
+template <bool BLOCK>
+__device__ void tri_lowest(const Group<BLOCK>& g, const double* alpha, const double* beta, int m, double* sv,
+                           double* tc, double* scal, double* theta, double* slast) {
+  double lo = 1e300, hi = -1e300;
+  for (int i = g.rank; i < m; i += g.size) {
+    const double r = (i > 0 ? fabs(beta[i - 1]) : 0.0) + (i < m - 1 ? fabs(beta[i]) : 0.0);
+    lo = fmin(lo, alpha[i] - r);
+    hi = fmax(hi, alpha[i] + r);
+  }
+  lo = -g.max(-lo);
+  hi = g.max(hi);
+  constexpr int P = 64;  // shifts tested per round, one per lane of the first wave
+  for (int round = 0; round < 64; ++round) {
+    if (hi - lo <= 4.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)) + 1e-300) break;
+    // lane r < P tests x_r = lo + (hi - lo) (r + 1) / (P + 1): is an eigenvalue below it?
+    bool below = false;
+    if (g.rank < P) {
+      const double xr = lo + (hi - lo) * (double)(g.rank + 1) / (double)(P + 1);
+      double q = alpha[0] - xr;
+      int c = q < 0.0 ? 1 : 0;
+      for (int i = 1; i < m; ++i) {
+        const double qq = q != 0.0 ? q : 1e-300;
+        q = alpha[i] - xr - beta[i - 1] * beta[i - 1] / qq;
+        c += q < 0.0 ? 1 : 0;
+      }
+      below = c >= 1;
+    }
+    // r1 = first shift with an eigenvalue below it (P if none): the lowest
+    // eigenvalue lies in [x_{r1-1}, x_{r1}] with x_{-1} = lo, x_P = hi
+    const int r1 = (int)(-g.max(below ? -(double)g.rank : -(double)P));
+    const double nlo = lo + (hi - lo) * (double)r1 / (double)(P + 1);
+    const double nhi = r1 >= P ? hi : lo + (hi - lo) * (double)(r1 + 1) / (double)(P + 1);
+    lo = nlo;
+    hi = nhi;
+  }
+  const double th = 0.5 * (lo + hi);
+  if (g.rank == 0) {
+    const double shift = th - fmax(1e-10 * fabs(th), 1e-12);
+    for (int i = 0; i < m; ++i) sv[i] = 1.0;
+    for (int it = 0; it < 2; ++it) {
+      // Thomas algorithm on (T - shift) x = sv: forward elimination, then back substitution
+      double c_prev = 0.0, d_prev = 0.0;
+      for (int i = 0; i < m; ++i) {
+        const double bl = i > 0 ? beta[i - 1] : 0.0;
+        const double den = (alpha[i] - shift) - bl * c_prev;
+        const double ci = i < m - 1 ? beta[i] / den : 0.0;
+        const double di = (sv[i] - bl * d_prev) / den;
+        tc[i] = ci;
+        sv[i] = di;
+        c_prev = ci;
+        d_prev = di;
+      }
+      for (int i = m - 2; i >= 0; --i) sv[i] -= tc[i] * sv[i + 1];
+      double nrm = 0.0;
+      for (int i = 0; i < m; ++i) nrm += sv[i] * sv[i];
+      nrm = 1.0 / sqrt(nrm);
+      for (int i = 0; i < m; ++i) sv[i] *= nrm;
+    }
+    if (sv[0] < 0.0)
+      for (int i = 0; i < m; ++i) sv[i] = -sv[i];
+    scal[0] = th;
+    scal[1] = fabs(sv[m - 1]);
+  }
+  g.sync();
+  *theta = scal[0];
+  *slast = scal[1];
+  g.sync();
+}
+
+struct cplx {
+  double re, im;
+};
+__device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
+  // Smith's algorithm
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double r = b.im / b.re, d = b.re + b.im * r;
+    return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  }
+  const double r = b.re / b.im, d = b.re * r + b.im;
+  return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+
+// s * norm2 / (s z + e0 - a0 - b0^2 / (s z + e0 - a1 - ...)), z = i w.
+__device__ cplx cf_value(const double* a, const double* b, int m, double norm2, double e0, double s, double w) {
+  const cplx z = {e0, s * w};
+  cplx t = {0.0, 0.0};
+  for (int j = m - 1; j >= 1; --j) t = cdiv({b[j - 1] * b[j - 1], 0.0}, {z.re - a[j] - t.re, z.im - t.im});
+  const cplx r = cdiv({s * norm2, 0.0}, {z.re - a[0] - t.re, z.im - t.im});
+  return r;
+}
+
+// Lanczos coefficients of the sector (U, Dn) from the start vector in vec(1)
+// (unnormalised, |u|^2 = n2), stopping at breakdown, at cap steps, or when the
+// continued fraction s n2 / (s z + e0 - a0 - ...) at z = i wmin has changed by
+// at most cf_tol (relative) over each of the last two 8-step checks (the
+// fraction converges last at the frequency closest to the real axis).
+// Returns the step count.
+template <bool BLOCK>
+__device__ int cf_coeffs(const Group<BLOCK>& g, const ModelParams& p, const SpinList& U, const SpinList& Dn,
+                         const SlotRef& sl, double n2, int cap, double e0, double sgn, double wmin, double cf_tol,
+                         double* a, double* b) {
+  double* u = sl.vec(1);
+  double* up = sl.vec(0);
+  double* w = sl.vec(2);
+  double* scal = sl.scal();
+  double sj = 1.0 / sqrt(n2), sp = 0.0, bj = 0.0;
+  int m = 0, calm = 0;
+  cplx prev = {0.0, 0.0};
+  for (int j = 0; j < cap; ++j) {
+    double aj, nn;
+    lanczos_step<BLOCK>(g, p, U, Dn, u, up, w, sj, sp, bj, false, 0.0, &aj, &nn);
+    a[j] = aj;
+    m = j + 1;
+    const double bn = sqrt(nn);
+    if (j == cap - 1) break;
+    if (!(bn > 1e-12 * (fabs(aj) + bj))) break;  // invariant subspace
+    b[j] = bn;
+    if (m % 8 == 0) {
+      g.sync();
+      if (g.rank == 0) {
+        const cplx c = cf_value(a, b, m, n2, e0, sgn, wmin);
+        const double d = hypot(c.re - prev.re, c.im - prev.im);
+        scal[4] = d <= cf_tol * hypot(c.re, c.im) ? 1.0 : 0.0;
+        prev = c;
+      }
+      g.sync();
+      calm = scal[4] != 0.0 ? calm + 1 : 0;
+      g.sync();
+      if (calm >= 2) break;
+    }
+    double* t = up;
+    up = u;
+    u = w;
+    w = t;
+    sp = sj;
+    sj = 1.0 / bn;
+    bj = bn;
+  }
+  return m;
+}
+
+__device__ __forceinline__ double start_value(uint64_t seed, int64_t k) {
+  return (double)(mix64(seed ^ (0xD1B54A32D192ED03ull * (uint64_t)(k + 1))) >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+}
+
+template <bool BLOCK>
+__device__ void solve_model(const Group<BLOCK>& g, const Config& cf, int mi, const SlotRef& sl) {
+  const ModelParams& p = cf.params[mi];
+  const int ns = p.nb + 1;
+  const int width = cf.sign_width;
+  const int n_m = build_list<BLOCK>(g, p, ns, p.n_up - 1, width, sl, 0);
+  const int n_0 = build_list<BLOCK>(g, p, ns, p.n_up, width, sl, 1);
+  const int n_p = build_list<BLOCK>(g, p, ns, p.n_up + 1, width, sl, 2);
+  const int nd = build_list<BLOCK>(g, p, ns, p.n_dn, width, sl, 3);
+  g.sync();
+  const SpinList U = sl.list(1, n_0);
+  const SpinList Dn = sl.list(3, nd);
+  const int64_t D = (int64_t)n_0 * nd;
+  double* const alpha = sl.alpha();
+  double* const beta = sl.beta();
+  double* const psi = sl.vec(3);
+
+  // ---- ground state, pass 1: coefficients and convergence ----
+  const uint64_t seed = cf.seed ^ (0xA0761D6478BD642Full * (uint64_t)(mi + 1));
+  double n2 = 0.0;
+  {
+    double* u0 = sl.vec(1);
+    for (int64_t k = g.rank; k < D; k += g.size) {
+      const double r = start_value(seed, k);
+      u0[k] = r;
+      n2 += r * r;
+    }
+  }
+  n2 = g.sum(n2);
+  g.sync();
+  const double s0 = 1.0 / sqrt(n2);
+  double* u = sl.vec(1);
+  double* up = sl.vec(0);
+  double* w = sl.vec(2);
+  double sj = s0, sp = 0.0, bj = 0.0;
+  int m = 0;
+  double theta = 0.0, slast = 0.0, ritz = 1e300;
+  const int cap = (int)((int64_t)cf.max_iter < D ? (int64_t)cf.max_iter : D);
+  for (int j = 0; j < cap; ++j) {
+    double aj, nn;
+    lanczos_step<BLOCK>(g, p, U, Dn, u, up, w, sj, sp, bj, false, 0.0, &aj, &nn);
+    m = j + 1;
+    const double bn = sqrt(nn);
+    if (g.rank == 0) {
+      alpha[j] = aj;
+      beta[j] = bn;
+    }
+    g.sync();
+    const bool breakdown = !(bn > 1e-12 * (fabs(aj) + bj));
+    const bool check = breakdown || m == cap || (m >= cf.check_every && m % cf.check_every == 0);
+    if (check) {
+      tri_lowest<BLOCK>(g, alpha, beta, m, sl.sv(), sl.tc(), sl.scal(), &theta, &slast);
+      ritz = breakdown ? 0.0 : bn * slast;
+      if (ritz <= cf.tol || breakdown) break;
+    }
+    double* t = up;
+    up = u;
+    u = w;
+    w = t;
+    sp = sj;
+    sj = 1.0 / bn;
+    bj = bn;
+  }
+  // ---- pass 2: psi = sum_j s_j v_j, regenerated with the stored coefficients ----
+  u = sl.vec(1);
+  up = sl.vec(0);
+  w = sl.vec(2);
+  for (int64_t k = g.rank; k < D; k += g.size) {
+    u[k] = start_value(seed, k);
+    psi[k] = 0.0;
+  }
+  g.sync();
+  sj = s0;
+  sp = 0.0;
+  bj = 0.0;
+  {
+    const double* sv = sl.sv();
+    for (int j = 0; j < m; ++j) {
+      const double cj = sv[j] * sj;
+      for (int64_t k = g.rank; k < D; k += g.size) psi[k] += cj * u[k];
+      if (j == m - 1) break;
+      double aj, nn;
+      lanczos_step<BLOCK>(g, p, U, Dn, u, up, w, sj, sp, bj, true, alpha[j], &aj, &nn);
+      const double bn = beta[j];
+      double* t = up;
+      up = u;
+      u = w;
+      w = t;
+      sp = sj;
+      sj = 1.0 / bn;
+      bj = bn;
+    }
+  }
+  g.sync();
+  // normalise psi and take the explicit residual |H psi - theta psi|
+  double pn = 0.0;
+  for (int64_t k = g.rank; k < D; k += g.size) pn += psi[k] * psi[k];
+  pn = g.sum(pn);
+  g.sync();
+  {
+    const double inv = 1.0 / sqrt(pn);
+    for (int64_t k = g.rank; k < D; k += g.size) psi[k] *= inv;
+  }
+  g.sync();
+  double res2;
+  {
+    double aj, nn;
+    // w = H psi - theta psi (unit scale, no previous vector, known coefficient)
+    double* wr = sl.vec(2);
+    lanczos_step<BLOCK>(g, p, U, Dn, psi, sl.vec(0), wr, 1.0, 0.0, 0.0, true, theta, &aj, &nn);
+    double r2 = 0.0;
+    for (int64_t k = g.rank; k < D; k += g.size) r2 += wr[k] * wr[k];
+    res2 = g.sum(r2);
+    g.sync();
+  }
+
+  // ---- particle (c+_{0,up} psi, sector n_up + 1) and hole (c_{0,up} psi, n_up - 1) ----
+  int mp = 0, mm = 0;
+  double np2 = 0.0, nm2 = 0.0;
+  for (int part = 0; part < 2; ++part) {
+    const int l = part == 0 ? 2 : 0;
+    const SpinList T = sl.list(l, part == 0 ? n_p : n_m);
+    if (T.n == 0) continue;
+    const int64_t Dt = (int64_t)T.n * nd;
+    double* phi = sl.vec(1);
+    double nn = 0.0;
+    // gather: target row it takes source row rank(st ^ 1) when its bit 0 fits
+    for (int64_t k = g.rank; k < Dt; k += g.size) {
+      const int it = (int)(k / nd), id = (int)(k - (int64_t)it * nd);
+      const uint32_t st = T.st[it];
+      const bool ok = part == 0 ? (st & 1u) != 0 : (st & 1u) == 0;
+      double v = 0.0;
+      if (ok) v = psi[(int64_t)comb_rank(st ^ 1u) * nd + id];
+      phi[k] = v;
+      nn += v * v;
+    }
+    nn = g.sum(nn);
+    g.sync();
+    if (!(nn > 0.0)) continue;
+    if (part == 0) {
+      np2 = nn;
+      mp = cf_coeffs<BLOCK>(g, p, T, Dn, sl, nn, cf.cf_steps, theta, 1.0, cf.wmin, cf.cf_tol, sl.cfc(0), sl.cfc(1));
+    } else {
+      nm2 = nn;
+      mm = cf_coeffs<BLOCK>(g, p, T, Dn, sl, nn, cf.cf_steps, theta, -1.0, cf.wmin, cf.cf_tol, sl.cfc(2), sl.cfc(3));
+    }
+  }
+  // coefficient arrays were written by every member with the same values
+  g.sync();
+  for (int n = g.rank; n < cf.nw; n += g.size) {
+    cplx gsum = {0.0, 0.0};
+    if (mp > 0) {
+      const cplx c = cf_value(sl.cfc(0), sl.cfc(1), mp, np2, theta, 1.0, cf.wn[n]);
+      gsum.re += c.re;
+      gsum.im += c.im;
+    }
+    if (mm > 0) {
+      // hole part: -norm2 / (-z + e0 - a0 - ...) = norm2 / (z - e0 + a0 + ...)
+      const cplx c = cf_value(sl.cfc(2), sl.cfc(3), mm, nm2, theta, -1.0, cf.wn[n]);
+      gsum.re += c.re;
+      gsum.im += c.im;
+    }
+    cf.g[((int64_t)mi * cf.nw + n) * 2] = gsum.re;
+    cf.g[((int64_t)mi * cf.nw + n) * 2 + 1] = gsum.im;
+  }
+  if (g.rank == 0) {
+    cf.e0[mi] = theta;
+    cf.iters[mi] = m;
+    cf.resid[2 * mi] = ritz;
+    cf.resid[2 * mi + 1] = sqrt(res2);
+  }
+  g.sync();
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_solve(Config cf) {
+  __shared__ double red[32];
+  __shared__ int popped;
+  if ((int)blockIdx.x < cf.wave_blocks) {
+    // wave mode
+    const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    Group<false> g{(int)(threadIdx.x & 63), 64, nullptr};
+    const int64_t slot = (int64_t)blockIdx.x * (blockDim.x >> 6) + wid;
+    const SlotRef sl{cf.ws_wave + slot * cf.slot_bytes_wave, cf.list_cap_wave, cf.vec_cap_wave, cf.max_iter,
+                     cf.cf_steps};
+    while (true) {
+      int q = 0;
+      if (g.rank == 0) q = atomicAdd(&cf.counters[0], 1);
+      q = __builtin_amdgcn_readfirstlane(__shfl(q, 0, 64));  // uniform model index
+      if (q >= cf.n_wave) break;
+      solve_model<false>(g, cf, __builtin_amdgcn_readfirstlane(cf.queue_wave[q]), sl);
+    }
+  } else {
+    Group<true> g{(int)threadIdx.x, (int)blockDim.x, red};
+    const int64_t slot = (int64_t)blockIdx.x - cf.wave_blocks;
+    const SlotRef sl{cf.ws_block + slot * cf.slot_bytes_block, cf.list_cap_block, cf.vec_cap_block, cf.max_iter,
+                     cf.cf_steps};
+    while (true) {
+      if (threadIdx.x == 0) popped = atomicAdd(&cf.counters[1], 1);
+      __syncthreads();
+      const int q = __builtin_amdgcn_readfirstlane(popped);
+      __syncthreads();
+      if (q >= cf.n_block) break;
+      solve_model<true>(g, cf, __builtin_amdgcn_readfirstlane(cf.queue_block[q]), sl);
+    }
+  }
+}
+
+__global__ void k_reset_counters(int32_t* c) {
+  if (threadIdx.x < 2) c[threadIdx.x] = 0;
+}
+
+}  // namespace batch
+}  // namespace ed
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+using namespace ed;
+using namespace ed::batch;
+
+struct ed_batch {
+  int device = 0;
+  int64_t num_models = 0;
+  std::vector<ModelParams> params;
+  std::vector<int64_t> dim;   // sector size per model
+  std::vector<int64_t> dmax;  // largest of the three sectors per model
+  std::vector<int32_t> order; // models by decreasing cost
+  int32_t max_iter = 300, cf_steps = 64, check_every = 8, sign_width = 0;
+  double tol = 1e-10, cf_tol = 1e-13;
+  uint64_t seed = 0x5EED;
+  // device
+  ModelParams* d_params = nullptr;
+  int32_t* d_queues = nullptr;  // [num_models]: wave queue then block queue
+  int32_t* d_counters = nullptr;
+  double *d_e0 = nullptr, *d_g = nullptr, *d_resid = nullptr, *d_wn = nullptr;
+  int32_t* d_iters = nullptr;
+  char *d_ws = nullptr;
+  int64_t ws_bytes = 0;
+  int32_t nw_cap = 0;
+  // launch configuration
+  int64_t wave_max_dim = -1;  // models with dim <= this run in wave mode (-1: not chosen)
+  int32_t grid = 0, wave_blocks = 0;
+  bool tuned = false;
+  double tune_ms = 0.0;
+  Config cfg{};
+  // captured graph of (reset counters, solve) for repeated solves
+  hipGraphExec_t graph = nullptr;
+  hipStream_t graph_stream = nullptr;
+};
+
+static int64_t sector_dim(int ns, int nu, int nd) {
+  return (int64_t)binom_u(ns, nu) * (int64_t)binom_u(ns, nd);
+}
+
+static void free_batch(ed_batch* b) {
+  if (!b) return;
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  for (void* p : {(void*)b->d_params, (void*)b->d_queues, (void*)b->d_counters, (void*)b->d_e0, (void*)b->d_g,
+                  (void*)b->d_resid, (void*)b->d_wn, (void*)b->d_iters, (void*)b->d_ws})
+    if (p) (void)hipFree(p);
+  delete b;
+}
+
+template <typename T>
+static int balloc(T** p, int64_t n) {
+  *p = nullptr;
+  if (n <= 0) n = 1;
+  const hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_error(ED_ERR_NOMEM, "hipMalloc of %lld bytes failed: %s", (long long)(sizeof(T) * n),
+                     hipGetErrorString(e));
+  }
+  return ED_OK;
+}
+
+// Lays out queues and workspace for a wave/block split at wave_max_dim and a
+// grid size; (re)allocates the workspace when it grows.
+static int configure(ed_batch* b, int64_t wave_max_dim, int grid_blocks, int wave_blocks, hipStream_t s) {
+  std::vector<int32_t> qw, qb;
+  int64_t vw = 1, vb = 1;
+  int32_t lw = 1, lb = 1;
+  for (int32_t mi : b->order) {
+    const ModelParams& p = b->params[mi];
+    const int ns = p.nb + 1;
+    int32_t lcap = 1;
+    for (int d = -1; d <= 1; ++d) lcap = std::max<int32_t>(lcap, (int32_t)binom_u(ns, std::max(0, p.n_up + d)));
+    lcap = std::max<int32_t>(lcap, (int32_t)binom_u(ns, p.n_dn));
+    if (b->dim[mi] <= wave_max_dim) {
+      qw.push_back(mi);
+      vw = std::max(vw, b->dmax[mi]);
+      lw = std::max(lw, lcap);
+    } else {
+      qb.push_back(mi);
+      vb = std::max(vb, b->dmax[mi]);
+      lb = std::max(lb, lcap);
+    }
+  }
+  if (qw.empty()) wave_blocks = 0;
+  if (qb.empty()) wave_blocks = grid_blocks;
+  if (!qw.empty() && wave_blocks == 0) wave_blocks = 1;
+  if (!qb.empty() && wave_blocks >= grid_blocks) grid_blocks = wave_blocks + 1;
+  const int64_t sw = SlotRef{nullptr, lw, vw, b->max_iter, b->cf_steps}.bytes();
+  const int64_t sb = SlotRef{nullptr, lb, vb, b->max_iter, b->cf_steps}.bytes();
+  const int64_t n_wave_slots = (int64_t)wave_blocks * (kBlock / 64);
+  const int64_t n_block_slots = grid_blocks - wave_blocks;
+  const int64_t bytes = n_wave_slots * sw + n_block_slots * sb + 256;
+  if (bytes > b->ws_bytes) {
+    if (b->d_ws) (void)hipFree(b->d_ws);
+    b->d_ws = nullptr;
+    b->ws_bytes = 0;
+    ED_TRY(balloc(&b->d_ws, bytes));
+    b->ws_bytes = bytes;
+  }
+  std::vector<int32_t> q(qw);
+  q.insert(q.end(), qb.begin(), qb.end());
+  ED_HIP(hipMemcpyAsync(b->d_queues, q.data(), sizeof(int32_t) * q.size(), hipMemcpyHostToDevice, s));
+  ED_HIP(hipStreamSynchronize(s));
+  Config& c = b->cfg;
+  c.num_models = b->num_models;
+  c.params = b->d_params;
+  c.queue_wave = b->d_queues;
+  c.queue_block = b->d_queues + qw.size();
+  c.n_wave = (int32_t)qw.size();
+  c.n_block = (int32_t)qb.size();
+  c.counters = b->d_counters;
+  c.wave_blocks = wave_blocks;
+  c.ws_wave = b->d_ws;
+  c.ws_block = b->d_ws + n_wave_slots * sw;
+  c.slot_bytes_wave = sw;
+  c.slot_bytes_block = sb;
+  c.list_cap_wave = lw;
+  c.list_cap_block = lb;
+  c.vec_cap_wave = vw;
+  c.vec_cap_block = vb;
+  c.max_iter = b->max_iter;
+  c.cf_steps = b->cf_steps;
+  c.check_every = b->check_every;
+  c.sign_width = b->sign_width;
+  c.tol = b->tol;
+  c.cf_tol = b->cf_tol;
+  c.seed = b->seed;
+  c.e0 = b->d_e0;
+  c.g = b->d_g;
+  c.iters = b->d_iters;
+  c.resid = b->d_resid;
+  b->wave_max_dim = wave_max_dim;
+  b->grid = grid_blocks;
+  b->wave_blocks = wave_blocks;
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return ED_OK;
+}
+
+static int launch(ed_batch* b, hipStream_t s) {
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, b->d_counters);
+  hipLaunchKernelGGL(k_batch_solve, dim3((unsigned)b->grid), dim3(kBlock), 0, s, b->cfg);
+  ED_HIP(hipGetLastError());
+  return ED_OK;
+}
+
+static double time_launch(ed_batch* b, hipStream_t s, int* rc) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    *rc = set_error(ED_ERR_HIP, "hipEventCreate failed");
+    return 0.0;
+  }
+  (void)hipEventRecord(e0, s);
+  *rc = launch(b, s);
+  (void)hipEventRecord(e1, s);
+  float ms = 0.0f;
+  if (*rc == ED_OK) {
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      *rc = set_error(ED_ERR_HIP, "batch solve failed: %s", hipGetErrorString(hipGetLastError()));
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms;
+}
+
+// Autotuner: tries wave/block splits and grid sizes on the batch itself and
+// keeps the fastest (results are recomputed by the caller's solve).
+static int autotune(ed_batch* b, hipStream_t s) {
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+  std::vector<int64_t> dims(b->dim);
+  std::sort(dims.begin(), dims.end());
+  dims.erase(std::unique(dims.begin(), dims.end()), dims.end());
+  std::vector<int64_t> splits = {-1};  // all block mode
+  for (int64_t d : dims)
+    if (d <= 65536) splits.push_back(d);
+  double best = 1e300;
+  int64_t best_split = -1;
+  int best_grid = cus, best_wave = 0;
+  for (int64_t split : splits) {
+    int64_t n_small = 0, n_large = 0;
+    for (int64_t d : b->dim) (d <= split ? n_small : n_large)++;
+    for (int per_cu : {1, 2}) {
+      const int grid = cus * per_cu;
+      // wave blocks in proportion to the small models' share of the work
+      double w_small = 0.0, w_all = 0.0;
+      for (size_t i = 0; i < b->dim.size(); ++i) {
+        const double c = (double)b->dim[i];
+        w_all += c;
+        if (b->dim[i] <= split) w_small += c;
+      }
+      int wb = n_small == 0 ? 0 : std::max(1, (int)std::lround(grid * w_small / std::max(w_all, 1.0)));
+      if (n_large > 0) wb = std::min(wb, grid - std::max<int>(1, (int)std::min<int64_t>(n_large, grid / 2)));
+      wb = std::max(wb, n_small > 0 ? 1 : 0);
+      ED_TRY(configure(b, split, grid, wb, s));
+      int rc = ED_OK;
+      const double ms = time_launch(b, s, &rc);
+      ED_TRY(rc);
+      if (ms < best) {
+        best = ms;
+        best_split = split;
+        best_grid = b->grid;
+        best_wave = b->wave_blocks;
+      }
+    }
+  }
+  ED_TRY(configure(b, best_split, best_grid, best_wave, s));
+  b->tuned = true;
+  b->tune_ms = best;
+  return ED_OK;
+}
+
+extern "C" int ed_batch_create(ed_batch** out, const ed_batch_desc* d, void* stream) {
+  clear_error();
+  ED_REQUIRE(out && d, "NULL argument");
+  *out = nullptr;
+  ED_REQUIRE(d->num_models > 0, "num_models must be positive");
+  ED_REQUIRE(d->num_bath && d->n_up && d->n_dn && d->u && d->eps_imp && d->eps_bath && d->v, "NULL model array");
+  ED_REQUIRE(d->max_bath >= 1, "max_bath must be positive");
+  ED_REQUIRE(d->max_iter >= 1 && d->max_iter <= 2000, "max_iter out of range (1..2000)");
+  ED_REQUIRE(d->cf_steps >= 1 && d->cf_steps <= 2000, "cf_steps out of range (1..2000)");
+  ED_REQUIRE(d->tol >= 0.0 && d->cf_tol >= 0.0, "tol and cf_tol must be non-negative");
+  ed_batch* b = new ed_batch();
+  (void)hipGetDevice(&b->device);
+  b->num_models = d->num_models;
+  b->max_iter = d->max_iter;
+  b->cf_steps = d->cf_steps;
+  b->check_every = d->check_every > 0 ? d->check_every : 8;
+  b->sign_width = d->sign_width;
+  b->tol = d->tol;
+  b->cf_tol = d->cf_tol;
+  b->seed = d->seed;
+  b->params.resize(d->num_models);
+  b->dim.resize(d->num_models);
+  b->dmax.resize(d->num_models);
+  for (int64_t i = 0; i < d->num_models; ++i) {
+    ModelParams& p = b->params[i];
+    std::memset(&p, 0, sizeof(p));
+    p.nb = d->num_bath[i];
+    p.n_up = d->n_up[i];
+    p.n_dn = d->n_dn[i];
+    const int ns = p.nb + 1;
+    if (p.nb < 1 || p.nb > kMaxBath || p.nb > d->max_bath || p.n_up < 0 || p.n_up > ns || p.n_dn < 0 ||
+        p.n_dn > ns) {
+      free_batch(b);
+      return set_error(ED_ERR_INVALID, "model %lld: num_bath must be 1..%d (and <= max_bath) with 0 <= n <= sites",
+                       (long long)i, kMaxBath);
+    }
+    if (d->sign_width < 0 || d->sign_width > 32) {
+      free_batch(b);
+      return set_error(ED_ERR_INVALID, "sign_width must be 0..32");
+    }
+    p.u = d->u[i];
+    p.eps_imp = d->eps_imp[i];
+    bool finite = std::isfinite(p.u) && std::isfinite(p.eps_imp);
+    for (int k = 0; k < p.nb; ++k) {
+      p.eps[k] = d->eps_bath[i * d->max_bath + k];
+      p.v[k] = d->v[i * d->max_bath + k];
+      finite = finite && std::isfinite(p.eps[k]) && std::isfinite(p.v[k]);
+    }
+    if (!finite) {
+      free_batch(b);
+      return set_error(ED_ERR_INVALID, "model %lld: parameters must be finite", (long long)i);
+    }
+    b->dim[i] = sector_dim(ns, p.n_up, p.n_dn);
+    int64_t dm = b->dim[i];
+    if (p.n_up + 1 <= ns) dm = std::max(dm, sector_dim(ns, p.n_up + 1, p.n_dn));
+    if (p.n_up >= 1) dm = std::max(dm, sector_dim(ns, p.n_up - 1, p.n_dn));
+    b->dmax[i] = dm;
+    if (b->dim[i] > (int64_t)1 << 24) {
+      free_batch(b);
+      return set_error(ED_ERR_LIMIT, "model %lld: sector of %lld states exceeds the batched solver's limit",
+                       (long long)i, (long long)b->dim[i]);
+    }
+  }
+  b->order.resize(d->num_models);
+  std::iota(b->order.begin(), b->order.end(), 0);
+  std::stable_sort(b->order.begin(), b->order.end(), [&](int32_t a, int32_t c) { return b->dim[a] > b->dim[c]; });
+  hipStream_t s = as_stream(stream);
+  int rc = ED_OK;
+  if ((rc = balloc(&b->d_params, d->num_models)) != ED_OK || (rc = balloc(&b->d_queues, d->num_models)) != ED_OK ||
+      (rc = balloc(&b->d_counters, 2)) != ED_OK || (rc = balloc(&b->d_e0, d->num_models)) != ED_OK ||
+      (rc = balloc(&b->d_iters, d->num_models)) != ED_OK || (rc = balloc(&b->d_resid, 2 * d->num_models)) != ED_OK) {
+    free_batch(b);
+    return rc;
+  }
+  if (hipMemcpyAsync(b->d_params, b->params.data(), sizeof(ModelParams) * d->num_models, hipMemcpyHostToDevice, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    free_batch(b);
+    return set_error(ED_ERR_HIP, "upload of model parameters failed");
+  }
+  *out = b;
+  return ED_OK;
+}
+
+extern "C" int ed_batch_destroy(ed_batch* b) {
+  clear_error();
+  free_batch(b);
+  return ED_OK;
+}
+
+extern "C" int ed_batch_config(ed_batch* b, int64_t wave_max_dim, int32_t grid_blocks, int32_t wave_blocks,
+                               void* stream) {
+  clear_error();
+  ED_REQUIRE(b, "NULL batch");
+  ED_REQUIRE(grid_blocks >= 1 && grid_blocks <= 65536, "grid_blocks out of range");
+  ED_REQUIRE(wave_blocks >= 0 && wave_blocks <= grid_blocks, "wave_blocks out of range");
+  b->tuned = true;
+  return configure(b, wave_max_dim, grid_blocks, wave_blocks, as_stream(stream));
+}
+
+extern "C" int ed_batch_solve(ed_batch* b, const double* wn, int32_t nw, double* e0, double* g, int32_t* iters,
+                              double* resid, double* elapsed_ms, void* stream) {
+  clear_error();
+  ED_REQUIRE(b, "NULL batch");
+  ED_REQUIRE(nw >= 0 && (nw == 0 || wn), "bad frequency list");
+  hipStream_t s = as_stream(stream);
+  if (nw > b->nw_cap) {
+    if (b->d_g) (void)hipFree(b->d_g);
+    if (b->d_wn) (void)hipFree(b->d_wn);
+    b->d_g = nullptr;
+    b->d_wn = nullptr;
+    b->nw_cap = 0;
+    ED_TRY(balloc(&b->d_g, 2 * b->num_models * (int64_t)nw));
+    ED_TRY(balloc(&b->d_wn, nw));
+    b->nw_cap = nw;
+    if (b->graph) {
+      (void)hipGraphExecDestroy(b->graph);
+      b->graph = nullptr;
+    }
+  }
+  double wmin = 1.0;
+  for (int32_t k = 0; k < nw; ++k) {
+    ED_REQUIRE(std::isfinite(wn[k]) && wn[k] != 0.0, "frequencies must be finite and non-zero");
+    wmin = k == 0 ? fabs(wn[k]) : std::min(wmin, fabs(wn[k]));
+  }
+  if (nw > 0) ED_HIP(hipMemcpyAsync(b->d_wn, wn, sizeof(double) * nw, hipMemcpyHostToDevice, s));
+  if (!b->tuned) {
+    b->cfg.g = b->d_g;  // the tuner's configure() calls set the rest of cfg
+    b->cfg.wn = b->d_wn;
+    b->cfg.nw = nw;
+    b->cfg.wmin = wmin;
+    ED_TRY(autotune(b, s));
+  }
+  if (b->cfg.nw != nw || b->cfg.g != b->d_g || b->cfg.wn != b->d_wn || b->cfg.wmin != wmin) {
+    b->cfg.g = b->d_g;
+    b->cfg.wn = b->d_wn;
+    b->cfg.nw = nw;
+    if (b->graph) {
+      (void)hipGraphExecDestroy(b->graph);
+      b->graph = nullptr;
+    }
+  }
+  b->cfg.g = b->d_g;
+  b->cfg.wn = b->d_wn;
+  b->cfg.nw = nw;
+  b->cfg.wmin = wmin;
+  // capture (reset counters, solve) once per configuration; replay afterwards
+  if (!b->graph || b->graph_stream != s) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    hipStream_t cs;
+    ED_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t graph;
+    int rc = ED_OK;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipStreamDestroy(cs);
+      return set_error(ED_ERR_HIP, "hipStreamBeginCapture failed");
+    }
+    rc = launch(b, cs);
+    const hipError_t ec = hipStreamEndCapture(cs, &graph);
+    (void)hipStreamDestroy(cs);
+    ED_TRY(rc);
+    if (ec != hipSuccess) return set_error(ED_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      b->graph = nullptr;
+      return set_error(ED_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+    }
+    b->graph_stream = s;
+  }
+  hipEvent_t ev0, ev1;
+  ED_HIP(hipEventCreate(&ev0));
+  ED_HIP(hipEventCreate(&ev1));
+  (void)hipEventRecord(ev0, s);
+  const hipError_t el = hipGraphLaunch(b->graph, s);
+  (void)hipEventRecord(ev1, s);
+  if (el != hipSuccess) {
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    return set_error(ED_ERR_HIP, "hipGraphLaunch failed: %s", hipGetErrorString(el));
+  }
+  const hipError_t es = hipEventSynchronize(ev1);
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, ev0, ev1);
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  if (es != hipSuccess) return set_error(ED_ERR_HIP, "batch solve failed: %s", hipGetErrorString(es));
+  if (elapsed_ms) *elapsed_ms = ms;
+  const int64_t M = b->num_models;
+  if (e0) ED_HIP(hipMemcpyAsync(e0, b->d_e0, sizeof(double) * M, hipMemcpyDeviceToHost, s));
+  if (g && nw > 0) ED_HIP(hipMemcpyAsync(g, b->d_g, sizeof(double) * 2 * M * nw, hipMemcpyDeviceToHost, s));
+  if (iters) ED_HIP(hipMemcpyAsync(iters, b->d_iters, sizeof(int32_t) * M, hipMemcpyDeviceToHost, s));
+  if (resid) ED_HIP(hipMemcpyAsync(resid, b->d_resid, sizeof(double) * 2 * M, hipMemcpyDeviceToHost, s));
+  ED_HIP(hipStreamSynchronize(s));
+  return ED_OK;
+}
+
+extern "C" int ed_batch_info(const ed_batch* b, int64_t* wave_max_dim, int32_t* grid_blocks, int32_t* wave_blocks,
+                             double* tune_ms) {
+  clear_error();
+  ED_REQUIRE(b, "NULL batch");
+  if (wave_max_dim) *wave_max_dim = b->wave_max_dim;
+  if (grid_blocks) *grid_blocks = b->grid;
+  if (wave_blocks) *wave_blocks = b->wave_blocks;
+  if (tune_ms) *tune_ms = b->tune_ms;
+  return ED_OK;
+}

@@ -242,6 +242,64 @@ int ed_occupation(const double* vecs_dev, int64_t nvec, int num_sites, int n_up,
  * after vector: a[k*n + i] = component i of eigenvector k.                */
 int ed_dense_eigh(double* a_dev, int64_t n, double* w_dev, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Batched Anderson impurity solver.  No reference counterpart: cmpy solves one
+ * SingleImpurityAndersonModel per call (cmpy/models/anderson.py:163-164).
+ * One persistent launch (replayed from a captured HIP graph) solves every
+ * model of the batch: ground state of its (n_up, n_dn) sector by Lanczos to
+ * Ritz residual tol, then the impurity Green's function at Matsubara
+ * frequencies, G(i w_n) = particle + hole continued fraction from
+ * c+_{0,up} psi0 and c_{0,up} psi0 (Lanczos until G at the lowest frequency
+ * has converged to cf_tol, at most cf_steps steps each).  Each model
+ * is built as SingleImpurityAndersonModel does (anderson.py:147-158): site 0
+ * the impurity with eps_imp - mu and U, bath sites 1..nb with eps_bath (not
+ * shifted), bonds (0, k) with hopping v_k, hopping-sign width sign_width
+ * (0 = the reference's all-positive signs).                                 */
+/* ------------------------------------------------------------------------ */
+#define ED_BATCH_MAX_BATH 15
+
+typedef struct ed_batch_desc {
+  int64_t num_models;
+  const int32_t* num_bath;  /* host [num_models], 1..ED_BATCH_MAX_BATH          */
+  const int32_t* n_up;      /* host [num_models]                                */
+  const int32_t* n_dn;      /* host [num_models]                                */
+  const double* u;          /* host [num_models]                                */
+  const double* eps_imp;    /* host [num_models], impurity level minus mu       */
+  const double* eps_bath;   /* host [num_models][max_bath] (first num_bath used) */
+  const double* v;          /* host [num_models][max_bath]                      */
+  int32_t max_bath;         /* row stride of eps_bath and v                     */
+  int32_t sign_width;       /* 0: reference signs; >= num_sites: fermionic      */
+  int32_t max_iter;         /* ground-state Lanczos steps cap (1..2000)         */
+  int32_t cf_steps;         /* continued-fraction depth cap (1..2000)           */
+  int32_t check_every;      /* steps between ground-state convergence tests     */
+  int32_t pad;
+  double tol;               /* ground-state Ritz residual tolerance              */
+  uint64_t seed;            /* Lanczos start-vector seed                        */
+  double cf_tol;            /* continued-fraction stop: relative change of G at
+                               the lowest frequency over two 8-step checks      */
+} ed_batch_desc;
+
+typedef struct ed_batch ed_batch;
+
+int ed_batch_create(ed_batch** b, const ed_batch_desc* desc, void* stream);
+int ed_batch_destroy(ed_batch* b);
+
+/* Launch shape: models whose sector has at most wave_max_dim states run one
+ * per wave, in workgroups [0, wave_blocks); the rest run one per workgroup.
+ * Without this call the first ed_batch_solve autotunes the shape (it times
+ * candidate splits and grid sizes on the batch itself).                    */
+int ed_batch_config(ed_batch* b, int64_t wave_max_dim, int32_t grid_blocks, int32_t wave_blocks, void* stream);
+int ed_batch_info(const ed_batch* b, int64_t* wave_max_dim, int32_t* grid_blocks, int32_t* wave_blocks,
+                  double* tune_ms);
+
+/* Solve every model.  wn: host [nw] Matsubara frequencies w_n (z = i w_n).
+ * Host outputs (any may be NULL): e0[num_models]; g[num_models][nw] complex
+ * interleaved; iters[num_models] ground-state Lanczos steps;
+ * resid[num_models][2] = (Ritz residual, explicit |H psi - e0 psi|);
+ * *elapsed_ms = device time of the launch.                                 */
+int ed_batch_solve(ed_batch* b, const double* wn, int32_t nw, double* e0, double* g, int32_t* iters,
+                   double* resid, double* elapsed_ms, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
