@@ -82,7 +82,7 @@ class BatchDesc(C.Structure):
         ("u", C.POINTER(C.c_double)), ("eps_imp", C.POINTER(C.c_double)),
         ("eps_bath", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)),
         ("max_bath", C.c_int32), ("sign_width", C.c_int32), ("max_iter", C.c_int32),
-        ("cf_steps", C.c_int32), ("check_every", C.c_int32), ("pad", C.c_int32),
+        ("cf_steps", C.c_int32), ("check_every", C.c_int32), ("keep_basis", C.c_int32),
         ("tol", C.c_double), ("seed", C.c_uint64), ("cf_tol", C.c_double),
     ]
 
@@ -129,6 +129,8 @@ SIGNATURES = {
     "ed_batch_config": (C.c_int, [vp, i64, i32, i32, vp]),
     "ed_batch_info": (C.c_int, [vp, P(i64), P(i32), P(i32), P(dbl)]),
     "ed_batch_solve": (C.c_int, [vp, P(dbl), i32, P(dbl), P(dbl), P(i32), P(dbl), P(dbl), vp]),
+    "ed_batch_backward": (C.c_int, [vp, P(dbl), P(dbl), P(dbl), P(dbl), P(i32), P(dbl), vp]),
+    "ed_batch_set_bath": (C.c_int, [vp, P(dbl), P(dbl), vp]),
 }
 
 _lib = None

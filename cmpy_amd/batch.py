@@ -41,6 +41,7 @@ class BatchResult:
     ritz_residual: np.ndarray  # [M]
     residual: np.ndarray      # [M] explicit |H psi - E0 psi|
     elapsed_ms: float         # device time of the launch
+    cf_steps: Optional[np.ndarray] = None  # [M, 2] particle / hole continued-fraction steps
 
 
 def _ptr(a, t):
@@ -59,7 +60,7 @@ class AndersonBatch:
 
     def __init__(self, u, eps_imp, eps_bath: Sequence, v: Sequence, mu=None, n_up=None, n_dn=None,
                  sign_mode: str = "reference", max_iter: int = 300, cf_steps: int = 512, tol: float = 1e-10,
-                 check_every: int = 8, seed: int = 0x5EED, cf_tol: float = 1e-13):
+                 check_every: int = 8, seed: int = 0x5EED, cf_tol: float = 1e-13, keep_basis: bool = False):
         _lib.require_gpu()
         if sign_mode not in ("reference", "fermion"):
             raise ValueError("sign_mode must be 'reference' or 'fermion'")
@@ -100,7 +101,9 @@ class AndersonBatch:
                            _ptr(self._keep[2], C.c_int32), _ptr(self._keep[3], C.c_double),
                            _ptr(self._keep[4], C.c_double), _ptr(self._keep[5], C.c_double),
                            _ptr(self._keep[6], C.c_double), mb, width, int(max_iter), int(cf_steps),
-                           int(check_every), 0, float(tol), int(seed) & 0xFFFFFFFFFFFFFFFF, float(cf_tol))
+                           int(check_every), int(bool(keep_basis)), float(tol), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                           float(cf_tol))
+        self.keep_basis = bool(keep_basis)
         h = C.c_void_p()
         _lib.call("ed_batch_create", C.byref(h), C.byref(d), _lib.stream_ptr())
         self._h = h
@@ -139,13 +142,44 @@ class AndersonBatch:
         M, nw = self.num_models, len(wn)
         e0 = np.empty(M)
         g = np.empty((M, nw, 2))
-        it = np.empty(M, dtype=np.int32)
+        it = np.empty((M, 3), dtype=np.int32)
         res = np.empty((M, 2))
         ms = C.c_double()
         _lib.call("ed_batch_solve", self._h, _ptr(wn, C.c_double), nw, _ptr(e0, C.c_double),
                   _ptr(g, C.c_double), _ptr(it, C.c_int32), _ptr(res, C.c_double), C.byref(ms),
                   _lib.stream_ptr())
-        return BatchResult(e0, g[..., 0] + 1j * g[..., 1], it, res[:, 0], res[:, 1], ms.value)
+        return BatchResult(e0, g[..., 0] + 1j * g[..., 1], it[:, 0].copy(), res[:, 0], res[:, 1], ms.value,
+                           it[:, 1:].copy())
+
+    def set_bath(self, eps_bath, v) -> None:
+        """Replace the bath parameters ([M, num_bath] arrays or per-model lists)."""
+        eb = np.zeros_like(self.eps_bath)
+        vv = np.zeros_like(self.v)
+        for i in range(self.num_models):
+            nb = self.num_bath[i]
+            eb[i, :nb] = np.asarray(eps_bath[i], dtype=np.float64)[:nb]
+            vv[i, :nb] = np.asarray(v[i], dtype=np.float64)[:nb]
+        self.eps_bath, self.v = eb, vv
+        _lib.call("ed_batch_set_bath", self._h, _ptr(eb, C.c_double), _ptr(vv, C.c_double), _lib.stream_ptr())
+
+    def backward(self, dl_dre, dl_dim):
+        """Adjoint backward of the last solve: (dL/d eps_bath, dL/d v, cg_iters,
+        elapsed_ms) for a real loss with dL/dRe G, dL/dIm G of shape [M, nw]."""
+        if not self.keep_basis:
+            raise ValueError("create the batch with keep_basis=True to differentiate it")
+        gr = np.ascontiguousarray(dl_dre, dtype=np.float64)
+        gi = np.ascontiguousarray(dl_dim, dtype=np.float64)
+        M, mb = self.eps_bath.shape
+        ge = np.zeros((M, mb))
+        gv = np.zeros((M, mb))
+        cg = np.empty(M, dtype=np.int32)
+        ms = C.c_double()
+        _lib.call("ed_batch_backward", self._h, _ptr(gr, C.c_double), _ptr(gi, C.c_double), _ptr(ge, C.c_double),
+                  _ptr(gv, C.c_double), _ptr(cg, C.c_int32), C.byref(ms), _lib.stream_ptr())
+        for i in range(M):  # entries past a model's bath are not written
+            ge[i, self.num_bath[i]:] = 0.0
+            gv[i, self.num_bath[i]:] = 0.0
+        return ge, gv, cg, ms.value
 
 
 def random_batch(num_models: int, seed: int, nb_range=(5, 9), u_range=(1.0, 8.0), eps_range=(-3.0, 3.0),
@@ -158,3 +192,35 @@ def random_batch(num_models: int, seed: int, nb_range=(5, 9), u_range=(1.0, 8.0)
     eps_bath = [rng.uniform(*eps_range, size=n) for n in nb]
     v = [rng.uniform(*v_range, size=n) for n in nb]
     return dict(u=u, eps_imp=-u / 2, eps_bath=eps_bath, v=v, mu=np.zeros(num_models))
+
+
+class _AndersonG(__import__("torch").autograd.Function):
+    """G(i w_n) of a batch as a differentiable function of (eps_bath, v)."""
+
+    @staticmethod
+    def forward(ctx, eps_bath, v, batch, wn):
+        batch.set_bath(eps_bath.detach().cpu().numpy(), v.detach().cpu().numpy())
+        res = batch.solve(wn)
+        ctx.batch = batch
+        import torch
+
+        return torch.from_numpy(res.g).to(eps_bath.device)
+
+    @staticmethod
+    def backward(ctx, grad_g):
+        # for a real loss, torch passes dL/dRe G + i dL/dIm G
+        import torch
+
+        gr = grad_g.real.detach().cpu().numpy()
+        gi = grad_g.imag.detach().cpu().numpy()
+        ge, gv, _, _ = ctx.batch.backward(gr, gi)
+        dev = grad_g.device
+        return torch.from_numpy(ge).to(dev), torch.from_numpy(gv).to(dev), None, None
+
+
+def anderson_gf(eps_bath, v, batch: AndersonBatch, wn):
+    """Differentiable G(i w_n) [M, nw] (complex128) of ``batch`` at bath
+    parameters ``eps_bath``, ``v`` (torch float64 [M, max_bath]).  The batch
+    must be created with ``keep_basis=True``; one forward/backward pair at a
+    time per batch (the backward uses the batch's last solve)."""
+    return _AndersonG.apply(eps_bath, v, batch, np.ascontiguousarray(wn, dtype=np.float64))

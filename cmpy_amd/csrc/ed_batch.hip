@@ -67,8 +67,17 @@ struct Config {
   // outputs
   double* e0;     // [num_models]
   double* g;      // [num_models][nw][2]
-  int32_t* iters; // [num_models]
+  int32_t* iters; // [num_models][3]: ground-state, particle and hole Lanczos steps
   double* resid;  // [num_models][2]: Ritz residual, explicit residual
+  // kept for the adjoint backward (keep != 0): per model psi [D], the
+  // continued-fraction Krylov vectors state-major Qt[D_part][kcap] (particle,
+  // hole) and coefficients [a+ | b+ | a- | b- (kcap each) | e0, n2+, n2-, m+, m-]
+  int32_t keep, kcap;
+  const int64_t* off_psi;  // [num_models]
+  const int64_t* off_q;    // [num_models][2]
+  double* store_psi;
+  double* store_q;
+  double* store_coef;      // [num_models][4 * kcap + 8]
 };
 
 // ---------------------------------------------------------------------------
@@ -124,7 +133,7 @@ struct SpinList {
   const uint32_t* st;
   const double* dg;
   const int32_t* cnt;
-  const int32_t* tgt;  // [n][kMaxBath] hop targets
+  const int32_t* tgt;  // [n][kMaxBath] hop target | bond << 16 | sign << 31
   const double* val;   // [n][kMaxBath] hop values sign * V_k
   int32_t n;
 };
@@ -200,8 +209,9 @@ __device__ int build_list(const Group<BLOCK>& g, const ModelParams& p, int ns, i
       const uint32_t m = 1u | (2u << k);
       const uint32_t b = s & m;
       if (b == 0 || b == m) continue;
-      tgt[i * kMaxBath + c] = (int32_t)comb_rank(s ^ m);
-      val[i * kMaxBath + c] = hop_sign(s, width, 0, k + 1) < 0 ? -p.v[k] : p.v[k];
+      const bool neg = hop_sign(s, width, 0, k + 1) < 0;
+      tgt[i * kMaxBath + c] = (int32_t)((uint32_t)comb_rank(s ^ m) | ((uint32_t)k << 16) | (neg ? 0x80000000u : 0u));
+      val[i * kMaxBath + c] = neg ? -p.v[k] : p.v[k];
       ++c;
     }
     cnt[i] = c;
@@ -229,9 +239,10 @@ __device__ void lanczos_step(const Group<BLOCK>& g, const ModelParams& p, const 
       double acc = (U.dg[iu] + Dn.dg[id] + ((U.st[iu] & Dn.st[id] & 1u) ? p.u : 0.0)) * xk;
       const int cu = U.cnt[iu];
       for (int e = 0; e < cu; ++e)
-        acc += U.val[iu * kMaxBath + e] * u[(int64_t)U.tgt[iu * kMaxBath + e] * nd + id];
+        acc += U.val[iu * kMaxBath + e] * u[(int64_t)(U.tgt[iu * kMaxBath + e] & 0xffff) * nd + id];
       const int cd = Dn.cnt[id];
-      for (int e = 0; e < cd; ++e) acc += Dn.val[id * kMaxBath + e] * u[(int64_t)iu * nd + Dn.tgt[id * kMaxBath + e]];
+      for (int e = 0; e < cd; ++e)
+        acc += Dn.val[id * kMaxBath + e] * u[(int64_t)iu * nd + (Dn.tgt[id * kMaxBath + e] & 0xffff)];
       double wk = sj * acc;
       if (bj != 0.0) wk -= bj * sp * uprev[k];
       w[k] = wk;
@@ -363,7 +374,7 @@ __device__ cplx cf_value(const double* a, const double* b, int m, double norm2, 
 template <bool BLOCK>
 __device__ int cf_coeffs(const Group<BLOCK>& g, const ModelParams& p, const SpinList& U, const SpinList& Dn,
                          const SlotRef& sl, double n2, int cap, double e0, double sgn, double wmin, double cf_tol,
-                         double* a, double* b) {
+                         double* a, double* b, double* qt, int kcap) {
   double* u = sl.vec(1);
   double* up = sl.vec(0);
   double* w = sl.vec(2);
@@ -371,8 +382,11 @@ __device__ int cf_coeffs(const Group<BLOCK>& g, const ModelParams& p, const Spin
   double sj = 1.0 / sqrt(n2), sp = 0.0, bj = 0.0;
   int m = 0, calm = 0;
   cplx prev = {0.0, 0.0};
+  const int64_t D = (int64_t)U.n * Dn.n;
   for (int j = 0; j < cap; ++j) {
     double aj, nn;
+    if (qt)  // keep the normalised Krylov vector v_j = sj u, state-major
+      for (int64_t k = g.rank; k < D; k += g.size) qt[k * kcap + j] = sj * u[k];
     lanczos_step<BLOCK>(g, p, U, Dn, u, up, w, sj, sp, bj, false, 0.0, &aj, &nn);
     a[j] = aj;
     m = j + 1;
@@ -549,10 +563,14 @@ __device__ void solve_model(const Group<BLOCK>& g, const Config& cf, int mi, con
     if (!(nn > 0.0)) continue;
     if (part == 0) {
       np2 = nn;
-      mp = cf_coeffs<BLOCK>(g, p, T, Dn, sl, nn, cf.cf_steps, theta, 1.0, cf.wmin, cf.cf_tol, sl.cfc(0), sl.cfc(1));
+      double* qt = cf.keep ? cf.store_q + cf.off_q[2 * mi] : nullptr;
+      mp = cf_coeffs<BLOCK>(g, p, T, Dn, sl, nn, cf.cf_steps, theta, 1.0, cf.wmin, cf.cf_tol, sl.cfc(0), sl.cfc(1), qt,
+                            cf.kcap);
     } else {
       nm2 = nn;
-      mm = cf_coeffs<BLOCK>(g, p, T, Dn, sl, nn, cf.cf_steps, theta, -1.0, cf.wmin, cf.cf_tol, sl.cfc(2), sl.cfc(3));
+      double* qt = cf.keep ? cf.store_q + cf.off_q[2 * mi + 1] : nullptr;
+      mm = cf_coeffs<BLOCK>(g, p, T, Dn, sl, nn, cf.cf_steps, theta, -1.0, cf.wmin, cf.cf_tol, sl.cfc(2), sl.cfc(3),
+                            qt, cf.kcap);
     }
   }
   // coefficient arrays were written by every member with the same values
@@ -573,9 +591,30 @@ __device__ void solve_model(const Group<BLOCK>& g, const Config& cf, int mi, con
     cf.g[((int64_t)mi * cf.nw + n) * 2] = gsum.re;
     cf.g[((int64_t)mi * cf.nw + n) * 2 + 1] = gsum.im;
   }
+  if (cf.keep) {
+    double* ps = cf.store_psi + cf.off_psi[mi];
+    for (int64_t k = g.rank; k < D; k += g.size) ps[k] = psi[k];
+    double* co = cf.store_coef + (int64_t)mi * (4 * cf.kcap + 8);
+    for (int j = g.rank; j < cf.kcap; j += g.size) {
+      co[j] = j < mp ? sl.cfc(0)[j] : 0.0;
+      co[cf.kcap + j] = j < mp - 1 ? sl.cfc(1)[j] : 0.0;
+      co[2 * cf.kcap + j] = j < mm ? sl.cfc(2)[j] : 0.0;
+      co[3 * cf.kcap + j] = j < mm - 1 ? sl.cfc(3)[j] : 0.0;
+    }
+    if (g.rank == 0) {
+      double* sc = co + 4 * cf.kcap;
+      sc[0] = theta;
+      sc[1] = np2;
+      sc[2] = nm2;
+      sc[3] = mp;
+      sc[4] = mm;
+    }
+  }
   if (g.rank == 0) {
     cf.e0[mi] = theta;
-    cf.iters[mi] = m;
+    cf.iters[3 * mi] = m;
+    cf.iters[3 * mi + 1] = mp;
+    cf.iters[3 * mi + 2] = mm;
     cf.resid[2 * mi] = ritz;
     cf.resid[2 * mi + 1] = sqrt(res2);
   }
@@ -615,6 +654,344 @@ __global__ __launch_bounds__(kBlock) void k_batch_solve(Config cf) {
   }
 }
 
+// ===========================================================================
+// Adjoint backward of the batch (bath fit).  For L = sum_n |G_n - T_n|^2 the
+// caller passes dL/dRe G and dL/dIm G; with w_n = dL/dRe G_n - i dL/dIm G_n,
+//   dL/dtheta = S+ - S- + (B - A) dE0/dtheta - 2 psi^T (dH/dtheta) zeta,
+// where, for the particle (+) and hole (-) parts with Krylov bases Q and
+// resolvent vectors x_n = Q u_n (u_n = sigma |phi| (sigma z_n + E0 - T)^-1 e1):
+//   K = Q Re(W) Q^T,  W = sum_n w_n u_n u_n^T,
+//   S = tr(dH/dtheta K) over the part's sector,  A (B) = tr K of the particle (hole) part,
+//   dE0/dtheta = psi^T (dH/dtheta) psi  (Hellmann-Feynman),
+//   zeta: (H - E0) zeta = P r, P = 1 - psi psi^T, r = Re(c X + c+ Y),
+//   X = sum_n w_n x_n (particle), Y = sum_n w_n y_n (hole).
+// theta = (eps_bath, v): dH/d eps_k = n_k,up + n_k,dn and dH/d v_k = the bond
+// (0, k) hopping with unit amplitude (and the model's hopping signs).
+// One workgroup per model; no orthogonality of Q is assumed (everything is
+// evaluated through K in the sector's state space).
+// ===========================================================================
+struct BwdConfig {
+  const ModelParams* params;
+  int32_t num_models;
+  int32_t* counter;
+  char* ws;
+  int64_t slot_bytes;
+  int32_t lcap;
+  int64_t vcap;
+  int32_t kcap, nw, sign_width, max_bath;
+  const double* wn;
+  const double* gre;  // [num_models][nw] dL/dRe G
+  const double* gim;  // [num_models][nw] dL/dIm G
+  const int64_t* off_psi;
+  const int64_t* off_q;
+  const double* store_psi;
+  const double* store_q;
+  const double* store_coef;
+  int32_t cg_max;
+  double cg_tol;
+  double* grad_eps;   // [num_models][max_bath]
+  double* grad_v;     // [num_models][max_bath]
+  int32_t* cg_iters;  // [num_models]
+};
+
+// Backward slot: the four spin lists (same layout as SlotRef), then
+// N-sector vectors zeta, res, pvec, q; X re/im, Y re/im; diag d; U, U2
+// ([nw][kcap] complex each); ReW [kcap][kcap]; ubar (complex [kcap]);
+// P [vcap][kcap]; scalars.
+struct BwdSlot {
+  SlotRef lists;  // only the list accessors are used
+  __host__ __device__ int64_t vb() const { return align_up(8 * lists.vcap, 256); }
+  __host__ __device__ int64_t ub(int nw) const { return align_up(16ll * nw * lists.csteps, 256); }
+  __host__ __device__ int64_t wb() const { return align_up(8ll * lists.csteps * lists.csteps, 256); }
+  __host__ __device__ int64_t pb() const { return align_up(8ll * lists.vcap * lists.csteps, 256); }
+  __host__ __device__ int64_t bytes(int nw) const {
+    return 4 * lists.list_bytes() + 9 * vb() + 2 * ub(nw) + wb() + align_up(16ll * lists.csteps, 256) + pb() +
+           align_up(8 * 64, 256);
+  }
+  __device__ char* after_lists() const { return lists.base + 4 * lists.list_bytes(); }
+  // 0 zeta, 1 res, 2 pvec, 3 q, 4 Xre, 5 Xim, 6 Yre, 7 Yim, 8 diag
+  __device__ double* vec(int i) const { return (double*)(after_lists() + i * vb()); }
+  __device__ double* U(int which, int nw) const { return (double*)(after_lists() + 9 * vb() + which * ub(nw)); }
+  __device__ double* ReW(int nw) const { return (double*)(after_lists() + 9 * vb() + 2 * ub(nw)); }
+  __device__ double* ubar(int nw) const { return (double*)((char*)ReW(nw) + wb()); }
+  __device__ double* P(int nw) const { return (double*)((char*)ubar(nw) + align_up(16ll * lists.csteps, 256)); }
+  __device__ double* scal(int nw) const { return (double*)((char*)P(nw) + pb()); }
+};
+
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+
+// Occupation of bath site k (bond k couples sites 0 and k + 1) in a state pair.
+__device__ __forceinline__ double occ_k(uint32_t su, uint32_t sd, int k) {
+  return (double)(((su >> (k + 1)) & 1u) + ((sd >> (k + 1)) & 1u));
+}
+
+// sum over directed hops a -> t on bond k of the sector (U, Dn): sign * f(a, t)
+template <typename F>
+__device__ __forceinline__ double bond_hops(const SpinList& U, const SpinList& Dn, int64_t a, int k, F f) {
+  const int nd = Dn.n;
+  const int iu = (int)(a / nd), id = (int)(a - (int64_t)iu * nd);
+  double s = 0.0;
+  for (int e = 0; e < U.cnt[iu]; ++e) {
+    const uint32_t t = (uint32_t)U.tgt[iu * kMaxBath + e];
+    if ((int)((t >> 16) & 0xff) != k) continue;
+    const double sg = (t & 0x80000000u) ? -1.0 : 1.0;
+    s += sg * f(a, (int64_t)(t & 0xffff) * nd + id);
+  }
+  for (int e = 0; e < Dn.cnt[id]; ++e) {
+    const uint32_t t = (uint32_t)Dn.tgt[id * kMaxBath + e];
+    if ((int)((t >> 16) & 0xff) != k) continue;
+    const double sg = (t & 0x80000000u) ? -1.0 : 1.0;
+    s += sg * f(a, (int64_t)iu * nd + (t & 0xffff));
+  }
+  return s;
+}
+
+__device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi, const BwdSlot& bs) {
+  const ModelParams& p = bc.params[mi];
+  const int ns = p.nb + 1;
+  const int nb = p.nb;
+  const SlotRef& sl = bs.lists;
+  const int n_m = build_list<true>(g, p, ns, p.n_up - 1, bc.sign_width, sl, 0);
+  const int n_0 = build_list<true>(g, p, ns, p.n_up, bc.sign_width, sl, 1);
+  const int n_p = build_list<true>(g, p, ns, p.n_up + 1, bc.sign_width, sl, 2);
+  const int nd = build_list<true>(g, p, ns, p.n_dn, bc.sign_width, sl, 3);
+  g.sync();
+  const SpinList U0 = sl.list(1, n_0);
+  const SpinList Dn = sl.list(3, nd);
+  const int64_t D = (int64_t)n_0 * nd;
+  const int kcap = bc.kcap, nw = bc.nw;
+  const double* co = bc.store_coef + (int64_t)mi * (4 * kcap + 8);
+  const double e0 = co[4 * kcap];
+  const double* psi = bc.store_psi + bc.off_psi[mi];
+  double* scal = bs.scal(nw);
+  // scal layout: [0..15] misc, [16 .. 16+nb) S_eps, [32 .. 32+nb) S_v
+  if (g.rank < 64) scal[g.rank] = 0.0;
+  g.sync();
+  double trK0 = 0.0, trK1 = 0.0;  // trace of K of the particle and hole parts
+  for (int part = 0; part < 2; ++part) {
+    const int m = (int)co[4 * kcap + 3 + part];
+    if (m <= 0) continue;
+    const double sigma = part == 0 ? 1.0 : -1.0;
+    const double* a = co + (part == 0 ? 0 : 2 * kcap);
+    const double* b = co + (part == 0 ? kcap : 3 * kcap);
+    const double nphi = sqrt(co[4 * kcap + 1 + part]);
+    const SpinList T = sl.list(part == 0 ? 2 : 0, part == 0 ? n_p : n_m);
+    const int64_t Dt = (int64_t)T.n * nd;
+    const double* qt = bc.store_q + bc.off_q[2 * mi + part];
+    double* Uc = bs.U(0, nw);   // c' then x
+    double* Ur = bs.U(1, nw);   // rhs'
+    // (a) u_n = sigma |phi| (sigma z_n + e0 - T)^-1 e1, complex Thomas, one thread per frequency
+    for (int n = g.rank; n < nw; n += g.size) {
+      const cplx zs = {e0, sigma * bc.wn[n]};
+      cplx cp = {0.0, 0.0}, rp = {0.0, 0.0};
+      for (int i = 0; i < m; ++i) {
+        const double offl = i > 0 ? -b[i - 1] : 0.0;
+        const cplx den = {zs.re - a[i] - offl * cp.re, zs.im - offl * cp.im};
+        const cplx ci = i < m - 1 ? cdiv({-b[i], 0.0}, den) : cplx{0.0, 0.0};
+        const cplx ri = cdiv({(i == 0 ? 1.0 : 0.0) - offl * rp.re, -offl * rp.im}, den);
+        Uc[2 * ((int64_t)n * kcap + i)] = ci.re;
+        Uc[2 * ((int64_t)n * kcap + i) + 1] = ci.im;
+        Ur[2 * ((int64_t)n * kcap + i)] = ri.re;
+        Ur[2 * ((int64_t)n * kcap + i) + 1] = ri.im;
+        cp = ci;
+        rp = ri;
+      }
+      cplx x = rp;
+      Uc[2 * ((int64_t)n * kcap + m - 1)] = sigma * nphi * x.re;
+      Uc[2 * ((int64_t)n * kcap + m - 1) + 1] = sigma * nphi * x.im;
+      for (int i = m - 2; i >= 0; --i) {
+        const cplx ci = {Uc[2 * ((int64_t)n * kcap + i)], Uc[2 * ((int64_t)n * kcap + i) + 1]};
+        const cplx ri = {Ur[2 * ((int64_t)n * kcap + i)], Ur[2 * ((int64_t)n * kcap + i) + 1]};
+        const cplx t = cmul(ci, x);
+        x = {ri.re - t.re, ri.im - t.im};
+        Uc[2 * ((int64_t)n * kcap + i)] = sigma * nphi * x.re;
+        Uc[2 * ((int64_t)n * kcap + i) + 1] = sigma * nphi * x.im;
+      }
+    }
+    g.sync();
+    // (b) Re W and ubar
+    double* W = bs.ReW(nw);
+    double* ub = bs.ubar(nw);
+    const double* gre = bc.gre + (int64_t)mi * nw;
+    const double* gim = bc.gim + (int64_t)mi * nw;
+    for (int t = g.rank; t < m * m; t += g.size) {
+      const int i = t / m, j = t - i * m;
+      if (j < i) continue;
+      double acc = 0.0;
+      for (int n = 0; n < nw; ++n) {
+        const cplx ui = {Uc[2 * ((int64_t)n * kcap + i)], Uc[2 * ((int64_t)n * kcap + i) + 1]};
+        const cplx uj = {Uc[2 * ((int64_t)n * kcap + j)], Uc[2 * ((int64_t)n * kcap + j) + 1]};
+        const cplx q = cmul(ui, uj);
+        acc += gre[n] * q.re + gim[n] * q.im;  // Re((gre - i gim) q)
+      }
+      W[(int64_t)i * kcap + j] = acc;
+      W[(int64_t)j * kcap + i] = acc;
+    }
+    for (int i = g.rank; i < m; i += g.size) {
+      double re = 0.0, im = 0.0;
+      for (int n = 0; n < nw; ++n) {
+        const double ur = Uc[2 * ((int64_t)n * kcap + i)], ui = Uc[2 * ((int64_t)n * kcap + i) + 1];
+        re += gre[n] * ur + gim[n] * ui;
+        im += gre[n] * ui - gim[n] * ur;
+      }
+      ub[2 * i] = re;
+      ub[2 * i + 1] = im;
+    }
+    g.sync();
+    // (c) P = Qt Re W (rows of the sector's states), (e) X = Qt ubar
+    double* P = bs.P(nw);
+    double* Xr = bs.vec(part == 0 ? 4 : 6);
+    double* Xi = bs.vec(part == 0 ? 5 : 7);
+    for (int64_t r = g.rank; r < Dt; r += g.size) {
+      const double* q = qt + r * kcap;
+      double xr = 0.0, xi = 0.0;
+      for (int i = 0; i < m; ++i) {
+        xr += q[i] * ub[2 * i];
+        xi += q[i] * ub[2 * i + 1];
+      }
+      Xr[r] = xr;
+      Xi[r] = xi;
+      for (int j = 0; j < m; ++j) {
+        double acc = 0.0;
+        for (int i = 0; i < m; ++i) acc += q[i] * W[(int64_t)i * kcap + j];
+        P[r * kcap + j] = acc;
+      }
+    }
+    g.sync();
+    // (d) diag of K, its trace, and the per-bond sums
+    double* dg = bs.vec(8);
+    double tr = 0.0;
+    for (int64_t r = g.rank; r < Dt; r += g.size) {
+      double acc = 0.0;
+      for (int j = 0; j < m; ++j) acc += P[r * kcap + j] * qt[r * kcap + j];
+      dg[r] = acc;
+      tr += acc;
+    }
+    tr = g.sum(tr);
+    if (part == 0)
+      trK0 = tr;
+    else
+      trK1 = tr;
+    g.sync();
+    for (int k = 0; k < nb; ++k) {
+      double se = 0.0, sv = 0.0;
+      for (int64_t r = g.rank; r < Dt; r += g.size) {
+        const int iu = (int)(r / nd), id = (int)(r - (int64_t)iu * nd);
+        se += occ_k(T.st[iu], Dn.st[id], k) * dg[r];
+        sv += bond_hops(T, Dn, r, k, [&](int64_t aa, int64_t bb) {
+          double acc = 0.0;
+          for (int j = 0; j < m; ++j) acc += P[aa * kcap + j] * qt[bb * kcap + j];
+          return acc;
+        });
+      }
+      se = g.sum(se);
+      sv = g.sum(sv);
+      if (g.rank == 0) {
+        scal[16 + k] += sigma * se;
+        scal[32 + k] += sigma * sv;
+      }
+      g.sync();
+    }
+  }
+  // (f) r = Re(c X + c+ Y) in the N sector, projected off psi
+  double* zeta = bs.vec(0);
+  double* res = bs.vec(1);
+  double* pv = bs.vec(2);
+  double* qv = bs.vec(3);
+  const double* Xr = bs.vec(4);
+  const double* Yr = bs.vec(6);
+  const bool has_p = co[4 * kcap + 3] > 0, has_m = co[4 * kcap + 4] > 0;
+  double pr = 0.0;
+  for (int64_t k = g.rank; k < D; k += g.size) {
+    const int iu = (int)(k / nd), id = (int)(k - (int64_t)iu * nd);
+    const uint32_t st = U0.st[iu];
+    double r = 0.0;
+    if (!(st & 1u) && has_p) r += Xr[(int64_t)comb_rank(st | 1u) * nd + id];
+    if ((st & 1u) && has_m) r += Yr[(int64_t)comb_rank(st ^ 1u) * nd + id];
+    res[k] = r;
+    pr += r * psi[k];
+  }
+  pr = g.sum(pr);
+  g.sync();
+  double rho = 0.0;
+  for (int64_t k = g.rank; k < D; k += g.size) {
+    const double r = res[k] - pr * psi[k];
+    res[k] = r;
+    pv[k] = r;
+    zeta[k] = 0.0;
+    rho += r * r;
+  }
+  rho = g.sum(rho);
+  g.sync();
+  // (g) projected conjugate gradients on (H - E0) over the complement of psi
+  const double rho0 = rho;
+  int it = 0;
+  for (; it < bc.cg_max && rho > bc.cg_tol * bc.cg_tol * rho0 && rho > 0.0; ++it) {
+    double aj, nn;
+    lanczos_step<true>(g, p, U0, Dn, pv, bs.vec(8), qv, 1.0, 0.0, 0.0, true, e0, &aj, &nn);  // q = (H - E0) p
+    double qp = 0.0, qpsi = 0.0;
+    for (int64_t k = g.rank; k < D; k += g.size) {
+      qp += qv[k] * pv[k];
+      qpsi += qv[k] * psi[k];
+    }
+    qp = g.sum(qp);
+    qpsi = g.sum(qpsi);
+    const double alpha = rho / qp;
+    double rn = 0.0;
+    for (int64_t k = g.rank; k < D; k += g.size) {
+      const double q = qv[k] - qpsi * psi[k];
+      zeta[k] += alpha * pv[k];
+      const double r = res[k] - alpha * q;
+      res[k] = r;
+      rn += r * r;
+    }
+    rn = g.sum(rn);
+    g.sync();
+    const double beta = rn / rho;
+    rho = rn;
+    for (int64_t k = g.rank; k < D; k += g.size) pv[k] = res[k] + beta * pv[k];
+    g.sync();
+  }
+  // (h) dE0/dtheta and psi^T dH zeta, (i) the gradient
+  for (int k = 0; k < nb; ++k) {
+    double de = 0.0, te = 0.0, dv = 0.0, tv = 0.0;
+    for (int64_t r = g.rank; r < D; r += g.size) {
+      const int iu = (int)(r / nd), id = (int)(r - (int64_t)iu * nd);
+      const double o = occ_k(U0.st[iu], Dn.st[id], k);
+      de += o * psi[r] * psi[r];
+      te += o * psi[r] * zeta[r];
+      dv += bond_hops(U0, Dn, r, k, [&](int64_t aa, int64_t bb) { return psi[aa] * psi[bb]; });
+      tv += bond_hops(U0, Dn, r, k, [&](int64_t aa, int64_t bb) { return psi[aa] * zeta[bb]; });
+    }
+    de = g.sum(de);
+    te = g.sum(te);
+    dv = g.sum(dv);
+    tv = g.sum(tv);
+    if (g.rank == 0) {
+      const double ba = trK1 - trK0;
+      bc.grad_eps[(int64_t)mi * bc.max_bath + k] = scal[16 + k] + ba * de - 2.0 * te;
+      bc.grad_v[(int64_t)mi * bc.max_bath + k] = scal[32 + k] + ba * dv - 2.0 * tv;
+    }
+  }
+  if (g.rank == 0) bc.cg_iters[mi] = it;
+  g.sync();
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_backward(BwdConfig bc) {
+  __shared__ double red[32];
+  __shared__ int popped;
+  Group<true> g{(int)threadIdx.x, (int)blockDim.x, red};
+  const BwdSlot bs{SlotRef{bc.ws + (int64_t)blockIdx.x * bc.slot_bytes, bc.lcap, bc.vcap, 0, bc.kcap}};
+  while (true) {
+    if (threadIdx.x == 0) popped = atomicAdd(bc.counter, 1);
+    __syncthreads();
+    const int q = __builtin_amdgcn_readfirstlane(popped);
+    __syncthreads();
+    if (q >= bc.num_models) break;
+    backward_model(g, bc, q, bs);
+  }
+}
+
 __global__ void k_reset_counters(int32_t* c) {
   if (threadIdx.x < 2) c[threadIdx.x] = 0;
 }
@@ -637,6 +1014,7 @@ struct ed_batch {
   std::vector<int32_t> order; // models by decreasing cost
   int32_t max_iter = 300, cf_steps = 64, check_every = 8, sign_width = 0;
   double tol = 1e-10, cf_tol = 1e-13;
+  int32_t max_bath = 1;
   uint64_t seed = 0x5EED;
   // device
   ModelParams* d_params = nullptr;
@@ -645,6 +1023,17 @@ struct ed_batch {
   double *d_e0 = nullptr, *d_g = nullptr, *d_resid = nullptr, *d_wn = nullptr;
   int32_t* d_iters = nullptr;
   char *d_ws = nullptr;
+  // adjoint backward (keep_basis)
+  bool keep = false;
+  int32_t kcap = 0;
+  int64_t *d_off_psi = nullptr, *d_off_q = nullptr;
+  double *d_store_psi = nullptr, *d_store_q = nullptr, *d_store_coef = nullptr;
+  double *d_gre = nullptr, *d_gim = nullptr, *d_grad_eps = nullptr, *d_grad_v = nullptr;
+  int32_t* d_cg = nullptr;
+  int32_t* d_bcounter = nullptr;
+  char* d_bws = nullptr;
+  int64_t bws_bytes = 0;
+  int32_t solved_nw = -1;  // frequencies of the last solve (the backward uses them)
   int64_t ws_bytes = 0;
   int32_t nw_cap = 0;
   // launch configuration
@@ -666,7 +1055,10 @@ static void free_batch(ed_batch* b) {
   if (!b) return;
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   for (void* p : {(void*)b->d_params, (void*)b->d_queues, (void*)b->d_counters, (void*)b->d_e0, (void*)b->d_g,
-                  (void*)b->d_resid, (void*)b->d_wn, (void*)b->d_iters, (void*)b->d_ws})
+                  (void*)b->d_resid, (void*)b->d_wn, (void*)b->d_iters, (void*)b->d_ws, (void*)b->d_off_psi,
+                  (void*)b->d_off_q, (void*)b->d_store_psi, (void*)b->d_store_q, (void*)b->d_store_coef,
+                  (void*)b->d_gre, (void*)b->d_gim, (void*)b->d_grad_eps, (void*)b->d_grad_v, (void*)b->d_cg,
+                  (void*)b->d_bcounter, (void*)b->d_bws})
     if (p) (void)hipFree(p);
   delete b;
 }
@@ -754,6 +1146,13 @@ static int configure(ed_batch* b, int64_t wave_max_dim, int grid_blocks, int wav
   c.g = b->d_g;
   c.iters = b->d_iters;
   c.resid = b->d_resid;
+  c.keep = b->keep ? 1 : 0;
+  c.kcap = b->kcap;
+  c.off_psi = b->d_off_psi;
+  c.off_q = b->d_off_q;
+  c.store_psi = b->d_store_psi;
+  c.store_q = b->d_store_q;
+  c.store_coef = b->d_store_coef;
   b->wave_max_dim = wave_max_dim;
   b->grid = grid_blocks;
   b->wave_blocks = wave_blocks;
@@ -908,7 +1307,7 @@ extern "C" int ed_batch_create(ed_batch** out, const ed_batch_desc* d, void* str
   int rc = ED_OK;
   if ((rc = balloc(&b->d_params, d->num_models)) != ED_OK || (rc = balloc(&b->d_queues, d->num_models)) != ED_OK ||
       (rc = balloc(&b->d_counters, 2)) != ED_OK || (rc = balloc(&b->d_e0, d->num_models)) != ED_OK ||
-      (rc = balloc(&b->d_iters, d->num_models)) != ED_OK || (rc = balloc(&b->d_resid, 2 * d->num_models)) != ED_OK) {
+      (rc = balloc(&b->d_iters, 3 * d->num_models)) != ED_OK || (rc = balloc(&b->d_resid, 2 * d->num_models)) != ED_OK) {
     free_batch(b);
     return rc;
   }
@@ -918,6 +1317,39 @@ extern "C" int ed_batch_create(ed_batch** out, const ed_batch_desc* d, void* str
     free_batch(b);
     return set_error(ED_ERR_HIP, "upload of model parameters failed");
   }
+  if (d->keep_basis) {
+    // per-model stores for the adjoint backward
+    b->keep = true;
+    b->kcap = d->cf_steps;
+    const int64_t M = d->num_models;
+    std::vector<int64_t> off_psi(M), off_q(2 * M);
+    int64_t np = 0, nq = 0;
+    for (int64_t i = 0; i < M; ++i) {
+      const ModelParams& p = b->params[i];
+      const int ns = p.nb + 1;
+      off_psi[i] = np;
+      np += b->dim[i];
+      off_q[2 * i] = nq;
+      if (p.n_up + 1 <= ns) nq += sector_dim(ns, p.n_up + 1, p.n_dn) * b->kcap;
+      off_q[2 * i + 1] = nq;
+      if (p.n_up >= 1) nq += sector_dim(ns, p.n_up - 1, p.n_dn) * b->kcap;
+    }
+    if ((rc = balloc(&b->d_off_psi, M)) != ED_OK || (rc = balloc(&b->d_off_q, 2 * M)) != ED_OK ||
+        (rc = balloc(&b->d_store_psi, np)) != ED_OK || (rc = balloc(&b->d_store_q, nq)) != ED_OK ||
+        (rc = balloc(&b->d_store_coef, M * (4 * (int64_t)b->kcap + 8))) != ED_OK ||
+        (rc = balloc(&b->d_grad_eps, M * d->max_bath)) != ED_OK || (rc = balloc(&b->d_grad_v, M * d->max_bath)) != ED_OK ||
+        (rc = balloc(&b->d_cg, M)) != ED_OK || (rc = balloc(&b->d_bcounter, 1)) != ED_OK) {
+      free_batch(b);
+      return rc;
+    }
+    if (hipMemcpyAsync(b->d_off_psi, off_psi.data(), sizeof(int64_t) * M, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(b->d_off_q, off_q.data(), sizeof(int64_t) * 2 * M, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      free_batch(b);
+      return set_error(ED_ERR_HIP, "upload of store offsets failed");
+    }
+  }
+  b->max_bath = d->max_bath;
   *out = b;
   return ED_OK;
 }
@@ -1027,10 +1459,11 @@ extern "C" int ed_batch_solve(ed_batch* b, const double* wn, int32_t nw, double*
   (void)hipEventDestroy(ev1);
   if (es != hipSuccess) return set_error(ED_ERR_HIP, "batch solve failed: %s", hipGetErrorString(es));
   if (elapsed_ms) *elapsed_ms = ms;
+  b->solved_nw = nw;
   const int64_t M = b->num_models;
   if (e0) ED_HIP(hipMemcpyAsync(e0, b->d_e0, sizeof(double) * M, hipMemcpyDeviceToHost, s));
   if (g && nw > 0) ED_HIP(hipMemcpyAsync(g, b->d_g, sizeof(double) * 2 * M * nw, hipMemcpyDeviceToHost, s));
-  if (iters) ED_HIP(hipMemcpyAsync(iters, b->d_iters, sizeof(int32_t) * M, hipMemcpyDeviceToHost, s));
+  if (iters) ED_HIP(hipMemcpyAsync(iters, b->d_iters, sizeof(int32_t) * 3 * M, hipMemcpyDeviceToHost, s));
   if (resid) ED_HIP(hipMemcpyAsync(resid, b->d_resid, sizeof(double) * 2 * M, hipMemcpyDeviceToHost, s));
   ED_HIP(hipStreamSynchronize(s));
   return ED_OK;
@@ -1044,5 +1477,115 @@ extern "C" int ed_batch_info(const ed_batch* b, int64_t* wave_max_dim, int32_t* 
   if (grid_blocks) *grid_blocks = b->grid;
   if (wave_blocks) *wave_blocks = b->wave_blocks;
   if (tune_ms) *tune_ms = b->tune_ms;
+  return ED_OK;
+}
+
+__global__ void k_reset_one(int32_t* c) {
+  if (threadIdx.x == 0) *c = 0;
+}
+
+extern "C" int ed_batch_backward(ed_batch* b, const double* dl_dre, const double* dl_dim, double* grad_eps,
+                                 double* grad_v, int32_t* cg_iters, double* elapsed_ms, void* stream) {
+  clear_error();
+  ED_REQUIRE(b, "NULL batch");
+  ED_REQUIRE(b->keep, "the batch was created without keep_basis");
+  ED_REQUIRE(b->solved_nw > 0, "call ed_batch_solve before ed_batch_backward");
+  ED_REQUIRE(dl_dre && dl_dim, "NULL gradient input");
+  hipStream_t s = as_stream(stream);
+  const int64_t M = b->num_models;
+  const int32_t nw = b->solved_nw;
+  if (!b->d_gre) {
+    ED_TRY(balloc(&b->d_gre, M * nw));
+    ED_TRY(balloc(&b->d_gim, M * nw));
+  }
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+  const int grid = (int)std::min<int64_t>(M, 2 * (int64_t)cus);
+  int32_t lcap = 1;
+  int64_t vcap = 1;
+  for (int64_t i = 0; i < M; ++i) {
+    const ModelParams& p = b->params[i];
+    const int ns = p.nb + 1;
+    for (int d = -1; d <= 1; ++d) lcap = std::max<int32_t>(lcap, (int32_t)binom_u(ns, std::max(0, p.n_up + d)));
+    lcap = std::max<int32_t>(lcap, (int32_t)binom_u(ns, p.n_dn));
+    vcap = std::max(vcap, b->dmax[i]);
+  }
+  const int64_t slot = BwdSlot{SlotRef{nullptr, lcap, vcap, 0, b->kcap}}.bytes(nw);
+  if (slot * grid > b->bws_bytes) {
+    if (b->d_bws) (void)hipFree(b->d_bws);
+    b->d_bws = nullptr;
+    b->bws_bytes = 0;
+    ED_TRY(balloc(&b->d_bws, slot * grid));
+    b->bws_bytes = slot * grid;
+  }
+  ED_HIP(hipMemcpyAsync(b->d_gre, dl_dre, sizeof(double) * M * nw, hipMemcpyHostToDevice, s));
+  ED_HIP(hipMemcpyAsync(b->d_gim, dl_dim, sizeof(double) * M * nw, hipMemcpyHostToDevice, s));
+  BwdConfig bc{};
+  bc.params = b->d_params;
+  bc.num_models = (int32_t)M;
+  bc.counter = b->d_bcounter;
+  bc.ws = b->d_bws;
+  bc.slot_bytes = slot;
+  bc.lcap = lcap;
+  bc.vcap = vcap;
+  bc.kcap = b->kcap;
+  bc.nw = nw;
+  bc.sign_width = b->sign_width;
+  bc.max_bath = b->max_bath;
+  bc.wn = b->d_wn;
+  bc.gre = b->d_gre;
+  bc.gim = b->d_gim;
+  bc.off_psi = b->d_off_psi;
+  bc.off_q = b->d_off_q;
+  bc.store_psi = b->d_store_psi;
+  bc.store_q = b->d_store_q;
+  bc.store_coef = b->d_store_coef;
+  bc.cg_max = 4000;
+  bc.cg_tol = 1e-12;
+  bc.grad_eps = b->d_grad_eps;
+  bc.grad_v = b->d_grad_v;
+  bc.cg_iters = b->d_cg;
+  hipEvent_t e0, e1;
+  ED_HIP(hipEventCreate(&e0));
+  ED_HIP(hipEventCreate(&e1));
+  (void)hipEventRecord(e0, s);
+  hipLaunchKernelGGL(k_reset_one, dim3(1), dim3(64), 0, s, b->d_bcounter);
+  hipLaunchKernelGGL(k_batch_backward, dim3((unsigned)grid), dim3(kBlock), 0, s, bc);
+  const hipError_t el = hipGetLastError();
+  (void)hipEventRecord(e1, s);
+  const hipError_t es = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (el != hipSuccess || es != hipSuccess)
+    return set_error(ED_ERR_HIP, "batch backward failed: %s", hipGetErrorString(el != hipSuccess ? el : es));
+  if (elapsed_ms) *elapsed_ms = ms;
+  if (grad_eps)
+    ED_HIP(hipMemcpyAsync(grad_eps, b->d_grad_eps, sizeof(double) * M * b->max_bath, hipMemcpyDeviceToHost, s));
+  if (grad_v) ED_HIP(hipMemcpyAsync(grad_v, b->d_grad_v, sizeof(double) * M * b->max_bath, hipMemcpyDeviceToHost, s));
+  if (cg_iters) ED_HIP(hipMemcpyAsync(cg_iters, b->d_cg, sizeof(int32_t) * M, hipMemcpyDeviceToHost, s));
+  ED_HIP(hipStreamSynchronize(s));
+  return ED_OK;
+}
+
+extern "C" int ed_batch_set_bath(ed_batch* b, const double* eps_bath, const double* v, void* stream) {
+  clear_error();
+  ED_REQUIRE(b && eps_bath && v, "NULL argument");
+  for (int64_t i = 0; i < b->num_models; ++i) {
+    ModelParams& p = b->params[i];
+    for (int k = 0; k < p.nb; ++k) {
+      const double e = eps_bath[i * b->max_bath + k], x = v[i * b->max_bath + k];
+      ED_REQUIRE(std::isfinite(e) && std::isfinite(x), "model %lld: parameters must be finite", (long long)i);
+      p.eps[k] = e;
+      p.v[k] = x;
+    }
+  }
+  hipStream_t s = as_stream(stream);
+  ED_HIP(hipMemcpyAsync(b->d_params, b->params.data(), sizeof(ModelParams) * b->num_models, hipMemcpyHostToDevice,
+                        s));
+  ED_HIP(hipStreamSynchronize(s));
+  b->solved_nw = -1;  // the kept solve no longer matches the parameters
   return ED_OK;
 }

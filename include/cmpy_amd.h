@@ -272,7 +272,7 @@ typedef struct ed_batch_desc {
   int32_t max_iter;         /* ground-state Lanczos steps cap (1..2000)         */
   int32_t cf_steps;         /* continued-fraction depth cap (1..2000)           */
   int32_t check_every;      /* steps between ground-state convergence tests     */
-  int32_t pad;
+  int32_t keep_basis;       /* 1: keep psi and the Krylov bases for ed_batch_backward */
   double tol;               /* ground-state Ritz residual tolerance              */
   uint64_t seed;            /* Lanczos start-vector seed                        */
   double cf_tol;            /* continued-fraction stop: relative change of G at
@@ -294,11 +294,28 @@ int ed_batch_info(const ed_batch* b, int64_t* wave_max_dim, int32_t* grid_blocks
 
 /* Solve every model.  wn: host [nw] Matsubara frequencies w_n (z = i w_n).
  * Host outputs (any may be NULL): e0[num_models]; g[num_models][nw] complex
- * interleaved; iters[num_models] ground-state Lanczos steps;
+ * interleaved; iters[num_models][3] Lanczos steps (ground state, particle,
+ * hole continued fraction);
  * resid[num_models][2] = (Ritz residual, explicit |H psi - e0 psi|);
  * *elapsed_ms = device time of the launch.                                 */
 int ed_batch_solve(ed_batch* b, const double* wn, int32_t nw, double* e0, double* g, int32_t* iters,
                    double* resid, double* elapsed_ms, void* stream);
+
+/* Replace every model's bath (host [num_models][max_bath] eps_bath and v, the
+ * first num_bath entries used); the launch shape and the captured graph are
+ * kept, so a fit loop solves again without re-tuning.                       */
+int ed_batch_set_bath(ed_batch* b, const double* eps_bath, const double* v, void* stream);
+
+/* Adjoint backward of the last ed_batch_solve (needs keep_basis).  No
+ * reference counterpart: it turns the bath fit of cmpy/dmft into a gradient
+ * method.  Input: dL/dRe G and dL/dIm G, host [num_models][nw] each, for a
+ * real loss L.  Output (host, [num_models][max_bath]): dL/d eps_bath and
+ * dL/d v; cg_iters[num_models]: iterations of the projected ground-state
+ * response solve.  Uses Hellmann-Feynman for dE0, the kept Krylov bases for
+ * the resolvents and conjugate gradients on (H - E0) over the complement of
+ * the ground state for the state response.                                 */
+int ed_batch_backward(ed_batch* b, const double* dl_dre, const double* dl_dim, double* grad_eps, double* grad_v,
+                      int32_t* cg_iters, double* elapsed_ms, void* stream);
 
 #ifdef __cplusplus
 }

@@ -836,6 +836,104 @@ def free_fermion_energy(num_sites, bonds, hop, eps, n_up, n_dn):
 
 
 # Which check pins each function (besides retyping it):
+# =============================================================================
+# Bath-fit gradient (no reference function: the reference fits the bath with
+# scipy optimisers, cmpy/dmft/utils.py:163-280; this is the FP64 CPU reference
+# gradient for the batched adjoint backward)
+# =============================================================================
+
+
+def siam_gf_t0(u, eps_imp, eps_bath, v, mu, n_up, n_dn, wn, width=0):
+    """Impurity G(i w_n) at T = 0 by dense Lehmann sums (lehmann_t0)."""
+    ns = len(eps_bath) + 1
+    mode = "reference" if width == 0 else "jordan_wigner"
+    op = siam_operator(n_up, n_dn, u, eps_imp, eps_bath, v, mu, width)
+    w, vv = np.linalg.eigh(op.csr().toarray())
+    e0, psi = w[0], vv[:, 0]
+    opp = siam_operator(n_up + 1, n_dn, u, eps_imp, eps_bath, v, mu, width) if n_up + 1 <= ns else None
+    opm = siam_operator(n_up - 1, n_dn, u, eps_imp, eps_bath, v, mu, width) if n_up >= 1 else None
+    cd = apply_cdag(psi, ns, n_up, n_dn, 0, UP, mode) if opp is not None else None
+    c = apply_c(psi, ns, n_up, n_dn, 0, UP, mode) if opm is not None else None
+    return lehmann_t0(op, opp, opm, psi, e0, cd, c, 1j * np.asarray(wn))
+
+
+def bathfit_grad_dense(u, eps_imp, eps_bath, v, mu, n_up, n_dn, wn, g_target, width=0):
+    """L = sum_n |G(i w_n) - g_target_n|^2 and dL/d(eps_bath, v), FP64, dense.
+
+    Adjoint formula with exact resolvents (dense solves) and exact ground
+    state (dense eigh); w_n = 2 conj(G_n - T_n):
+      dL/dtheta = S+ - S- + (B - A) dE0/dtheta - 2 psi^T dH zeta,
+    S+ = Re sum_n w_n x_n^T dH+ x_n, A = Re sum_n w_n x_n^T x_n
+    (x_n = (z_n + E0 - H+)^-1 c+ psi), S-, B likewise with
+    y_n = (z_n - E0 + H-)^-1 c psi, dE0/dtheta = psi^T dH psi, and
+    (H - E0) zeta = P Re(c X + c+ Y), X = sum w_n x_n, Y = sum w_n y_n.
+    dH/d eps_k and dH/d v_k are the operators of the same model with unit
+    eps_k (resp. v_k) and every other parameter zero (H is linear in them).
+    Pinned by central finite differences of :func:`siam_gf_t0` in the tests.
+    """
+    eps_bath = np.asarray(eps_bath, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    nb = len(eps_bath)
+    ns = nb + 1
+    z = 1j * np.asarray(wn, dtype=np.float64)
+    mode = "reference" if width == 0 else "jordan_wigner"
+
+    def dense(nu, eb, vv, uu=u, ei=eps_imp, m=mu):
+        return siam_operator(nu, n_dn, uu, ei, eb, vv, m, width).csr().toarray()
+
+    H = dense(n_up, eps_bath, v)
+    w, vv = np.linalg.eigh(H)
+    e0, psi = w[0], vv[:, 0]
+    D = len(psi)
+    has_p, has_m = n_up + 1 <= ns, n_up >= 1
+    eye = np.eye(D)
+    G = np.zeros(len(z), dtype=np.complex128)
+    xs = ys = None
+    if has_p:
+        Hp = dense(n_up + 1, eps_bath, v)
+        phip = apply_cdag(psi, ns, n_up, n_dn, 0, UP, mode)
+        xs = np.array([np.linalg.solve((zz + e0) * np.eye(len(phip)) - Hp, phip) for zz in z])
+        G += xs @ phip
+        Cd = np.array([apply_cdag(e, ns, n_up, n_dn, 0, UP, mode) for e in eye]).T
+    if has_m:
+        Hm = dense(n_up - 1, eps_bath, v)
+        phim = apply_c(psi, ns, n_up, n_dn, 0, UP, mode)
+        ys = np.array([np.linalg.solve((zz - e0) * np.eye(len(phim)) + Hm, phim) for zz in z])
+        G += ys @ phim
+        Cm = np.array([apply_c(e, ns, n_up, n_dn, 0, UP, mode) for e in eye]).T
+    diff = G - np.asarray(g_target)
+    loss = float(np.sum(np.abs(diff) ** 2))
+    wgt = 2 * np.conj(diff)
+    r = np.zeros(D)
+    A = B = 0.0
+    if has_p:
+        r += np.real(Cd.T @ (wgt[:, None] * xs).sum(0))
+        A = float(np.real(np.sum(wgt * np.einsum("na,na->n", xs, xs))))
+    if has_m:
+        r += np.real(Cm.T @ (wgt[:, None] * ys).sum(0))
+        B = float(np.real(np.sum(wgt * np.einsum("na,na->n", ys, ys))))
+    proj = eye - np.outer(psi, psi)
+    zeta = proj @ np.linalg.lstsq(H - e0 * eye, proj @ r, rcond=None)[0]
+    grads = {}
+    for kind in ("eps", "v"):
+        g = np.zeros(nb)
+        for k in range(nb):
+            unit = np.zeros(nb)
+            unit[k] = 1.0
+            eb1, v1 = (unit, np.zeros(nb)) if kind == "eps" else (np.zeros(nb), unit)
+            dHn = dense(n_up, eb1, v1, 0.0, 0.0, 0.0)
+            val = (B - A) * (psi @ dHn @ psi) - 2 * psi @ dHn @ zeta
+            if has_p:
+                dHp = dense(n_up + 1, eb1, v1, 0.0, 0.0, 0.0)
+                val += np.real(np.sum(wgt * np.einsum("na,ab,nb->n", xs, dHp, xs)))
+            if has_m:
+                dHm = dense(n_up - 1, eb1, v1, 0.0, 0.0, 0.0)
+                val -= np.real(np.sum(wgt * np.einsum("na,ab,nb->n", ys, dHm, ys)))
+            g[k] = val
+        grads[kind] = g
+    return loss, grads["eps"], grads["v"]
+
+
 PARITY = {
     "generate_states_permutations": "test_basis.py:124-141 state tables",
     "sector_states": "generate_states_permutations for ns<=8; test_basis.py:124-141",
@@ -856,4 +954,6 @@ PARITY = {
     "gf_lehmann": "test_exactdiag.py:24-96 (U=0 vs gf0_lehmann, atomic peaks, <n>=1/2)",
     "gf0_lehmann": "chain tight-binding resolvent computed by direct inversion",
     "impurity_gf_ref": "lehmann_t0 on the two-site SIAM",
+    "siam_gf_t0": "lehmann_t0 (pinned above) on the Anderson model",
+    "bathfit_grad_dense": "central finite differences (h=1e-5) of the loss built on siam_gf_t0",
 }

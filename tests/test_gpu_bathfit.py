@@ -1,0 +1,141 @@
+"""Adjoint backward of the batched Anderson solve (bath-fit gradients).
+
+Checked against the oracle's dense FP64 reference gradient (small baths) and
+against central finite differences (h = 1e-5) of the batched forward at the
+bath-fit workload's size (7 bath sites, D = 4,900).  Tolerance: 1e-6 of the
+largest gradient entry of each model.
+"""
+import numpy as np
+import pytest
+
+from oracle import ed_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GRAD_RTOL = 1e-6
+
+
+def _loss_grads(b, wn, target):
+    """Loss, dL/dRe G and dL/dIm G of sum |G - target|^2 at the batch's last solve."""
+    res = b.solve(wn)
+    diff = res.g - target
+    return np.sum(np.abs(diff) ** 2, axis=1), 2 * diff.real, 2 * diff.imag, res
+
+
+def _check(got, ref, what):
+    for i in range(ref.shape[0]):
+        scale = np.abs(ref[i]).max()
+        assert np.abs(got[i] - ref[i]).max() <= GRAD_RTOL * scale, (what, i, got[i], ref[i])
+
+
+@pytest.mark.parametrize("width_mode", ["reference", "fermion"])
+def test_backward_matches_dense_reference(gpu, width_mode):
+    from cmpy_amd.batch import AndersonBatch, matsubara
+
+    rng = np.random.default_rng(17)
+    M = 6
+    nb = rng.integers(3, 6, size=M)
+    u = np.full(M, 3.0)
+    eb = [rng.uniform(-3, 3, n) for n in nb]
+    v = [rng.uniform(0.1, 1.2, n) for n in nb]
+    wn = matsubara(100.0, 48)
+    b = AndersonBatch(u=u, eps_imp=-u / 2, eps_bath=eb, v=v, mu=0.0, keep_basis=True, tol=1e-12,
+                      cf_tol=1e-14, sign_mode=width_mode)
+    targets = []
+    for i in range(M):
+        n = (nb[i] + 1) // 2
+        w = 0 if width_mode == "reference" else nb[i] + 1
+        targets.append(O.siam_gf_t0(3.0, -1.5, rng.uniform(-3, 3, nb[i]), rng.uniform(0.1, 1.2, nb[i]), 0.0,
+                                    n, n, wn, w))
+    target = np.array(targets)
+    loss, gr, gi, res = _loss_grads(b, wn, target)
+    ge, gv, cg, _ = b.backward(gr, gi)
+    ref_e = np.zeros_like(ge)
+    ref_v = np.zeros_like(gv)
+    for i in range(M):
+        n = (nb[i] + 1) // 2
+        w = 0 if width_mode == "reference" else nb[i] + 1
+        L, de, dv = O.bathfit_grad_dense(3.0, -1.5, eb[i], v[i], 0.0, n, n, wn, target[i], w)
+        assert loss[i] == pytest.approx(L, rel=1e-9)
+        ref_e[i, :nb[i]] = de
+        ref_v[i, :nb[i]] = dv
+    _check(np.concatenate([ge, gv], axis=1), np.concatenate([ref_e, ref_v], axis=1), "grad")
+    assert np.all(cg > 0)
+
+
+def test_backward_matches_finite_differences_at_fit_size(gpu):
+    """Seven bath sites, sector (4, 4), beta = 100 (the bath-fit workload's
+    model size), four models; every parameter differenced in all models at once."""
+    from cmpy_amd.batch import AndersonBatch, matsubara
+
+    rng = np.random.default_rng(23)
+    M, nb = 4, 7
+    u = np.full(M, 3.0)
+    eb = rng.uniform(-3, 3, (M, nb))
+    v = rng.uniform(0.1, 1.2, (M, nb))
+    wn = matsubara(100.0, 128)
+    tb = AndersonBatch(u=u, eps_imp=-u / 2, eps_bath=list(rng.uniform(-3, 3, (M, nb))),
+                       v=list(rng.uniform(0.1, 1.2, (M, nb))), mu=0.0, tol=1e-12, cf_tol=1e-14)
+    target = tb.solve(wn).g
+    b = AndersonBatch(u=u, eps_imp=-u / 2, eps_bath=list(eb), v=list(v), mu=0.0, keep_basis=True, tol=1e-12,
+                      cf_tol=1e-14)
+    loss, gr, gi, res = _loss_grads(b, wn, target)
+    ge, gv, cg, _ = b.backward(gr, gi)
+    h = 1e-5
+    fd = np.zeros((M, 2 * nb))
+    for kind in range(2):
+        for k in range(nb):
+            lp = []
+            for sgn in (1, -1):
+                e2, v2 = eb.copy(), v.copy()
+                (e2 if kind == 0 else v2)[:, k] += sgn * h
+                b.set_bath(e2, v2)
+                lp.append(_loss_grads(b, wn, target)[0])
+            fd[:, kind * nb + k] = (lp[0] - lp[1]) / (2 * h)
+    _check(np.concatenate([ge, gv], axis=1), fd, "fd")
+
+
+def test_autograd_function(gpu):
+    """anderson_gf inside a torch loss gives the batch backward's gradients."""
+    import torch
+
+    from cmpy_amd.batch import AndersonBatch, anderson_gf, matsubara
+
+    rng = np.random.default_rng(31)
+    M, nb = 3, 4
+    u = np.full(M, 2.0)
+    eb = rng.uniform(-2, 2, (M, nb))
+    v = rng.uniform(0.2, 1.0, (M, nb))
+    wn = matsubara(50.0, 32)
+    b = AndersonBatch(u=u, eps_imp=-u / 2, eps_bath=list(eb), v=list(v), mu=0.0, keep_basis=True, tol=1e-12,
+                      cf_tol=1e-14)
+    target = torch.from_numpy(b.solve(wn).g * 0.9)
+    te = torch.tensor(eb, requires_grad=True)
+    tv = torch.tensor(v, requires_grad=True)
+    g = anderson_gf(te, tv, b, wn)
+    loss = (g - target).abs().pow(2).sum()
+    loss.backward()
+    diff = g.detach().numpy() - target.numpy()
+    ge, gv, _, _ = b.backward(2 * diff.real, 2 * diff.imag)
+    np.testing.assert_allclose(te.grad.numpy(), ge, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(tv.grad.numpy(), gv, rtol=1e-12, atol=1e-14)
+    for i in range(M):
+        n = (nb + 1) // 2
+        L, de, dv = O.bathfit_grad_dense(2.0, -1.0, eb[i], v[i], 0.0, n, n, wn, target.numpy()[i])
+        assert np.abs(te.grad.numpy()[i] - de).max() <= GRAD_RTOL * np.abs(np.r_[de, dv]).max()
+
+
+def test_backward_errors(gpu):
+    from cmpy_amd import _lib as L
+    from cmpy_amd.batch import AndersonBatch, matsubara
+
+    b = AndersonBatch(u=2.0, eps_imp=-1.0, eps_bath=[np.zeros(3)], v=[np.ones(3)])
+    with pytest.raises(ValueError, match="keep_basis"):
+        b.backward(np.zeros((1, 4)), np.zeros((1, 4)))
+    bk = AndersonBatch(u=2.0, eps_imp=-1.0, eps_bath=[np.zeros(3)], v=[np.ones(3)], keep_basis=True)
+    with pytest.raises(L.EDInvalidArgument, match="solve"):
+        bk.backward(np.zeros((1, 4)), np.zeros((1, 4)))
+    bk.solve(matsubara(10.0, 4))
+    bk.set_bath([np.ones(3)], [np.ones(3)])
+    with pytest.raises(L.EDInvalidArgument, match="solve"):
+        bk.backward(np.zeros((1, 4)), np.zeros((1, 4)))

@@ -352,3 +352,34 @@ def test_gf_lehmann_atomic_and_occupation(ns, u, ref_fixtures):
     mk = lambda nu, nd: O.hubbard_operator(ns, nu, nd, O.chain_bonds(ns), u, 0.0, 1.0, mu=u / 2)
     m = O.gf_lehmann(mk, ns, z[:5], 10.0, pos=0, occ=True)
     assert abs(m.occ - 0.5) < 1e-3
+
+
+@pytest.mark.parametrize("nb,width", [(3, 0), (4, 0), (4, 5)])
+def test_bathfit_grad_dense_matches_finite_differences(nb, width):
+    """The CPU reference gradient of the bath-fit loss against central
+    differences (h = 1e-5) of the loss itself."""
+    rng = np.random.default_rng(40 + nb + width)
+    u, eimp = 3.0, -1.5
+    n_up = n_dn = (nb + 1) // 2
+    eb, v = rng.uniform(-3, 3, nb), rng.uniform(0.1, 1.2, nb)
+    wn = (2 * np.arange(32) + 1) * np.pi / 100
+    target = O.siam_gf_t0(u, eimp, rng.uniform(-3, 3, nb), rng.uniform(0.1, 1.2, nb), 0.0, n_up, n_dn, wn, width)
+    loss, ge, gv = O.bathfit_grad_dense(u, eimp, eb, v, 0.0, n_up, n_dn, wn, target, width)
+    assert loss == pytest.approx(np.sum(np.abs(O.siam_gf_t0(u, eimp, eb, v, 0.0, n_up, n_dn, wn, width) - target) ** 2))
+    h = 1e-5
+
+    def L(e, vv):
+        return np.sum(np.abs(O.siam_gf_t0(u, eimp, e, vv, 0.0, n_up, n_dn, wn, width) - target) ** 2)
+
+    fd = []
+    for kind in ("eps", "v"):
+        for k in range(nb):
+            d = np.zeros(nb)
+            d[k] = h
+            if kind == "eps":
+                fd.append((L(eb + d, v) - L(eb - d, v)) / (2 * h))
+            else:
+                fd.append((L(eb, v + d) - L(eb, v - d)) / (2 * h))
+    fd = np.array(fd)
+    got = np.concatenate([ge, gv])
+    assert np.max(np.abs(got - fd)) <= 1e-6 * np.max(np.abs(fd))
