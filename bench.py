@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--no-lanczos", action="store_true")
     p.add_argument("--no-secondary", action="store_true")
     p.add_argument("--size", choices=["4x4", "4x3"], default="4x4")
+    p.add_argument("--secondary-seconds", type=float, default=10.0,
+                   help="CPU seconds per secondary workload's baseline sample")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run device copies of one sector vector (the reference kernel for "
                         "scaling the FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -200,16 +202,190 @@ def main():
         del tw, xh, yh
 
     if not args.no_secondary:
-        try:
-            from cmpy_amd import bench_secondary  # optional workloads
-
-            result["secondary"] = bench_secondary.run(log)
-        except ImportError:
-            pass
-        except Exception as exc:  # report, never hide the headline
-            result["secondary"] = {"error": repr(exc)}
+        result["secondary"] = {}
+        for name, fn in (("batched_anderson", bench_batched), ("bathfit_gradients", bench_bathfit)):
+            try:
+                result["secondary"][name] = fn(args, log)
+            except Exception as exc:  # report, never hide the headline
+                result["secondary"][name] = {"error": repr(exc)}
 
     print(json.dumps(result), flush=True)
+
+
+def _cpu_single_thread():
+    """BLAS limited to one thread for the CPU baselines (cores = 1)."""
+    try:
+        from threadpoolctl import threadpool_limits
+
+        return threadpool_limits(limits=1)
+    except ImportError:  # pragma: no cover
+        import contextlib
+
+        return contextlib.nullcontext()
+
+
+def _cpu_impurity_solve(O, u, eimp, eb, v, nu, nd, wn, cf_steps=200):
+    """The oracle's port of one impurity solve: Lanczos/dense ground state,
+    then particle and hole continued fractions at the Matsubara points."""
+    import numpy as np
+
+    ns = len(eb) + 1
+    op = O.siam_operator(nu, nd, u, eimp, eb, v, 0.0)
+    e0, psi = O.groundstate(op, tol=1e-10)
+    g = np.zeros(len(wn), dtype=np.complex128)
+    for create, sign in ((True, 1), (False, -1)):
+        nn = nu + 1 if create else nu - 1
+        if nn < 0 or nn > ns:
+            continue
+        phi = (O.apply_cdag if create else O.apply_c)(psi, ns, nu, nd, 0, O.UP)
+        opx = O.siam_operator(nn, nd, u, eimp, eb, v, 0.0)
+        a, b, n2, _ = O.lanczos_normalized(opx.matvec, phi, min(cf_steps, opx.size))
+        g += O.cf_greens(a, b, n2, 1j * wn, e0, sign)
+    return e0, g
+
+
+def bench_batched(args, log):
+    """512 Anderson impurity models, bath sizes 5..9, ground state to 1e-10 and
+    G(i w_n) at 256 Matsubara frequencies (beta = 50), one persistent launch."""
+    import time
+
+    import numpy as np
+
+    from cmpy_amd.batch import AndersonBatch, matsubara, random_batch
+    from cmpy_amd.exactdiag import greens_cf, groundstate
+    from cmpy_amd.models import SingleImpurityAndersonModel
+
+    M, seed = 512, 2024
+    p = random_batch(M, seed)
+    wn = matsubara(50.0, 256)
+    b = AndersonBatch(**p)
+    t0 = time.perf_counter()
+    r = b.solve(wn)  # first solve autotunes the launch shape
+    tune_s = time.perf_counter() - t0
+    ms = []
+    for _ in range(5):
+        r = b.solve(wn)
+        ms.append(r.elapsed_ms)
+    ms_med = float(np.median(ms))
+    out = {"metric": "solves/s", "value": round(M / (ms_med * 1e-3), 1), "unit": "solves/s", "models": M,
+           "matsubara": 256, "beta": 50.0, "ms_per_batch": round(ms_med, 3), "tune_s": round(tune_s, 2),
+           "launch": b.info(), "max_residual": float(r.residual.max()),
+           "ground_state_steps_mean": float(r.iterations.mean()),
+           "cf_steps_mean": [float(x) for x in r.cf_steps.mean(axis=0)],
+           "data": f"synthetic (random_batch seed {seed})"}
+    log(f"batched Anderson: {out['value']} solves/s ({ms_med:.1f} ms per 512)")
+    # plain per-model GPU path on a sample: one model, one call chain at a time
+    sample = list(range(0, M, M // 16))
+    t0 = time.perf_counter()
+    for i in sample:
+        m = SingleImpurityAndersonModel(u=p["u"][i], eps_imp=p["eps_imp"][i], eps_bath=p["eps_bath"][i],
+                                        v=p["v"][i], mu=0.0)
+        nu = (len(p["eps_bath"][i]) + 1) // 2
+        gs = groundstate(m, nu, nu, tol=1e-10)
+        greens_cf(m, 1j * wn, nu, nu, gs=gs)
+    t_plain = time.perf_counter() - t0
+    out["plain_gpu"] = {"value": round(len(sample) / t_plain, 2), "unit": "solves/s", "sample": len(sample)}
+    out["speedup_vs_plain"] = round(out["value"] / out["plain_gpu"]["value"], 1)
+    if not args.no_cpu:
+        import sys
+
+        sys.path.insert(0, str(ROOT))
+        from oracle import ed_oracle as O
+
+        done, t = 0, 0.0
+        with _cpu_single_thread():
+            for i in range(0, M, 7):
+                nu = (len(p["eps_bath"][i]) + 1) // 2
+                t1 = time.perf_counter()
+                _cpu_impurity_solve(O, p["u"][i], p["eps_imp"][i], p["eps_bath"][i], p["v"][i], nu, nu, wn)
+                t += time.perf_counter() - t1
+                done += 1
+                if t > args.secondary_seconds:
+                    break
+        out["cpu_baseline"] = {"value": round(done / t, 3), "unit": "solves/s", "cores": 1, "kind": "port",
+                               "sample": f"{done} models of the same batch (every 7th) by the oracle's numpy/scipy "
+                                         "port: ground state + continued fractions"}
+    return out
+
+
+def bench_bathfit(args, log):
+    """Bath-fit gradients: 256 Anderson models, 7 bath sites, sector (4,4)
+    (D = 4,900), U = 3, 512 Matsubara frequencies at beta = 100; loss
+    sum |G - G_target|^2 with G_target from a second seed's parameters."""
+    import time
+
+    import numpy as np
+
+    from cmpy_amd.batch import AndersonBatch, matsubara
+
+    M, nb = 256, 7
+    ra, rb = np.random.default_rng(101), np.random.default_rng(202)
+    u = np.full(M, 3.0)
+    ea, va = ra.uniform(-3, 3, (M, nb)), ra.uniform(0.1, 1.2, (M, nb))
+    eb_t, vb_t = rb.uniform(-3, 3, (M, nb)), rb.uniform(0.1, 1.2, (M, nb))
+    wn = matsubara(100.0, 512)
+    tb = AndersonBatch(u=u, eps_imp=-u / 2, eps_bath=list(eb_t), v=list(vb_t), mu=0.0)
+    target = tb.solve(wn).g
+    del tb
+    b = AndersonBatch(u=u, eps_imp=-u / 2, eps_bath=list(ea), v=list(va), mu=0.0, keep_basis=True)
+
+    def step(batch):
+        r = batch.solve(wn)
+        d = r.g - target
+        ge, gv, cg, ms_b = batch.backward(2 * d.real, 2 * d.imag)
+        return r, ms_b, ge, gv
+
+    step(b)  # tune + warm
+    fw, bw = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        r, ms_b, ge, gv = step(b)
+        fw.append(r.elapsed_ms)
+        bw.append(ms_b)
+    ms = float(np.median(np.array(fw) + np.array(bw)))
+    out = {"metric": "grads/s", "value": round(M / (ms * 1e-3), 1), "unit": "grads/s", "models": M,
+           "bath_sites": nb, "dim": 4900, "matsubara": 512, "beta": 100.0,
+           "ms_forward": round(float(np.median(fw)), 3), "ms_backward": round(float(np.median(bw)), 3),
+           "data": "synthetic (theta from seed 101, target from seed 202)"}
+    log(f"bath-fit gradients: {out['value']} grads/s")
+    # plain per-model GPU path: each model alone (its own launches)
+    t0 = time.perf_counter()
+    ns = 8
+    for i in range(ns):
+        bi = AndersonBatch(u=u[i:i + 1], eps_imp=-u[i:i + 1] / 2, eps_bath=[ea[i]], v=[va[i]], mu=0.0,
+                           keep_basis=True)
+        bi.config(-1, 1, 0)
+        r = bi.solve(wn)
+        d = r.g - target[i:i + 1]
+        bi.backward(2 * d.real, 2 * d.imag)
+    t_plain = time.perf_counter() - t0
+    out["plain_gpu"] = {"value": round(ns / t_plain, 2), "unit": "grads/s", "sample": ns}
+    out["speedup_vs_plain"] = round(out["value"] / out["plain_gpu"]["value"], 1)
+    if not args.no_cpu:
+        import sys
+
+        sys.path.insert(0, str(ROOT))
+        from oracle import ed_oracle as O
+
+        # CPU: central differences of the oracle's forward (the derivative a
+        # CPU fit of the reference would use), one model
+        h = 1e-5
+        t0 = time.perf_counter()
+        with _cpu_single_thread():
+            def loss(e, v):
+                _, g = _cpu_impurity_solve(O, 3.0, -1.5, e, v, 4, 4, wn)
+                return float(np.sum(np.abs(g - target[0]) ** 2))
+
+            for k in range(2 * nb):
+                d = np.zeros(2 * nb)
+                d[k] = h
+                loss(ea[0] + d[:nb], va[0] + d[nb:])
+                loss(ea[0] - d[:nb], va[0] - d[nb:])
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(1.0 / t, 4), "unit": "grads/s", "cores": 1, "kind": "port",
+                               "sample": "1 model: central differences (28 solves) with the oracle's numpy/scipy "
+                                         "port of the forward"}
+    return out
 
 
 if __name__ == "__main__":

@@ -368,7 +368,7 @@ __device__ cplx cf_value(const double* a, const double* b, int m, double norm2, 
 // Lanczos coefficients of the sector (U, Dn) from the start vector in vec(1)
 // (unnormalised, |u|^2 = n2), stopping at breakdown, at cap steps, or when the
 // continued fraction s n2 / (s z + e0 - a0 - ...) at z = i wmin has changed by
-// at most cf_tol (relative) over each of the last two 8-step checks (the
+// at most cf_tol * max(|G|, n2) over each of the last two 8-step checks (the
 // fraction converges last at the frequency closest to the real axis).
 // Returns the step count.
 template <bool BLOCK>
@@ -398,8 +398,10 @@ __device__ int cf_coeffs(const Group<BLOCK>& g, const ModelParams& p, const Spin
       g.sync();
       if (g.rank == 0) {
         const cplx c = cf_value(a, b, m, n2, e0, sgn, wmin);
+        // change measured against the part's spectral weight n2 as well, so a
+        // small |G(i wmin)| does not make the test unreachable
         const double d = hypot(c.re - prev.re, c.im - prev.im);
-        scal[4] = d <= cf_tol * hypot(c.re, c.im) ? 1.0 : 0.0;
+        scal[4] = d <= cf_tol * fmax(hypot(c.re, c.im), n2) ? 1.0 : 0.0;
         prev = c;
       }
       g.sync();
@@ -746,7 +748,10 @@ __device__ __forceinline__ double bond_hops(const SpinList& U, const SpinList& D
   return s;
 }
 
-__device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi, const BwdSlot& bs) {
+constexpr int kPc = 16;          // W columns per LDS chunk in the backward
+constexpr int kBwdMaxSteps = 512; // LDS chunk rows: continued-fraction steps kept
+
+__device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi, const BwdSlot& bs, double* wl) {
   const ModelParams& p = bc.params[mi];
   const int ns = p.nb + 1;
   const int nb = p.nb;
@@ -851,11 +856,28 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
       }
       Xr[r] = xr;
       Xi[r] = xi;
-      for (int j = 0; j < m; ++j) {
-        double acc = 0.0;
-        for (int i = 0; i < m; ++i) acc += q[i] * W[(int64_t)i * kcap + j];
-        P[r * kcap + j] = acc;
+    }
+    // columns of W in chunks of kPc staged in LDS (read by every thread)
+    for (int j0 = 0; j0 < m; j0 += kPc) {
+      const int jn = m - j0 < kPc ? m - j0 : kPc;
+      for (int t = g.rank; t < m * kPc; t += g.size) {
+        const int i = t / kPc, jj = t - i * kPc;
+        wl[t] = jj < jn ? W[(int64_t)i * kcap + j0 + jj] : 0.0;
       }
+      __syncthreads();
+      for (int64_t r = g.rank; r < Dt; r += g.size) {
+        const double* q = qt + r * kcap;
+        double acc[kPc];
+#pragma unroll
+        for (int jj = 0; jj < kPc; ++jj) acc[jj] = 0.0;
+        for (int i = 0; i < m; ++i) {
+          const double qi = q[i];
+#pragma unroll
+          for (int jj = 0; jj < kPc; ++jj) acc[jj] += qi * wl[i * kPc + jj];
+        }
+        for (int jj = 0; jj < jn; ++jj) P[r * kcap + j0 + jj] = acc[jj];
+      }
+      __syncthreads();
     }
     g.sync();
     // (d) diag of K, its trace, and the per-bond sums
@@ -980,6 +1002,7 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
 __global__ __launch_bounds__(kBlock) void k_batch_backward(BwdConfig bc) {
   __shared__ double red[32];
   __shared__ int popped;
+  __shared__ double wl[kBwdMaxSteps * kPc];  // 64 KiB: a column chunk of Re W
   Group<true> g{(int)threadIdx.x, (int)blockDim.x, red};
   const BwdSlot bs{SlotRef{bc.ws + (int64_t)blockIdx.x * bc.slot_bytes, bc.lcap, bc.vcap, 0, bc.kcap}};
   while (true) {
@@ -988,7 +1011,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_backward(BwdConfig bc) {
     const int q = __builtin_amdgcn_readfirstlane(popped);
     __syncthreads();
     if (q >= bc.num_models) break;
-    backward_model(g, bc, q, bs);
+    backward_model(g, bc, q, bs, wl);
   }
 }
 
@@ -1318,6 +1341,10 @@ extern "C" int ed_batch_create(ed_batch** out, const ed_batch_desc* d, void* str
     return set_error(ED_ERR_HIP, "upload of model parameters failed");
   }
   if (d->keep_basis) {
+    if (d->cf_steps > kBwdMaxSteps) {
+      free_batch(b);
+      return set_error(ED_ERR_INVALID, "keep_basis needs cf_steps <= %d", kBwdMaxSteps);
+    }
     // per-model stores for the adjoint backward
     b->keep = true;
     b->kcap = d->cf_steps;
