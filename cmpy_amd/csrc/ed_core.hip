@@ -173,6 +173,49 @@ __global__ void k_sorted_targets(const uint32_t* pk_rm, const int32_t* cnt, int6
   cnt_pn[i] = (uint32_t)p | ((uint32_t)(c - p) << 16);
 }
 
+// Pair form of the sign-sorted lists (see Tables::pair16).
+__global__ void k_pair_table(const int32_t* tgt, const uint32_t* cnt_pn, int64_t count, int tstride, int pstride,
+                             uint32_t* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t c = cnt_pn[i];
+  const int np = (int)(c & 0xffffu), nn = (int)(c >> 16);
+  const int32_t* t = tgt + i * tstride;
+  uint32_t* o = out + i * pstride;
+  const uint32_t pad = (uint32_t)count;
+  int w = 1;
+  for (int k = 0; k < np; k += 2) o[w++] = (uint32_t)t[k] | ((k + 1 < np ? (uint32_t)t[k + 1] : pad) << 16);
+  const int pp = w - 1;
+  for (int k = 0; k < nn; k += 2)
+    o[w++] = (uint32_t)t[np + k] | ((k + 1 < nn ? (uint32_t)t[np + k + 1] : pad) << 16);
+  const int pt = w - 1;
+  for (; w < pstride; ++w) o[w] = pad | (pad << 16);
+  o[0] = (uint32_t)pp | ((uint32_t)pt << 8);
+}
+
+// Quad form of the sign-sorted lists (see Tables::quad16).
+__global__ void k_quad_table(const int32_t* tgt, const uint32_t* cnt_pn, int64_t count, int tstride, int qstride,
+                             uint32_t* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t c = cnt_pn[i];
+  const int np = (int)(c & 0xffffu), nn = (int)(c >> 16);
+  const int32_t* t = tgt + i * tstride;
+  uint32_t* o = out + i * qstride;
+  const uint32_t pad = (uint32_t)count;
+  int w = 1;
+  for (int g = 0; g < 2; ++g) {
+    const int n = g == 0 ? np : nn;
+    const int32_t* tg = g == 0 ? t : t + np;
+    for (int k = 0; k < n; k += 2)
+      o[w++] = (uint32_t)tg[k] | ((k + 1 < n ? (uint32_t)tg[k + 1] : pad) << 16);
+    if (((n + 1) / 2) & 1) o[w++] = pad | (pad << 16);  // complete the last quad
+    if (g == 0) o[0] = (uint32_t)((w - 1) / 2);
+  }
+  o[0] |= (uint32_t)((w - 1) / 2) << 8;
+  for (; w < qstride; ++w) o[w] = pad | (pad << 16);
+}
+
 // Per-spin diagonal: sum_i eps_i over set bits, in ascending site order
 // (the summation order of weighted_element, cmpy/operators.py:246-250).
 __global__ void k_weighted(const uint32_t* states, int64_t count, const double* eps, int num_sites,
@@ -271,6 +314,8 @@ static void free_tables(Tables& t) {
   if (t.cmax_neg) (void)hipFree(t.cmax_neg);
   if (t.tgt_sorted) (void)hipFree(t.tgt_sorted);
   if (t.cnt_pn) (void)hipFree(t.cnt_pn);
+  if (t.pair16) (void)hipFree(t.pair16);
+  if (t.quad16) (void)hipFree(t.quad16);
   t = Tables{};
 }
 
@@ -303,6 +348,38 @@ static int dev_alloc(T** p, int64_t count) {
 
 // Builds the half-width table (see Tables::h16) on the host from the packed
 // column-major table and uploads it.
+// Pair table for the column pass when every row's pair list fits the unrolled
+// maximum; otherwise none is built and the column pass uses the plain lists.
+static int build_pair_table(Tables& tab, int64_t count, hipStream_t s) {
+  std::vector<uint32_t> cp(count);
+  ED_HIP(hipMemcpyAsync(cp.data(), tab.cnt_pn, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, s));
+  ED_HIP(hipStreamSynchronize(s));
+  int mx = 0;
+  for (uint32_t c : cp) mx = std::max(mx, (int)(((c & 0xffffu) + 1) / 2 + ((c >> 16) + 1) / 2));
+  if (mx > kPairMaxPairs) return ED_OK;
+  tab.pair_stride = 1 + mx;
+  // kPairMaxPairs words of slack: the column pass reads that many after any
+  // row's header
+  ED_TRY(dev_alloc(&tab.pair16, count * tab.pair_stride + kPairMaxPairs));
+  ED_HIP(hipMemsetAsync(tab.pair16 + count * tab.pair_stride, 0, sizeof(uint32_t) * kPairMaxPairs, s));
+  hipLaunchKernelGGL(k_pair_table, dim3(grid_for(count, 256)), dim3(256), 0, s, tab.tgt_sorted, tab.cnt_pn, count,
+                     tab.tgt_stride, tab.pair_stride, tab.pair16);
+  ED_HIP(hipGetLastError());
+  // quads
+  int mq = 0;
+  for (uint32_t c : cp) mq = std::max(mq, (int)(((c & 0xffffu) + 3) / 4 + ((c >> 16) + 3) / 4));
+  if (mq <= kQuadMaxQuads) {
+    tab.quad_stride = 1 + 2 * mq;
+    ED_TRY(dev_alloc(&tab.quad16, count * tab.quad_stride + tab.quad_stride + kQuadWords));
+    ED_HIP(hipMemsetAsync(tab.quad16 + count * tab.quad_stride, 0,
+                          sizeof(uint32_t) * (tab.quad_stride + kQuadWords), s));
+    hipLaunchKernelGGL(k_quad_table, dim3(grid_for(count, 256)), dim3(256), 0, s, tab.tgt_sorted, tab.cnt_pn,
+                       count, tab.tgt_stride, tab.quad_stride, tab.quad16);
+    ED_HIP(hipGetLastError());
+  }
+  return ED_OK;
+}
+
 static int build_half_tables(Tables& tab, int64_t count, int width, hipStream_t s) {
   constexpr int L = kColourLanes;
   std::vector<uint32_t> pk((size_t)count * width);
@@ -442,6 +519,7 @@ static int build_spin(ed_op* op, int n, uint32_t** states, double** diag, Tables
     if (count <= kHalfMaxStates && count >= 4096) ED_TRY(build_half_tables(tab, count, nb, s));
     hipLaunchKernelGGL(k_sorted_targets, dim3(grid_for(count, 256)), dim3(256), 0, s, tab.pk_rm, tab.cnt, count, nb,
                        tab.tgt_stride, tab.tgt_sorted, tab.cnt_pn);
+    if (count < 65536) ED_TRY(build_pair_table(tab, count, s));
   }
   ED_HIP(hipGetLastError());
   // host-side maximum hop count (used by kernels that size per-lane lists)

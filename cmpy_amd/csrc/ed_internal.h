@@ -152,7 +152,27 @@ struct Tables {
   int32_t* tgt_sorted = nullptr;
   uint32_t* cnt_pn = nullptr;
   int32_t tgt_stride = 0;
+  // Pair form of the sign-sorted lists (column pass over 32-column tiles),
+  // for D < 65536: row i is pair_stride words; word 0 = number of sign +1
+  // pairs | total pairs << 8, then the pairs, each two 16-bit targets (low
+  // half first).  The +1 targets fill the first pairs and the -1 targets the
+  // rest; a group of odd length ends in the pad entry D.  Words past the
+  // row's pairs hold pads, and kPairMaxPairs zero words follow the last row.
+  uint32_t* pair16 = nullptr;
+  int32_t pair_stride = 0;
+  // Quad form (column pass with 16-byte loads): row i is quad_stride words;
+  // word 0 = number of sign +1 quads | total quads << 8, then the quads, each
+  // four 16-bit targets in two words (entry 4m + q is half q & 1 of word
+  // 2m + (q >> 1)).  Each sign group is padded to a multiple of four with
+  // the pad entry D; words past the row's quads hold pads, and quad_stride
+  // words of slack follow the last row.
+  uint32_t* quad16 = nullptr;
+  int32_t quad_stride = 0;
 };
+
+constexpr int kPairMaxPairs = 16;  // longest pair list the column pass unrolls
+constexpr int kQuadMaxQuads = 12;  // longest quad list the column pass unrolls
+constexpr int kQuadWords = 1 + 2 * kQuadMaxQuads;
 
 // Largest spin-sector size with half-width row-pass tables: entries and the 32
 // pads D .. D+31 must fit 16 bits, and the staged row with its pads must fit

@@ -128,13 +128,15 @@ __global__ __launch_bounds__(256) void k_hv(HvArgs a, const double* __restrict__
 // ---------------------------------------------------------------------------
 // Two-pass sweep for large sectors.
 //
-// Pass R (rows): a workgroup stages R whole rows of x in LDS and writes
-//   y = s*(diag + dn hops) - c*xprev   (+ block partial of <y, x>)
-// so every dn-hop gather is an LDS read; x and y each cross HBM once.
-// Pass C (column slabs): y += s*(up hops).  Column tiles of TW columns are
+// Pass C (column slabs): y_up = s*(up hops).  Column tiles of TW columns are
 // assigned to XCDs round-robin and each XCD sweeps every row of a tile before
 // moving on, so the tile's slab of x (num_up x TW doubles) stays in that
 // XCD's L2 while ~17 source rows per output row are read from it.
+// Pass R (rows): a workgroup stages R whole rows of x in LDS and writes
+//   y = s*(diag + dn hops) + y_up - c*xprev   (+ block partial of <y, x>)
+// so every dn-hop gather is an LDS read and y_up is read back as whole
+// contiguous rows.  With the uniform-hopping tables pass C runs first and
+// writes y; otherwise pass R runs first and pass C adds to y.
 // ---------------------------------------------------------------------------
 struct Hv2Args {
   int64_t nu, nd;
@@ -157,6 +159,10 @@ struct Hv2Args {
   const int32_t* dn_hwords;     // [2 * ceil(nd / 64)] positive and total words per chunk
   const int32_t* up_tgt;        // [nu][up_w]
   const uint32_t* up_cnt_pn;    // [nu]
+  const uint32_t* up_pair;      // [nu][up_pair_stride] pair lists (see Tables::pair16)
+  int32_t up_pair_stride;
+  const uint32_t* up_quad;      // [nu][up_quad_stride] quad lists (see Tables::quad16)
+  int32_t up_quad_stride;
   const double* hop;
   int32_t nb;
   int32_t u_mode;
@@ -340,7 +346,7 @@ __device__ __forceinline__ void gather_list(const uint32_t* __restrict__ tab, in
 // a zero pad; y = s*(diag*x + t*(P - N)) - c*xprev with P (N) the sum of the
 // dn-hop sources of sign +1 (-1), gathered through the sign-split offset
 // tables.  The loop over a 64-column chunk runs that chunk's largest count.
-template <int RB>
+template <int RB, bool ADDUP>
 __global__ __launch_bounds__(kRowThreads) void k_hv_rows_off(Hv2Args a, int rows_per_block,
                                                              const double* __restrict__ x,
                                                              double* __restrict__ y, HvEpilogue e) {
@@ -369,10 +375,12 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_off(Hv2Args a, int rows
     const uint32_t sdn = a.st_dn[ids];
     const double ddn = a.d_dn[ids];
     for (int rg = 0; rg < R; rg += RB) {
-      double xp[RB];
+      double xp[RB], yu[RB];
 #pragma unroll
-      for (int r = 0; r < RB; ++r)
+      for (int r = 0; r < RB; ++r) {
         xp[r] = (e.xprev && act && rg + r < R) ? e.xprev[(r0 + rg + r) * nd + id] : 0.0;
+        yu[r] = (ADDUP && act && rg + r < R) ? __builtin_nontemporal_load(y + (r0 + rg + r) * nd + id) : 0.0;
+      }
       double accp[RB], accn[RB];
 #pragma unroll
       for (int r = 0; r < RB; ++r) accp[r] = accn[r] = 0.0;
@@ -384,7 +392,7 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_off(Hv2Args a, int rows
           const int64_t iu = r0 + rg + r;
           const double xo = lds[(int64_t)(rg + r) * stride + id];
           const double dg = a.d_up[iu] + ddn + u_term2(a, a.st_up[iu] & sdn);
-          const double v = sx * (dg * xo + t * (accp[r] - accn[r])) - cp * xp[r];
+          const double v = sx * (dg * xo + t * (accp[r] - accn[r])) + yu[r] - cp * xp[r];
           __builtin_nontemporal_store(v, y + iu * nd + id);
           dot += v * xo;
         }
@@ -440,6 +448,7 @@ __device__ __forceinline__ double sum_words_n(int nw, const char* tb, uint32_t n
 // block: the row is staged in LDS with 32 zero pads after it (indices nd ..
 // nd+31), and the dn hops gather through the bank-ordered half-width table,
 // whose slots each read 32 distinct bank pairs per half-wave.
+template <bool ADDUP>
 __global__ __launch_bounds__(kRowThreads) void k_hv_rows_half(Hv2Args a, int rows_per_block,
                                                               const double* __restrict__ x,
                                                               double* __restrict__ y, HvEpilogue e) {
@@ -471,13 +480,14 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_half(Hv2Args a, int row
     const int nw = uniform_load(a.dn_hwords + 2 * chunk + 1);
     const int64_t gi = iu * nd + ids;
     const double xp = e.xprev ? e.xprev[gi] : 0.0;
+    const double yu = ADDUP ? __builtin_nontemporal_load(y + gi) : 0.0;
     const uint32_t sdn = a.st_dn[ids];
     const double ddn = a.d_dn[ids];
     const double pn = sum_words_n(nw, (const char*)a.dn_h16, (uint32_t)nd * 4u, (uint32_t)ids * 4u, nwp, row);
     if (act) {
       const double xo = row[id];
       const double dg = dup + ddn + u_term2(a, sup & sdn);
-      const double v = sx * (dg * xo + t * pn) - cp * xp;
+      const double v = sx * (dg * xo + t * pn) + yu - cp * xp;
       __builtin_nontemporal_store(v, y + gi);
       dot += v * xo;
     }
@@ -650,6 +660,322 @@ __global__ __launch_bounds__(256) void k_hv_cols_split(Hv2Args a, const double* 
   }
 }
 
+// Buffer resource over `bytes` bytes from p: a load at a byte offset at or
+// past `bytes` returns zero without touching memory (the pad entries of the
+// pair lists address one row past the vector).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ double buffer_load_f64(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
+// Lane constants of a column tile in the pair-list column pass.
+struct PairLane {
+  uint32_t laneoff;  // byte offset of the lane's column within a row
+  uint32_t hsh;      // 0 or 16: which entry of a pair the lane's half-wave takes
+  int64_t colc;      // the lane's column (clamped into the sector)
+  bool wr;           // the lane writes y
+};
+
+template <int TW>
+__device__ __forceinline__ PairLane pair_lane(int64_t tile, int64_t nd) {
+  const int lane = threadIdx.x & 63;
+  const int half = TW == 32 ? lane >> 5 : 0;
+  const int64_t col = tile * TW + (lane % TW);
+  const bool act = col < nd;
+  PairLane L;
+  L.colc = act ? col : tile * TW;
+  L.laneoff = (uint32_t)L.colc * 8u;
+  L.hsh = (uint32_t)half * 16u;
+  L.wr = act && half == 0;
+  return L;
+}
+
+// One row's pair list: header and all kPairMaxPairs words (the table carries
+// kPairMaxPairs words of slack after its last row), read through the scalar
+// cache.
+struct PairList {
+  uint32_t hdr;
+  uint32_t w[kPairMaxPairs];
+};
+
+__device__ __forceinline__ void load_pair_list(const Hv2Args& a, int64_t iu, PairList& l) {
+  const uint32_t* wl = a.up_pair + iu * a.up_pair_stride;
+  l.hdr = uniform_load(wl);
+#pragma unroll
+  for (int m = 0; m < kPairMaxPairs; ++m) l.w[m] = uniform_load(wl + 1 + m);
+}
+
+// sum_pairs with a hook between issuing the loads and the first add: the
+// caller's prefetch of the next row's list goes there, after this row's list
+// has been consumed, so waiting for it never delays this row.
+template <int TW, int NPR, typename Pre>
+__device__ __forceinline__ double sum_pairs_pre(const uint32_t (&w)[kPairMaxPairs], int pp, uint32_t hsh,
+                                                uint32_t nd8, uint32_t laneoff, __amdgpu_buffer_rsrc_t rx,
+                                                Pre&& pre) {
+  constexpr int NL = TW == 32 ? NPR : 2 * NPR;
+  double g[NL];
+#pragma unroll
+  for (int m = 0; m < NPR; ++m) {
+    if (TW == 32) {
+      g[m] = buffer_load_f64(rx, __umul24((w[m] >> hsh) & 0xffffu, nd8) + laneoff);
+    } else {
+      g[2 * m] = buffer_load_f64(rx, __umul24(w[m] & 0xffffu, nd8) + laneoff);
+      g[2 * m + 1] = buffer_load_f64(rx, __umul24(w[m] >> 16, nd8) + laneoff);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  pre();
+  __builtin_amdgcn_sched_barrier(0);
+  double acc = 0.0;
+#pragma unroll
+  for (int m = 0; m < NPR; ++m) {
+    const double sg = m < pp ? 1.0 : -1.0;
+    if (TW == 32) {
+      acc = fma(sg, g[m], acc);
+    } else {
+      acc = fma(sg, g[2 * m], acc);
+      acc = fma(sg, g[2 * m + 1], acc);
+    }
+  }
+  return acc;
+}
+
+template <int TW, typename Pre>
+__device__ __forceinline__ double sum_pairs_pre_n(int npr, const uint32_t (&w)[kPairMaxPairs], int pp,
+                                                  uint32_t hsh, uint32_t nd8, uint32_t laneoff,
+                                                  __amdgpu_buffer_rsrc_t rx, Pre&& pre) {
+  switch (npr) {
+#define ED_PRE_CASE(n) \
+  case n: return sum_pairs_pre<TW, n>(w, pp, hsh, nd8, laneoff, rx, pre);
+    ED_PRE_CASE(1) ED_PRE_CASE(2) ED_PRE_CASE(3) ED_PRE_CASE(4) ED_PRE_CASE(5) ED_PRE_CASE(6)
+    ED_PRE_CASE(7) ED_PRE_CASE(8) ED_PRE_CASE(9) ED_PRE_CASE(10) ED_PRE_CASE(11) ED_PRE_CASE(12)
+    ED_PRE_CASE(13) ED_PRE_CASE(14) ED_PRE_CASE(15) ED_PRE_CASE(16)
+#undef ED_PRE_CASE
+    default: pre(); return 0.0;
+  }
+}
+
+// Column pass from the pair lists, after the row pass (y += up hops).  Block b
+// runs on XCD b & 7 and takes row chunk j % nrc of tile (j / nrc) * 8 +
+// (b & 7), j = b >> 3; the dispatcher issues blocks in order, so the blocks
+// resident on an XCD cover a window of consecutive row chunks of (mostly)
+// one tile and that tile's slab (num_up x TW doubles) is what its L2 holds.
+// The four waves of a block take the chunk's rows in turn, each prefetching
+// its next row's list while the current row's gathers are in flight.
+// Requires (num_up + 1) * num_dn * 8 < 2^32 (32-bit buffer offsets, the pad
+// row included).
+template <int TW>
+__global__ __launch_bounds__(256) void k_hv_cols_pipe(Hv2Args a, const double* __restrict__ x,
+                                                      double* __restrict__ y, HvEpilogue e, int64_t ntiles,
+                                                      int64_t nrc, int rch, double* partial) {
+  __shared__ double red[4];
+  const int64_t b = blockIdx.x;
+  const int64_t j = b >> 3;
+  const int64_t tile = (j / nrc) * 8 + (b & 7);
+  const int64_t row0 = (j % nrc) * rch;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  double dot = 0.0;
+  if (tile < ntiles) {
+    const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(x, (uint32_t)(a.nu * a.nd * 8));
+    const double st = (e.sx ? *e.sx : 1.0) * a.hop[0];
+    const PairLane L = pair_lane<TW>(tile, a.nd);
+    const uint32_t nd8 = (uint32_t)a.nd * 8u;
+    const int64_t rend = row0 + rch < a.nu ? row0 + rch : a.nu;
+    int64_t iu = row0 + wave;
+    PairList nxt;
+    if (iu < rend) load_pair_list(a, iu, nxt);
+    // the previous row's y store is issued after this row's gathers, so
+    // reusing its data register never waits for it while loads are pending
+    double pend = 0.0;
+    double* pend_at = nullptr;
+    while (iu < rend) {
+      const PairList cur = nxt;
+      const int64_t inx = iu + 4;
+      const int pp = (int)(cur.hdr & 0xffu), npr = (int)((cur.hdr >> 8) & 0xffu);
+      const int64_t gw = iu * a.nd + L.colc;
+      double yold = 0.0, xown = 0.0;
+      if (L.wr) {
+        yold = __builtin_nontemporal_load(y + gw);
+        if (partial) xown = x[gw];
+      }
+      double acc = sum_pairs_pre_n<TW>(npr, cur.w, pp, L.hsh, nd8, L.laneoff, rx, [&]() {
+        if (L.wr && pend_at) __builtin_nontemporal_store(pend, pend_at);
+        if (inx < rend) load_pair_list(a, inx, nxt);
+      });
+      if (TW == 32) acc += __shfl_xor(acc, 32, 64);
+      const double up = st * acc;
+      pend = yold + up;
+      pend_at = y + gw;
+      dot += up * xown;
+      iu = inx;
+    }
+    if (L.wr && pend_at) __builtin_nontemporal_store(pend, pend_at);
+  }
+  if (partial) {
+    const double s = block_sum_256(dot, red);
+    if (threadIdx.x == 0) partial[b] = s;
+  }
+}
+
+// Column pass run before the row pass: y = s * t * (up hops) over each tile,
+// written without reading y (the row pass then adds the diagonal, the dn
+// hops and the epilogue from whole contiguous rows, and forms the dot).  Grid,
+// row chunks and list prefetch as in k_hv_cols_pipe.
+template <int TW>
+__global__ __launch_bounds__(256) void k_hv_cols_first(Hv2Args a, const double* __restrict__ x,
+                                                       double* __restrict__ y, HvEpilogue e, int64_t ntiles,
+                                                       int64_t nrc, int rch, double*) {
+  const int64_t b = blockIdx.x;
+  const int64_t j = b >> 3;
+  const int64_t tile = (j / nrc) * 8 + (b & 7);
+  const int64_t row0 = (j % nrc) * rch;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  if (tile >= ntiles) return;
+  const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(x, (uint32_t)(a.nu * a.nd * 8));
+  const double st = (e.sx ? *e.sx : 1.0) * a.hop[0];
+  const PairLane L = pair_lane<TW>(tile, a.nd);
+  const uint32_t nd8 = (uint32_t)a.nd * 8u;
+  const int64_t rend = row0 + rch < a.nu ? row0 + rch : a.nu;
+  int64_t iu = row0 + wave;
+  PairList nxt;
+  if (iu < rend) load_pair_list(a, iu, nxt);
+  // the previous row's store is issued after this row's gathers (see
+  // k_hv_cols_pipe)
+  double pend = 0.0;
+  double* pend_at = nullptr;
+  while (iu < rend) {
+    const PairList cur = nxt;
+    const int64_t inx = iu + 4;
+    const int pp = (int)(cur.hdr & 0xffu), npr = (int)((cur.hdr >> 8) & 0xffu);
+    double acc = sum_pairs_pre_n<TW>(npr, cur.w, pp, L.hsh, nd8, L.laneoff, rx, [&]() {
+      if (L.wr && pend_at) __builtin_nontemporal_store(pend, pend_at);
+      if (inx < rend) load_pair_list(a, inx, nxt);
+    });
+    if (TW == 32) acc += __shfl_xor(acc, 32, 64);
+    pend = st * acc;
+    pend_at = y + iu * a.nd + L.colc;
+    iu = inx;
+  }
+  if (L.wr && pend_at) __builtin_nontemporal_store(pend, pend_at);
+}
+
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// One row's quad list: header and kQuadWords - 1 words (the table carries
+// that much slack after its last row), through the scalar cache.
+struct QuadList {
+  uint32_t hdr;
+  uint32_t w[2 * kQuadMaxQuads];
+};
+
+__device__ __forceinline__ void load_quad_list(const Hv2Args& a, int64_t iu, QuadList& l) {
+  const uint32_t* wl = a.up_quad + iu * a.up_quad_stride;
+  l.hdr = uniform_load(wl);
+#pragma unroll
+  for (int m = 0; m < 2 * kQuadMaxQuads; ++m) l.w[m] = uniform_load(wl + 1 + m);
+}
+
+// Up hops of one row over a 32-column tile from its quad list (NQ quads, the
+// first qp of sign +1): quarter-wave q takes entry q of every quad and each
+// lane reads two adjacent columns (16 bytes), so one load serves four hops.
+// pre() runs between issuing the loads and the first add.
+// psel: byte selector of the lane's quarter (entry q of a quad is bytes
+// 2q, 2q + 1 of the word pair, zero-extended).
+template <int NQ, typename Pre>
+__device__ __forceinline__ void sum_quads(const uint32_t (&w)[2 * kQuadMaxQuads], int qp, uint32_t psel,
+                                          uint32_t nd8, uint32_t laneoff, __amdgpu_buffer_rsrc_t rx, double& s0,
+                                          double& s1, Pre&& pre) {
+  f64x2 g[NQ];
+#pragma unroll
+  for (int m = 0; m < NQ; ++m) {
+    const uint32_t ent = __builtin_amdgcn_perm(w[2 * m + 1], w[2 * m], psel);
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rx, __umul24(ent, nd8) + laneoff, 0, 0);
+    g[m] = __builtin_bit_cast(f64x2, v);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  pre();
+  __builtin_amdgcn_sched_barrier(0);
+  double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+  for (int m = 0; m < NQ; ++m) {
+    const double sg = m < qp ? 1.0 : -1.0;
+    a0 = fma(sg, g[m].x, a0);
+    a1 = fma(sg, g[m].y, a1);
+  }
+  s0 = a0;
+  s1 = a1;
+}
+
+template <typename Pre>
+__device__ __forceinline__ void sum_quads_n(int nq, const uint32_t (&w)[2 * kQuadMaxQuads], int qp, uint32_t psel,
+                                            uint32_t nd8, uint32_t laneoff, __amdgpu_buffer_rsrc_t rx, double& s0,
+                                            double& s1, Pre&& pre) {
+  switch (nq) {
+#define ED_QUAD_CASE(n) \
+  case n: sum_quads<n>(w, qp, psel, nd8, laneoff, rx, s0, s1, pre); return;
+    ED_QUAD_CASE(1) ED_QUAD_CASE(2) ED_QUAD_CASE(3) ED_QUAD_CASE(4) ED_QUAD_CASE(5) ED_QUAD_CASE(6)
+    ED_QUAD_CASE(7) ED_QUAD_CASE(8) ED_QUAD_CASE(9) ED_QUAD_CASE(10) ED_QUAD_CASE(11) ED_QUAD_CASE(12)
+#undef ED_QUAD_CASE
+    default: pre(); s0 = 0.0; s1 = 0.0; return;
+  }
+}
+
+// Column pass run before the row pass (as k_hv_cols_first) from the quad
+// lists: 32-column tiles, 16-byte loads, four hops per load.  Requires an
+// even num_dn and (num_up + 1) * num_dn * 8 < 2^32.
+__global__ __launch_bounds__(256) void k_hv_cols_quad(Hv2Args a, const double* __restrict__ x,
+                                                      double* __restrict__ y, HvEpilogue e, int64_t ntiles,
+                                                      int64_t nrc, int rch, double*) {
+  const int64_t b = blockIdx.x;
+  const int64_t j = b >> 3;
+  const int64_t tile = (j / nrc) * 8 + (b & 7);
+  const int64_t row0 = (j % nrc) * rch;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  if (tile >= ntiles) return;
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 4;
+  // bytes 2q and 2q + 1 of {w[2m + 1], w[2m]} into the low half, zeros above
+  const uint32_t psel = 0x0c0c0000u | ((uint32_t)(2 * q + 1) << 8) | (uint32_t)(2 * q);
+  const int64_t nd = a.nd;
+  const int64_t col = tile * 32 + 2 * (lane & 15);
+  const bool act = col < nd;
+  const int64_t colc = act ? col : nd - 2;
+  const uint32_t laneoff = (uint32_t)colc * 8u;
+  const bool wr = act && q == 0;
+  const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(x, (uint32_t)(a.nu * nd * 8));
+  const double st = (e.sx ? *e.sx : 1.0) * a.hop[0];
+  const uint32_t nd8 = (uint32_t)nd * 8u;
+  const int64_t rend = row0 + rch < a.nu ? row0 + rch : a.nu;
+  int64_t iu = row0 + wave;
+  QuadList nxt;
+  if (iu < rend) load_quad_list(a, iu, nxt);
+  // the previous row's store is issued after this row's gathers (see
+  // k_hv_cols_pipe)
+  f64x2 pend = {0.0, 0.0};
+  f64x2* pend_at = nullptr;
+  while (iu < rend) {
+    const QuadList cur = nxt;
+    const int64_t inx = iu + 4;
+    const int qp = (int)(cur.hdr & 0xffu), nq = (int)((cur.hdr >> 8) & 0xffu);
+    double s0, s1;
+    sum_quads_n(nq, cur.w, qp, psel, nd8, laneoff, rx, s0, s1, [&]() {
+      if (wr && pend_at) __builtin_nontemporal_store(pend, pend_at);
+      if (inx < rend) load_quad_list(a, inx, nxt);
+    });
+    s0 += __shfl_xor(s0, 16, 64);
+    s1 += __shfl_xor(s1, 16, 64);
+    s0 += __shfl_xor(s0, 32, 64);
+    s1 += __shfl_xor(s1, 32, 64);
+    pend = f64x2{st * s0, st * s1};
+    pend_at = (f64x2*)(y + iu * nd + colc);
+    iu = inx;
+  }
+  if (wr && pend_at) __builtin_nontemporal_store(pend, pend_at);
+}
+
 static int env_int(const char* name, int dflt) {
   const char* v = getenv(name);
   return v && *v ? atoi(v) : dflt;
@@ -677,6 +1003,10 @@ static Hv2Args make_hv2_args(const ed_op* op) {
   a.dn_hwords = op->dn.hwords_chunk;
   a.up_tgt = op->up.tgt_sorted;
   a.up_cnt_pn = op->up.cnt_pn;
+  a.up_pair = op->up.pair16;
+  a.up_pair_stride = op->up.pair_stride;
+  a.up_quad = op->up.quad16;
+  a.up_quad_stride = op->up.quad_stride;
   a.hop = op->hop_dev;
   a.nb = op->num_bonds;
   a.u_mode = op->u_mode;
@@ -698,6 +1028,7 @@ struct TwoPassPlan {
   int64_t ntiles, nrc, grid_c;
   RowKernel rk;
   ColKernel ck;
+  bool col_first;  // column pass writes y first, the row pass adds to it
 };
 
 // Smallest compiled hop-list length >= m (0 if none fits).
@@ -786,10 +1117,10 @@ static TwoPassPlan plan_two_pass(const ed_op* op) {
   if (half) {
     p.rows_per_block = R = 1;
     p.lds_bytes = sizeof(double) * (size_t)(op->num_dn + 32 + kRowThreads / 64);
-    p.rk = k_hv_rows_half;
+    p.rk = k_hv_rows_half<false>;
   } else if (split) {
     p.lds_bytes = row_bytes * R;
-    p.rk = R == 1 ? k_hv_rows_off<1> : k_hv_rows_off<4>;
+    p.rk = R == 1 ? k_hv_rows_off<1, false> : k_hv_rows_off<4, false>;
   } else {
     p.lds_bytes = row_bytes * R + 8 * 128;
     p.rk = uni ? k_hv_rows<true> : k_hv_rows<false>;
@@ -818,6 +1149,44 @@ static TwoPassPlan plan_two_pass(const ed_op* op) {
   p.ntiles = (op->num_dn + p.tw - 1) / p.tw;
   p.nrc = (op->num_up + p.rch - 1) / p.rch;
   p.grid_c = 8 * ((p.ntiles + 7) / 8) * p.nrc;
+  // column pass from the pair or quad lists (32-bit buffer offsets with the
+  // pad row included); CMPY_AMD_HV_PAIR=0 turns it off, CMPY_AMD_HV_PTW picks
+  // 32- (default) or 64-column tiles for the pair lists, CMPY_AMD_HV_PRCH
+  // rows per block, CMPY_AMD_HV_FIRST=0 runs it after the row pass
+  if (split && op->up.pair16 && (op->num_up + 1) * op->num_dn * 8 < (1ll << 32) &&
+      env_int("CMPY_AMD_HV_PAIR", 1) != 0) {
+    p.tw = env_int("CMPY_AMD_HV_PTW", 32) == 64 ? 64 : 32;
+    p.rch = std::max(4, std::min(256, env_int("CMPY_AMD_HV_PRCH", 8))) / 4 * 4;
+    p.ntiles = (op->num_dn + p.tw - 1) / p.tw;
+    if (env_int("CMPY_AMD_HV_FIRST", 1) != 0) {
+      // column pass first, writing y; the row pass adds to it.  The quad
+      // lists with 16-byte loads where num_dn is even (CMPY_AMD_HV_QUAD=0
+      // turns them off)
+      p.col_first = true;
+      if (op->up.quad16 && op->num_dn % 2 == 0 && env_int("CMPY_AMD_HV_QUAD", 1) != 0) {
+        p.tw = 32;
+        p.ntiles = (op->num_dn + 31) / 32;
+        p.ck = k_hv_cols_quad;
+      } else {
+        p.ck = p.tw == 64 ? k_hv_cols_first<64> : k_hv_cols_first<32>;
+      }
+      p.nrc = (op->num_up + p.rch - 1) / p.rch;
+      if (p.rk == k_hv_rows_half<false>)
+        p.rk = k_hv_rows_half<true>;
+      else if (p.rk == k_hv_rows_off<1, false>)
+        p.rk = k_hv_rows_off<1, true>;
+      else
+        p.rk = k_hv_rows_off<4, true>;
+      if (!ensure_row_lds(p.rk, p.lds_bytes)) {
+        p.ok = false;
+        return p;
+      }
+    } else {
+      p.ck = p.tw == 64 ? k_hv_cols_pipe<64> : k_hv_cols_pipe<32>;
+      p.nrc = (op->num_up + p.rch - 1) / p.rch;
+    }
+    p.grid_c = 8 * ((p.ntiles + 7) / 8) * p.nrc;
+  }
   p.ok = p.grid_c + p.grid_r < (1ll << 31);
   return p;
 }
@@ -825,6 +1194,15 @@ static TwoPassPlan plan_two_pass(const ed_op* op) {
 static int launch_two_pass(const ed_op* op, const TwoPassPlan& p, const double* x, double* y,
                            const HvEpilogue& epi, hipStream_t s, int* num_partials) {
   const Hv2Args a = make_hv2_args(op);
+  if (p.col_first) {
+    hipLaunchKernelGGL(p.ck, dim3((unsigned)p.grid_c), dim3(256), 0, s, a, x, y, epi, p.ntiles, p.nrc, p.rch,
+                       nullptr);
+    hipLaunchKernelGGL(p.rk, dim3((unsigned)p.grid_r), dim3(kRowThreads), p.lds_bytes, s, a,
+                       p.rows_per_block, x, y, epi);
+    ED_HIP(hipGetLastError());
+    if (num_partials) *num_partials = (int)p.grid_r;
+    return ED_OK;
+  }
   hipLaunchKernelGGL(p.rk, dim3((unsigned)p.grid_r), dim3(kRowThreads), p.lds_bytes, s, a,
                      p.rows_per_block, x, y, epi);
   double* part_c = epi.partial ? epi.partial + p.grid_r : nullptr;
@@ -847,7 +1225,7 @@ static void grid_shape(const ed_op* op, int rb, int64_t* nrb, int64_t* nct) {
 
 int matvec_num_partials(const ed_op* op) {
   const TwoPassPlan p = plan_two_pass(op);
-  if (p.ok) return (int)(p.grid_r + p.grid_c);
+  if (p.ok) return (int)(p.col_first ? p.grid_r : p.grid_r + p.grid_c);
   int64_t nrb, nct;
   grid_shape(op, pick_rb(op), &nrb, &nct);
   return (int)(nrb * nct);

@@ -216,22 +216,27 @@ def test_large_sector_matvec_both_kernels(gpu, geom, monkeypatch):
     y_ref = ref.matvec(x)
     scale = np.abs(y_ref).max()
     # fused kernel; two-pass with sign-split tables (uniform hopping only) or
-    # general packed tables, column tiles of 16/32 columns
-    combos = (("0", "1", "16", "1"), ("1", "1", "16", "1"), ("1", "1", "32", "1"), ("1", "1", "64", "1"),
-              ("1", "1", "32", "0"), ("1", "0", "32", "1"), ("1", "0", "16", "1"))
-    for mode, split, tw, half in combos:
-        monkeypatch.setenv("CMPY_AMD_HV_MODE", mode)
-        monkeypatch.setenv("CMPY_AMD_HV_SPLIT", split)
-        monkeypatch.setenv("CMPY_AMD_HV_TW", tw)
-        monkeypatch.setenv("CMPY_AMD_HV_HALF", half)
+    # general packed tables, column tiles of 16/32/64 columns; the pair-list
+    # column passes (32- and 64-column tiles): run first and written into y
+    # (the default; quad lists with 16-byte loads, or pair lists), or after the
+    # row pass
+    base = dict(MODE="1", SPLIT="1", TW="32", HALF="1", PAIR="1", PTW="32", FIRST="1", QUAD="1")
+    combos = (dict(MODE="0"), dict(PAIR="0", TW="16"), dict(PAIR="0"), dict(PAIR="0", TW="64"), {},
+              dict(QUAD="0"), dict(QUAD="0", PTW="64"),
+              dict(HALF="0"), dict(FIRST="0"), dict(FIRST="0", PTW="64"), dict(SPLIT="0"),
+              dict(SPLIT="0", TW="16"))
+    for over in combos:
+        case = {**base, **over}
+        for k, v in case.items():
+            monkeypatch.setenv("CMPY_AMD_HV_" + k, v)
         op = SectorHamiltonOperator(ns, nu, nd, bonds, hops, eps, u, w)
         y = op.matvec(x)
-        assert np.abs(y - y_ref).max() <= 1e-13 * scale, (mode, split, tw, half)
+        assert np.abs(y - y_ref).max() <= 1e-13 * scale, case
         hops_u = np.full(len(bonds), 0.7)
         op_u = SectorHamiltonOperator(ns, nu, nd, bonds, hops_u, eps, u, w)
         ref_u = O.SectorOperator(ns, nu, nd, bonds, hops_u, eps, u, w)
         y_u = ref_u.matvec(x)
-        assert np.abs(op_u.matvec(x) - y_u).max() <= 1e-13 * np.abs(y_u).max(), (mode, split, tw, half)
+        assert np.abs(op_u.matvec(x) - y_u).max() <= 1e-13 * np.abs(y_u).max(), case
 
 
 def test_hv_4x4_matches_cpu_twin(gpu):

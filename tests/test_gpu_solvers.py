@@ -83,6 +83,23 @@ def test_cfg2_groundstate_12site_chain(gpu, u):
         assert abs(gs.energy - w[0]) <= E0_RTOL * abs(w[0])
 
 
+def test_groundstate_4x4_one_up_six_dn(gpu):
+    """4x4 torus, sector (1, 6), D = 128,128: the Lanczos epilogue through the
+    two-pass sweep with the half-width row tables (8008 dn states)."""
+    from cmpy_amd import HubbardModel, groundstate, square_neighbors
+    import scipy.sparse.linalg as sla
+
+    m = HubbardModel(16, square_neighbors(4, 4), inter=3.0, eps=0.0, hop=1.0)
+    gs, st = groundstate(m, 1, 6, tol=1e-10, reorth=False, return_stats=True, seed=3)
+    assert st.converged and st.residual <= 1e-10, st
+    ref = O.hubbard_operator(16, 1, 6, O.square_bonds(4, 4), 3.0, 0.0, 1.0)
+    psi = gs.state / np.linalg.norm(gs.state)
+    r = ref.matvec(psi) - gs.energy * psi
+    assert np.linalg.norm(r) <= 2e-10
+    w, _ = sla.eigsh(ref.csr(), k=1, which="SA", tol=1e-12, v0=psi)
+    assert abs(gs.energy - w[0]) <= E0_RTOL * abs(w[0])
+
+
 def test_lanczos_coeffs_match_reference_recurrence(gpu):
     from cmpy_amd import lanczos_coeffs
 
