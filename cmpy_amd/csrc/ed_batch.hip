@@ -59,6 +59,7 @@ struct Config {
   int64_t vec_cap_wave, vec_cap_block;     // vector capacity (doubles)
   // algorithm
   int32_t max_iter, cf_steps, check_every, sign_width;
+  int32_t staged;    // block mode: row-staged H.v sweep (LDS) where it applies
   double tol, cf_tol;
   uint64_t seed;
   const double* wn;  // [nw] Matsubara frequencies (imaginary parts)
@@ -83,10 +84,51 @@ struct Config {
 // ---------------------------------------------------------------------------
 // Group: one wave (wave mode) or the whole workgroup (block mode).
 // ---------------------------------------------------------------------------
+using lds_f64 = __attribute__((address_space(3))) double;
+using lds_u32 = __attribute__((address_space(3))) uint32_t;
+using lds_i32 = __attribute__((address_space(3))) int32_t;
+
+constexpr int kLdsChunks = 64;  // 64-state chunks of the dn sector the LDS cache covers
+// LDS sizes: staged rows (two buffers of R rows of at most 64 * 16 / R
+// columns: 2048 doubles), cached dn and up hop slots (12 bytes each) and
+// up-state counts
+constexpr int kLdsRows = 2048;
+constexpr int kLdsDnSolve = 4096;   // solve kernel: 48 KiB
+constexpr int kLdsUpSolve = 4096;   // 48 KiB
+constexpr int kLdsUpRowsSolve = 4096;
+constexpr int kLdsDnBwd = 2048;     // backward kernel (its LDS also holds a W chunk): 24 KiB
+constexpr int kLdsUpBwd = 2048;     // 24 KiB
+constexpr int kLdsUpRowsBwd = 2048;
+
+// LDS working set of a block-mode group (null in wave mode): rows of u staged
+// for the dn-hop gathers, and the dn sector's hop lists transposed
+// ([slot][state]), slots past a state's count holding target 0 and value 0.
+// meta[0] = states cached (0: none), meta[1] = slots per state, meta[2 + c] =
+// largest count in 64-state chunk c.
+struct BlockLds {
+  lds_f64* rows;
+  int rows_cap;  // doubles
+  lds_u32* tgt;
+  lds_f64* val;
+  int dn_cap;    // slots
+  lds_i32* meta;
+  // the up sector's hop lists, row-major ([state][slot], counts in ucnt);
+  // umeta[0] = states cached (0: none), umeta[1] = slots per state,
+  // umeta[2..3] = the cached list's address (which sector it is)
+  lds_u32* utgt;
+  lds_f64* uval;
+  lds_i32* ucnt;
+  int u_cap;     // slots
+  int u_rows;    // states
+  lds_i32* umeta;
+};
+
 template <bool BLOCK>
 struct Group {
   int rank, size;
   double* red;  // LDS: 2 * 16 doubles (block mode)
+  BlockLds lds{};
+  bool has_lds = false;
 
   __device__ void sync() const {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -219,6 +261,172 @@ __device__ int build_list(const Group<BLOCK>& g, const ModelParams& p, int ns, i
   return count;
 }
 
+// Fills the group's LDS dn-list cache for the sector Dn when it fits (else
+// marks it empty).  Every member must call it.
+__device__ __noinline__ void cache_dn(const Group<true>& g, const SpinList& Dn) {
+  if (!g.has_lds) return;
+  const BlockLds& L = g.lds;
+  int mc = 0;
+  for (int i = g.rank; i < Dn.n; i += g.size) mc = mc > Dn.cnt[i] ? mc : Dn.cnt[i];
+  const int slots = (int)g.max((double)mc);
+  const int nch = (Dn.n + 63) >> 6;
+  const bool fit = Dn.n > 0 && nch <= kLdsChunks && (int64_t)Dn.n * (slots > 0 ? slots : 1) <= L.dn_cap;
+  if (fit) {
+    for (int64_t k = g.rank; k < (int64_t)slots * Dn.n; k += g.size) {
+      const int e = (int)(k / Dn.n), i = (int)(k - (int64_t)e * Dn.n);
+      const bool on = e < Dn.cnt[i];
+      L.tgt[k] = on ? (uint32_t)Dn.tgt[i * kMaxBath + e] & 0xffffu : 0u;
+      L.val[k] = on ? Dn.val[i * kMaxBath + e] : 0.0;
+    }
+    for (int c = g.rank; c < nch; c += g.size) {
+      int m = 0;
+      for (int i = c * 64; i < Dn.n && i < c * 64 + 64; ++i) m = m > Dn.cnt[i] ? m : Dn.cnt[i];
+      L.meta[2 + c] = m;
+    }
+  }
+  if (g.rank == 0) {
+    L.meta[0] = fit ? Dn.n : 0;
+    L.meta[1] = slots;
+  }
+  g.sync();
+}
+
+// Fills the group's LDS up-list cache with the sector U when it fits (else
+// marks it empty).  Every member must call it.
+__device__ __noinline__ void cache_up(const Group<true>& g, const SpinList& U) {
+  const BlockLds& L = g.lds;
+  int mc = 0;
+  for (int i = g.rank; i < U.n; i += g.size) mc = mc > U.cnt[i] ? mc : U.cnt[i];
+  const int slots = (int)g.max((double)mc);
+  const bool fit = U.n > 0 && U.n <= L.u_rows && (int64_t)U.n * (slots > 0 ? slots : 1) <= L.u_cap;
+  if (fit) {
+    for (int i = g.rank; i < U.n; i += g.size) L.ucnt[i] = U.cnt[i];
+    for (int64_t k = g.rank; k < (int64_t)slots * U.n; k += g.size) {
+      const int i = (int)(k / slots), e = (int)(k - (int64_t)i * slots);
+      const bool on = e < U.cnt[i];
+      L.utgt[k] = on ? (uint32_t)U.tgt[i * kMaxBath + e] & 0xffffu : 0u;
+      L.uval[k] = on ? U.val[i * kMaxBath + e] : 0.0;
+    }
+  }
+  if (g.rank == 0) {
+    const uint64_t a = (uint64_t)(uintptr_t)U.cnt;
+    L.umeta[0] = fit ? U.n : 0;
+    L.umeta[1] = slots;
+    L.umeta[2] = (int32_t)(uint32_t)a;
+    L.umeta[3] = (int32_t)(uint32_t)(a >> 32);
+  }
+  g.sync();
+}
+
+__device__ __forceinline__ bool up_cached(const Group<true>& g, const SpinList& U) {
+  const uint64_t a = (uint64_t)(uintptr_t)U.cnt;
+  return g.lds.umeta[0] == U.n && g.lds.umeta[2] == (int32_t)(uint32_t)a &&
+         g.lds.umeta[3] == (int32_t)(uint32_t)(a >> 32);
+}
+
+// Wave-uniform copy of a value every lane reads from the same address (the
+// spin lists are written by this kernel, so they are read with vector loads,
+// not through the scalar cache).
+__device__ __forceinline__ int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double rfl_d(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+constexpr int kUpPre = 6;  // up hops whose loads start before the staging barrier
+
+// Block-mode sweep 1 of lanczos_step with staged rows, for dn sectors of at
+// most 64 * 16 states: w = sj H u - bj sp uprev, returning this thread's part
+// of <w, u>.  Each step stages R = 16 / (64-column chunks per row) rows of u
+// in LDS (one barrier per step, two row buffers); wave wv handles chunk wv % chunks of row wv / chunks: diagonal, up
+// hops from the row's list (uniform over the wave, each hop a contiguous
+// 64-column piece of another row, the first kUpPre loads issued before the
+// staging barrier) and dn hops gathered from the staged row through the
+// cached lists.
+__device__ double hv_rows_block(const Group<true>& g, const ModelParams& p, const SpinList& U, const SpinList& Dn,
+                                const double* __restrict__ u, const double* __restrict__ uprev,
+                                double* __restrict__ w, double sj, double sp, double bj) {
+  const int nd = Dn.n, nu = U.n;
+  const int nch = (nd + 63) >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int R = (int)(blockDim.x >> 6) / nch;
+  const int rr = wave / nch;
+  const int id = (wave - rr * nch) * 64 + (int)(threadIdx.x & 63);
+  const bool act = id < nd;
+  const int idc = act ? id : nd - 1;
+  const double bsp = bj * sp;
+  // per-column constants of the dn sector
+  const double ddn = Dn.dg[idc];
+  const uint32_t sdn = Dn.st[idc];
+  const int cm = rfl(g.lds.meta[2 + (wave - rr * nch)]);
+  const int us = g.lds.umeta[1];  // up slots per state
+  double dot = 0.0;
+  int buf = 0;
+  for (int r0 = 0; r0 < nu; r0 += R) {
+    // two row buffers: a wave writes the other one next step only after
+    // every wave has passed this step's barrier, i.e. finished the previous
+    // step's gathers from it
+    lds_f64* const row = g.lds.rows + (buf * R + rr) * nd;
+    buf ^= 1;
+    const int iu = r0 + rr;
+    const bool on = rr < R && iu < nu;  // uniform over the wave
+    double gu[kUpPre];
+    double vu[kUpPre];
+    int cu = 0;
+    double xp = 0.0, dgu = 0.0;
+    uint32_t stu = 0;
+    const int ro = iu * nd;
+    if (on) {
+      cu = rfl(g.lds.ucnt[iu]);
+#pragma unroll
+      for (int e = 0; e < kUpPre; ++e) {
+        if (e < cu) {
+          const int t = rfl((int)g.lds.utgt[iu * us + e]);
+          vu[e] = rfl_d(g.lds.uval[iu * us + e]);
+          gu[e] = u[t * nd + idc];
+        }
+      }
+      if (bsp != 0.0) xp = uprev[ro + idc];
+      if (act) row[id] = u[ro + id];
+      dgu = U.dg[iu];
+      stu = (uint32_t)rfl((int)U.st[iu]);
+    }
+    __syncthreads();
+    if (on) {
+      double acc = 0.0;
+#pragma unroll
+      for (int e = 0; e < kUpPre; ++e)
+        if (e < cu) acc = fma(vu[e], gu[e], acc);
+      // the rest in batches of kUpPre loads in flight
+      for (int e0 = kUpPre; e0 < cu; e0 += kUpPre) {
+#pragma unroll
+        for (int e = 0; e < kUpPre; ++e) {
+          if (e0 + e < cu) {
+            const int t = rfl((int)g.lds.utgt[iu * us + e0 + e]);
+            vu[e] = rfl_d(g.lds.uval[iu * us + e0 + e]);
+            gu[e] = u[t * nd + idc];
+          }
+        }
+#pragma unroll
+        for (int e = 0; e < kUpPre; ++e)
+          if (e0 + e < cu) acc = fma(vu[e], gu[e], acc);
+      }
+      for (int e = 0; e < cm; ++e) acc = fma(g.lds.val[e * nd + idc], row[g.lds.tgt[e * nd + idc]], acc);
+      const double xk = row[idc];
+      acc = fma(ddn + dgu + ((stu & sdn & 1u) ? p.u : 0.0), xk, acc);
+      const double wk = sj * acc - bsp * xp;
+      if (act) {
+        w[ro + id] = wk;
+        dot += wk * xk;
+      }
+    }
+  }
+  __syncthreads();  // the rows are free for the next sweep
+  return dot;
+}
+
 // One Lanczos step on sector (U, Dn), lazily scaled: v_j = sj * u, v_{j-1} =
 // sp * uprev.  w = sj H u - bj * sp * uprev; then (if alpha < 0 on entry
 // coefficient unknown) a = <w, v_j>; w -= a v_j; returns |w|^2 through *nrm2.
@@ -231,7 +439,19 @@ __device__ void lanczos_step(const Group<BLOCK>& g, const ModelParams& p, const 
   const int64_t D = (int64_t)U.n * nd;
   double dot = 0.0;
   // sweep 1: w = sj H u - bj sp uprev, dot <w, u>
-  {
+  bool staged = false;
+  if constexpr (BLOCK) {
+    const int nch = (nd + 63) >> 6;
+    const int nwv = (int)(blockDim.x >> 6);
+    if (g.has_lds && g.lds.meta[0] == nd && nd > 0 && nch <= nwv && 2ll * (nwv / nch) * nd <= g.lds.rows_cap &&
+        D < (1ll << 31)) {
+      if (up_cached(g, U)) {
+        dot = hv_rows_block(g, p, U, Dn, u, uprev, w, sj, sp, bj);
+        staged = true;
+      }
+    }
+  }
+  if (!staged) {
     int64_t k = g.rank;
     int iu = (int)(k / (nd > 0 ? nd : 1)), id = (int)(k - (int64_t)iu * nd);
     for (; k < D; k += g.size) {
@@ -436,6 +656,12 @@ __device__ void solve_model(const Group<BLOCK>& g, const Config& cf, int mi, con
   g.sync();
   const SpinList U = sl.list(1, n_0);
   const SpinList Dn = sl.list(3, nd);
+  if constexpr (BLOCK) {
+    if (g.has_lds) {
+      cache_dn(g, Dn);
+      cache_up(g, U);
+    }
+  }
   const int64_t D = (int64_t)n_0 * nd;
   double* const alpha = sl.alpha();
   double* const beta = sl.beta();
@@ -563,6 +789,9 @@ __device__ void solve_model(const Group<BLOCK>& g, const Config& cf, int mi, con
     nn = g.sum(nn);
     g.sync();
     if (!(nn > 0.0)) continue;
+    if constexpr (BLOCK) {
+      if (g.has_lds) cache_up(g, T);
+    }
     if (part == 0) {
       np2 = nn;
       double* qt = cf.keep ? cf.store_q + cf.off_q[2 * mi] : nullptr;
@@ -641,7 +870,20 @@ __global__ __launch_bounds__(kBlock) void k_batch_solve(Config cf) {
       solve_model<false>(g, cf, __builtin_amdgcn_readfirstlane(cf.queue_wave[q]), sl);
     }
   } else {
+    __shared__ double s_rows[kLdsRows];
+    __shared__ uint32_t s_tgt[kLdsDnSolve];
+    __shared__ double s_val[kLdsDnSolve];
+    __shared__ int32_t s_meta[2 + kLdsChunks];
+    __shared__ uint32_t s_utgt[kLdsUpSolve];
+    __shared__ double s_uval[kLdsUpSolve];
+    __shared__ int32_t s_ucnt[kLdsUpRowsSolve];
+    __shared__ int32_t s_umeta[4];
     Group<true> g{(int)threadIdx.x, (int)blockDim.x, red};
+    g.lds = BlockLds{(lds_f64*)s_rows, kLdsRows,          (lds_u32*)s_tgt,  (lds_f64*)s_val,
+                     kLdsDnSolve,      (lds_i32*)s_meta,  (lds_u32*)s_utgt, (lds_f64*)s_uval,
+                     (lds_i32*)s_ucnt, kLdsUpSolve,       kLdsUpRowsSolve,  (lds_i32*)s_umeta};
+    if (threadIdx.x == 0) s_umeta[0] = 0;
+    g.has_lds = cf.staged != 0;
     const int64_t slot = (int64_t)blockIdx.x - cf.wave_blocks;
     const SlotRef sl{cf.ws_block + slot * cf.slot_bytes_block, cf.list_cap_block, cf.vec_cap_block, cf.max_iter,
                      cf.cf_steps};
@@ -681,6 +923,7 @@ struct BwdConfig {
   int32_t lcap;
   int64_t vcap;
   int32_t kcap, nw, sign_width, max_bath;
+  int32_t staged;     // row-staged H.v sweep (LDS) where it applies
   const double* wn;
   const double* gre;  // [num_models][nw] dL/dRe G
   const double* gim;  // [num_models][nw] dL/dIm G
@@ -763,6 +1006,10 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
   g.sync();
   const SpinList U0 = sl.list(1, n_0);
   const SpinList Dn = sl.list(3, nd);
+  if (g.has_lds) {
+    cache_dn(g, Dn);
+    cache_up(g, U0);
+  }
   const int64_t D = (int64_t)n_0 * nd;
   const int kcap = bc.kcap, nw = bc.nw;
   const double* co = bc.store_coef + (int64_t)mi * (4 * kcap + 8);
@@ -1003,7 +1250,20 @@ __global__ __launch_bounds__(kBlock) void k_batch_backward(BwdConfig bc) {
   __shared__ double red[32];
   __shared__ int popped;
   __shared__ double wl[kBwdMaxSteps * kPc];  // 64 KiB: a column chunk of Re W
+  __shared__ double s_rows[kLdsRows];
+  __shared__ uint32_t s_tgt[kLdsDnBwd];
+  __shared__ double s_val[kLdsDnBwd];
+  __shared__ int32_t s_meta[2 + kLdsChunks];
+  __shared__ uint32_t s_utgt[kLdsUpBwd];
+  __shared__ double s_uval[kLdsUpBwd];
+  __shared__ int32_t s_ucnt[kLdsUpRowsBwd];
+  __shared__ int32_t s_umeta[4];
   Group<true> g{(int)threadIdx.x, (int)blockDim.x, red};
+  g.lds = BlockLds{(lds_f64*)s_rows, kLdsRows,          (lds_u32*)s_tgt,  (lds_f64*)s_val,
+                   kLdsDnBwd,        (lds_i32*)s_meta,  (lds_u32*)s_utgt, (lds_f64*)s_uval,
+                   (lds_i32*)s_ucnt, kLdsUpBwd,         kLdsUpRowsBwd,    (lds_i32*)s_umeta};
+  if (threadIdx.x == 0) s_umeta[0] = 0;
+  g.has_lds = bc.staged != 0;
   const BwdSlot bs{SlotRef{bc.ws + (int64_t)blockIdx.x * bc.slot_bytes, bc.lcap, bc.vcap, 0, bc.kcap}};
   while (true) {
     if (threadIdx.x == 0) popped = atomicAdd(bc.counter, 1);
@@ -1027,6 +1287,11 @@ __global__ void k_reset_counters(int32_t* c) {
 // ===========================================================================
 using namespace ed;
 using namespace ed::batch;
+
+static int env_flag(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return v && *v ? atoi(v) : dflt;
+}
 
 struct ed_batch {
   int device = 0;
@@ -1162,6 +1427,7 @@ static int configure(ed_batch* b, int64_t wave_max_dim, int grid_blocks, int wav
   c.cf_steps = b->cf_steps;
   c.check_every = b->check_every;
   c.sign_width = b->sign_width;
+  c.staged = env_flag("CMPY_AMD_BATCH_STAGED", 1);
   c.tol = b->tol;
   c.cf_tol = b->cf_tol;
   c.seed = b->seed;
@@ -1559,6 +1825,7 @@ extern "C" int ed_batch_backward(ed_batch* b, const double* dl_dre, const double
   bc.kcap = b->kcap;
   bc.nw = nw;
   bc.sign_width = b->sign_width;
+  bc.staged = env_flag("CMPY_AMD_BATCH_STAGED", 1);
   bc.max_bath = b->max_bath;
   bc.wn = b->d_wn;
   bc.gre = b->d_gre;
