@@ -138,6 +138,36 @@ def test_batch_fermion_signs(gpu):
         _check(res, e0, g, i)
 
 
+def test_block_mode_staged_sweep_matches_plain_sweep(gpu, monkeypatch):
+    """Block mode with the row-staged H.v sweep (LDS-cached hop lists) against
+    the plain element-wise sweep, at the batch's largest size (9 bath sites,
+    D = 63,504) and a small one, both sign modes; the oracle pins the small
+    ones."""
+    from cmpy_amd.batch import matsubara
+
+    p = _params(21, 6, (4, 9))
+    wn = matsubara(50.0, 64)
+    out = {}
+    for staged in ("1", "0"):
+        monkeypatch.setenv("CMPY_AMD_BATCH_STAGED", staged)
+        for mode in ("reference", "fermion"):
+            b = _batch(sign_mode=mode, **p)
+            b.config(-1, 16, 0)
+            out[staged, mode] = b.solve(wn)
+    for mode in ("reference", "fermion"):
+        a, c = out["1", mode], out["0", mode]
+        assert np.all(a.residual <= 1e-9)
+        np.testing.assert_allclose(a.e0, c.e0, rtol=E0_RTOL, atol=0)
+        for i in range(len(a.e0)):
+            assert np.abs(a.g[i] - c.g[i]).max() <= G_TOL * np.abs(c.g[i]).max()
+    b = _batch(**p)
+    for i in range(b.num_models):
+        if len(p["eps_bath"][i]) <= 5:
+            e0, g = _oracle_model(p["u"][i], p["eps_imp"][i], p["eps_bath"][i], p["v"][i], p["mu"][i],
+                                  int(b.n_up[i]), int(b.n_dn[i]), wn)
+            _check(out["1", "reference"], e0, g, i)
+
+
 def test_batch_invalid_arguments(gpu):
     from cmpy_amd import _lib as L
 
