@@ -1032,57 +1032,124 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
     const double* qt = bc.store_q + bc.off_q[2 * mi + part];
     double* Uc = bs.U(0, nw);   // c' then x
     double* Ur = bs.U(1, nw);   // rhs'
-    // (a) u_n = sigma |phi| (sigma z_n + e0 - T)^-1 e1, complex Thomas, one thread per frequency
+    // (a) u_n = sigma |phi| (sigma z_n + e0 - T)^-1 e1, complex Thomas, one
+    // thread per frequency; U is stored frequency-fastest ([i][n], complex),
+    // so a step's stores and loads are coalesced over the frequencies.  The
+    // denominators are O(|z| + |T|), far from overflow, so 1/den is taken as
+    // conj(den) / |den|^2 (one division per step)
+    auto ux = [&](int n, int i) { return 2 * ((int64_t)i * nw + n); };
     for (int n = g.rank; n < nw; n += g.size) {
       const cplx zs = {e0, sigma * bc.wn[n]};
       cplx cp = {0.0, 0.0}, rp = {0.0, 0.0};
       for (int i = 0; i < m; ++i) {
         const double offl = i > 0 ? -b[i - 1] : 0.0;
         const cplx den = {zs.re - a[i] - offl * cp.re, zs.im - offl * cp.im};
-        const cplx ci = i < m - 1 ? cdiv({-b[i], 0.0}, den) : cplx{0.0, 0.0};
-        const cplx ri = cdiv({(i == 0 ? 1.0 : 0.0) - offl * rp.re, -offl * rp.im}, den);
-        Uc[2 * ((int64_t)n * kcap + i)] = ci.re;
-        Uc[2 * ((int64_t)n * kcap + i) + 1] = ci.im;
-        Ur[2 * ((int64_t)n * kcap + i)] = ri.re;
-        Ur[2 * ((int64_t)n * kcap + i) + 1] = ri.im;
+        const double s2 = 1.0 / (den.re * den.re + den.im * den.im);
+        const cplx inv = {den.re * s2, -den.im * s2};
+        const cplx ci = i < m - 1 ? cplx{-b[i] * inv.re, -b[i] * inv.im} : cplx{0.0, 0.0};
+        const cplx ri = cmul({(i == 0 ? 1.0 : 0.0) - offl * rp.re, -offl * rp.im}, inv);
+        Uc[ux(n, i)] = ci.re;
+        Uc[ux(n, i) + 1] = ci.im;
+        Ur[ux(n, i)] = ri.re;
+        Ur[ux(n, i) + 1] = ri.im;
         cp = ci;
         rp = ri;
       }
       cplx x = rp;
-      Uc[2 * ((int64_t)n * kcap + m - 1)] = sigma * nphi * x.re;
-      Uc[2 * ((int64_t)n * kcap + m - 1) + 1] = sigma * nphi * x.im;
+      Uc[ux(n, m - 1)] = sigma * nphi * x.re;
+      Uc[ux(n, m - 1) + 1] = sigma * nphi * x.im;
       for (int i = m - 2; i >= 0; --i) {
-        const cplx ci = {Uc[2 * ((int64_t)n * kcap + i)], Uc[2 * ((int64_t)n * kcap + i) + 1]};
-        const cplx ri = {Ur[2 * ((int64_t)n * kcap + i)], Ur[2 * ((int64_t)n * kcap + i) + 1]};
+        const cplx ci = {Uc[ux(n, i)], Uc[ux(n, i) + 1]};
+        const cplx ri = {Ur[ux(n, i)], Ur[ux(n, i) + 1]};
         const cplx t = cmul(ci, x);
         x = {ri.re - t.re, ri.im - t.im};
-        Uc[2 * ((int64_t)n * kcap + i)] = sigma * nphi * x.re;
-        Uc[2 * ((int64_t)n * kcap + i) + 1] = sigma * nphi * x.im;
+        Uc[ux(n, i)] = sigma * nphi * x.re;
+        Uc[ux(n, i) + 1] = sigma * nphi * x.im;
       }
     }
     g.sync();
-    // (b) Re W and ubar
+    // (b) Re W = Re(U diag(gre - i gim) U^T), upper triangle in 4 x 4 tiles
+    // (one per thread and pass), frequencies staged through LDS nc at a
+    // time as v_in = (gre_n - i gim_n) u_in and u_jn; then ubar
     double* W = bs.ReW(nw);
     double* ub = bs.ubar(nw);
     const double* gre = bc.gre + (int64_t)mi * nw;
     const double* gim = bc.gim + (int64_t)mi * nw;
-    for (int t = g.rank; t < m * m; t += g.size) {
-      const int i = t / m, j = t - i * m;
-      if (j < i) continue;
-      double acc = 0.0;
-      for (int n = 0; n < nw; ++n) {
-        const cplx ui = {Uc[2 * ((int64_t)n * kcap + i)], Uc[2 * ((int64_t)n * kcap + i) + 1]};
-        const cplx uj = {Uc[2 * ((int64_t)n * kcap + j)], Uc[2 * ((int64_t)n * kcap + j) + 1]};
-        const cplx q = cmul(ui, uj);
-        acc += gre[n] * q.re + gim[n] * q.im;  // Re((gre - i gim) q)
+    {
+      const int nbk = (m + 3) / 4;
+      const int ntile = nbk * (nbk + 1) / 2;
+      const int ncap = (kBwdMaxSteps * kPc) / (4 * m);
+      const int nc = ncap < nw ? (ncap > 0 ? ncap : 1) : nw;
+      double* vs = wl;              // [nc][m] complex
+      double* us = wl + 2 * nc * m; // [nc][m] complex
+      for (int t0 = 0; t0 < ntile; t0 += g.size) {
+        const int t = t0 + g.rank;
+        const bool on = t < ntile;
+        // tile (bi, bj), bi <= bj, row-major over the upper triangle
+        int bi = 0, bj = 0;
+        if (on) {
+          const double q = 2.0 * nbk + 1.0;
+          bi = (int)((q - sqrt(q * q - 8.0 * t)) * 0.5);
+          while (bi > 0 && bi * nbk - bi * (bi - 1) / 2 > t) --bi;
+          while ((bi + 1) * nbk - (bi + 1) * bi / 2 <= t) ++bi;
+          bj = bi + (t - (bi * nbk - bi * (bi - 1) / 2));
+        }
+        double acc[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc[k] = 0.0;
+        for (int n0 = 0; n0 < nw; n0 += nc) {
+          const int nn = nw - n0 < nc ? nw - n0 : nc;
+          __syncthreads();
+          for (int k = g.rank; k < nn * m; k += g.size) {
+            const int i = k / nn, nl = k - i * nn;
+            const int n = n0 + nl;
+            const int o = nl * m + i;
+            const double ur = Uc[ux(n, i)], ui = Uc[ux(n, i) + 1];
+            const double cr = gre[n], ci = -gim[n];
+            vs[2 * o] = cr * ur - ci * ui;
+            vs[2 * o + 1] = cr * ui + ci * ur;
+            us[2 * o] = ur;
+            us[2 * o + 1] = ui;
+          }
+          __syncthreads();
+          if (on) {
+            for (int nl = 0; nl < nn; ++nl) {
+              double vr[4], vi[4], wr[4], wi[4];
+#pragma unroll
+              for (int a = 0; a < 4; ++a) {
+                const int i = bi * 4 + a < m ? bi * 4 + a : m - 1;
+                const int j = bj * 4 + a < m ? bj * 4 + a : m - 1;
+                vr[a] = vs[2 * (nl * m + i)];
+                vi[a] = vs[2 * (nl * m + i) + 1];
+                wr[a] = us[2 * (nl * m + j)];
+                wi[a] = us[2 * (nl * m + j) + 1];
+              }
+#pragma unroll
+              for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[a * 4 + c] = fma(vr[a], wr[c], fma(-vi[a], wi[c], acc[a * 4 + c]));
+            }
+          }
+        }
+        if (on) {
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              const int i = bi * 4 + a, j = bj * 4 + c;
+              if (i < m && j < m && i <= j) {
+                W[(int64_t)i * kcap + j] = acc[a * 4 + c];
+                W[(int64_t)j * kcap + i] = acc[a * 4 + c];
+              }
+            }
+        }
       }
-      W[(int64_t)i * kcap + j] = acc;
-      W[(int64_t)j * kcap + i] = acc;
+      __syncthreads();
     }
     for (int i = g.rank; i < m; i += g.size) {
       double re = 0.0, im = 0.0;
       for (int n = 0; n < nw; ++n) {
-        const double ur = Uc[2 * ((int64_t)n * kcap + i)], ui = Uc[2 * ((int64_t)n * kcap + i) + 1];
+        const double ur = Uc[ux(n, i)], ui = Uc[ux(n, i) + 1];
         re += gre[n] * ur + gim[n] * ui;
         im += gre[n] * ui - gim[n] * ur;
       }
@@ -1104,25 +1171,70 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
       Xr[r] = xr;
       Xi[r] = xi;
     }
-    // columns of W in chunks of kPc staged in LDS (read by every thread)
-    for (int j0 = 0; j0 < m; j0 += kPc) {
-      const int jn = m - j0 < kPc ? m - j0 : kPc;
-      for (int t = g.rank; t < m * kPc; t += g.size) {
-        const int i = t / kPc, jj = t - i * kPc;
-        wl[t] = jj < jn ? W[(int64_t)i * kcap + j0 + jj] : 0.0;
-      }
-      __syncthreads();
-      for (int64_t r = g.rank; r < Dt; r += g.size) {
-        const double* q = qt + r * kcap;
-        double acc[kPc];
+    // P = Qt Re W with FP64 matrix cores (16 x 16 x 4): wave (wr, wc) of the
+    // workgroup accumulates the 32 x 32 block of P at rows rb = R0 + 32 wr,
+    // columns 32 wc, as 2 x 2 tiles; Re W is staged through LDS kc rows at a
+    // time.  Operand layout of the instruction: lane l holds A[l % 16][l / 16]
+    // and B[l / 16][l % 16], and its four results are D[l / 16 + 4 r][l % 16].
+    // Krylov indices at or past m contribute zeros (Qt holds no data there).
+    {
+      typedef double f64x4 __attribute__((ext_vector_type(4)));
+      const int ncb = (m + 31) / 32;
+      const int wpr = (int)(g.size >> 6) / ncb;
+      const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+      const int lane = threadIdx.x & 63;
+      const int wr = wave / ncb, wc = wave - wr * ncb;
+      const bool won = wr < wpr;
+      const int ldw = 32 * ncb;
+      const int kc = ((kBwdMaxSteps * kPc) / ldw) / 4 * 4;
+      for (int64_t R0 = 0; R0 < Dt; R0 += 32 * (int64_t)wpr) {
+        const int64_t rb = R0 + 32 * (int64_t)wr;
+        f64x4 acc[2][2];
 #pragma unroll
-        for (int jj = 0; jj < kPc; ++jj) acc[jj] = 0.0;
-        for (int i = 0; i < m; ++i) {
-          const double qi = q[i];
+        for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-          for (int jj = 0; jj < kPc; ++jj) acc[jj] += qi * wl[i * kPc + jj];
+          for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = f64x4{0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < m; k0 += kc) {
+          const int kn = m - k0 < kc ? m - k0 : kc;
+          __syncthreads();
+          for (int t = g.rank; t < kc * ldw; t += g.size) {
+            const int kk = t / ldw, c = t - kk * ldw;
+            wl[t] = (kk < kn && c < m) ? W[(int64_t)(k0 + kk) * kcap + c] : 0.0;
+          }
+          __syncthreads();
+          if (won && rb < Dt) {
+            for (int ks = 0; ks < kn; ks += 4) {
+              const int kk = ks + lane / 16;
+              const bool kon = kk < kn;
+              double av[2], bv[2];
+#pragma unroll
+              for (int ti = 0; ti < 2; ++ti) {
+                const int64_t row = rb + 16 * ti + (lane % 16);
+                const int64_t rowc = row < Dt ? row : Dt - 1;
+                av[ti] = kon ? qt[rowc * kcap + k0 + kk] : 0.0;
+              }
+#pragma unroll
+              for (int tj = 0; tj < 2; ++tj) bv[tj] = wl[kk * ldw + 32 * wc + 16 * tj + (lane % 16)];
+#pragma unroll
+              for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj)
+                  acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], bv[tj], acc[ti][tj], 0, 0, 0);
+            }
+          }
         }
-        for (int jj = 0; jj < jn; ++jj) P[r * kcap + j0 + jj] = acc[jj];
+        if (won) {
+#pragma unroll
+          for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                const int64_t row = rb + 16 * ti + lane / 16 + 4 * r;
+                const int col = 32 * wc + 16 * tj + (lane % 16);
+                if (row < Dt && col < m) P[row * kcap + col] = acc[ti][tj][r];
+              }
+        }
       }
       __syncthreads();
     }
