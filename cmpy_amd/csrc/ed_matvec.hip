@@ -498,6 +498,78 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_half(Hv2Args a, int row
   }
 }
 
+// k_hv_rows_half with the row's y_up (ADDUP) and x_prev values loaded
+// together with the staging of x, held in registers (one per column chunk of
+// the thread, indexed by the uniform chunk counter), so no chunk waits for
+// HBM: only table words (L2) and LDS gathers remain on each chunk's path.
+// For num_dn <= kRowHeld * kRowThreads.
+constexpr int kRowHeld = 16;
+
+template <bool ADDUP>
+__global__ __launch_bounds__(kRowThreads) void k_hv_rows_held(Hv2Args a, int rows_per_block,
+                                                              const double* __restrict__ x,
+                                                              double* __restrict__ y, HvEpilogue e) {
+  extern __shared__ double dyn_lds[];
+  (void)dyn_lds;
+  lds_f64* row = (lds_f64*)(uintptr_t)0;
+  const int64_t nd = a.nd;
+  lds_f64* red = row + nd + 32;
+  const int64_t iu = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const double cp = e.cprev ? *e.cprev : 0.0;
+  // held[k] = y_up - c x_prev of the thread's chunk k (both loads issued
+  // before the staging loads, so one HBM round trip covers all of them)
+  double held[kRowHeld], ph[kRowHeld];
+#pragma unroll
+  for (int k = 0; k < kRowHeld; ++k) {
+    const int64_t c0 = (int64_t)wave * 64 + (int64_t)k * kRowThreads;
+    held[k] = 0.0;
+    ph[k] = 0.0;
+    if (c0 < nd) {
+      const int64_t id = c0 + lane;
+      const int64_t gi = iu * nd + (id < nd ? id : c0);
+      if (ADDUP) held[k] = __builtin_nontemporal_load(y + gi);
+      if (e.xprev) ph[k] = e.xprev[gi];
+    }
+  }
+  stage_rows<kRowThreads, 16>(row, x + iu * nd, nd);
+  if (threadIdx.x < 32) row[nd + threadIdx.x] = 0.0;
+#pragma unroll
+  for (int k = 0; k < kRowHeld; ++k) held[k] = fma(-cp, ph[k], held[k]);
+  __syncthreads();
+  const double sx = e.sx ? *e.sx : 1.0;
+  const double t = a.hop[0];
+  const double dup = a.d_up[iu];
+  const uint32_t sup = a.st_up[iu];
+  double dot = 0.0;
+  for (int k = 0; k < kRowHeld; ++k) {
+    const int64_t c0 = (int64_t)wave * 64 + (int64_t)k * kRowThreads;
+    if (c0 >= nd) break;
+    const int64_t id = c0 + lane;
+    const bool act = id < nd;
+    const int64_t ids = act ? id : c0;
+    const int64_t chunk = c0 >> 6;
+    const int nwp = uniform_load(a.dn_hwords + 2 * chunk);
+    const int nw = uniform_load(a.dn_hwords + 2 * chunk + 1);
+    const int64_t gi = iu * nd + ids;
+    const uint32_t sdn = a.st_dn[ids];
+    const double ddn = a.d_dn[ids];
+    const double pn = sum_words_n(nw, (const char*)a.dn_h16, (uint32_t)nd * 4u, (uint32_t)ids * 4u, nwp, row);
+    if (act) {
+      const double xo = row[id];
+      const double dg = dup + ddn + u_term2(a, sup & sdn);
+      const double v = sx * (dg * xo + t * pn) + held[k];
+      __builtin_nontemporal_store(v, y + gi);
+      dot += v * xo;
+    }
+  }
+  if (e.partial) {
+    const double s = block_sum_n<kRowThreads>(dot, red);
+    if (threadIdx.x == 0) e.partial[blockIdx.x] = s;
+  }
+}
+
 constexpr int kColMaxRows = 64;
 
 // Column pass, general tables: the block stages its rows' up-hop entries in
@@ -1172,7 +1244,9 @@ static TwoPassPlan plan_two_pass(const ed_op* op) {
       }
       p.nrc = (op->num_up + p.rch - 1) / p.rch;
       if (p.rk == k_hv_rows_half<false>)
-        p.rk = k_hv_rows_half<true>;
+        p.rk = op->num_dn <= (int64_t)kRowHeld * kRowThreads && env_int("CMPY_AMD_HV_HELD", 1) != 0
+                   ? k_hv_rows_held<true>
+                   : k_hv_rows_half<true>;
       else if (p.rk == k_hv_rows_off<1, false>)
         p.rk = k_hv_rows_off<1, true>;
       else
