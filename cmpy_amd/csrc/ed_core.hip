@@ -309,6 +309,7 @@ static void free_tables(Tables& t) {
   if (t.off_pos) (void)hipFree(t.off_pos);
   if (t.off_neg) (void)hipFree(t.off_neg);
   if (t.h16) (void)hipFree(t.h16);
+  if (t.h16q) (void)hipFree(t.h16q);
   if (t.hwords_chunk) (void)hipFree(t.hwords_chunk);
   if (t.cmax_pos) (void)hipFree(t.cmax_pos);
   if (t.cmax_neg) (void)hipFree(t.cmax_neg);
@@ -433,6 +434,23 @@ static int build_half_tables(Tables& tab, int64_t count, int width, hipStream_t 
       out[(size_t)w * count + i] = v[0] | (v[1] << 16);
     }
   }
+  // lane-contiguous copy: chunk c, group g, lane l, word 4g + j
+  tab.hgroups = (maxw + 3) / 4;
+  std::vector<uint32_t> outq((size_t)nchunk * tab.hgroups * 256);
+  for (int64_t c = 0; c < nchunk; ++c)
+    for (int g = 0; g < tab.hgroups; ++g)
+      for (int l = 0; l < 64; ++l) {
+        const int64_t i = c * 64 + l;
+        const uint32_t lane_pad = (uint32_t)(count + ((l % L) - count % L + L) % L);
+        for (int j = 0; j < 4; ++j) {
+          const int w = 4 * g + j;
+          uint32_t v = lane_pad | (lane_pad << 16);
+          if (i < count && w < maxw) v = out[(size_t)w * count + i];
+          outq[(((size_t)c * tab.hgroups + g) * 64 + l) * 4 + j] = v;
+        }
+      }
+  ED_TRY(dev_alloc(&tab.h16q, (int64_t)outq.size()));
+  ED_HIP(hipMemcpyAsync(tab.h16q, outq.data(), sizeof(uint32_t) * outq.size(), hipMemcpyHostToDevice, s));
   ED_TRY(dev_alloc(&tab.h16, (int64_t)out.size()));
   ED_TRY(dev_alloc(&tab.hwords_chunk, 2 * nchunk));
   ED_HIP(hipMemcpyAsync(tab.h16, out.data(), sizeof(uint32_t) * out.size(), hipMemcpyHostToDevice, s));

@@ -145,6 +145,11 @@ struct Tables {
   uint32_t* h16 = nullptr;
   int32_t* hwords_chunk = nullptr;  // [2 * ceil(D / 64)]
   int32_t hwords = 0;
+  // The same words with each lane's words of a chunk contiguous, four per
+  // 16-byte group: chunk c's group g is 64 lanes x 4 words (1 KiB) at
+  // (c * hgroups + g) KiB; words past a chunk's count are the lane's pads.
+  uint32_t* h16q = nullptr;
+  int32_t hgroups = 0;  // groups per chunk, ceil(hwords / 4)
   // Sign-sorted target lists (uniform hopping, column pass): row i holds its
   // targets with sign +1 first, then those with sign -1; cnt_pn[i] = number
   // positive | number negative << 16.  [D][tgt_stride], tgt_stride = width

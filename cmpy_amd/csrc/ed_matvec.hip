@@ -156,6 +156,8 @@ struct Hv2Args {
   const int32_t* dn_cmax_pos;   // [ceil(nd / 64)]
   const int32_t* dn_cmax_neg;
   const uint32_t* dn_h16;       // [hwords][nd] half-width index pairs
+  const uint32_t* dn_h16q;      // the same, lane-contiguous groups of four words (see Tables::h16q)
+  int32_t dn_hgroups;
   const int32_t* dn_hwords;     // [2 * ceil(nd / 64)] positive and total words per chunk
   const int32_t* up_tgt;        // [nu][up_w]
   const uint32_t* up_cnt_pn;    // [nu]
@@ -498,6 +500,40 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_half(Hv2Args a, int row
   }
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// P - N over a chunk's words in the lane-contiguous layout: NG 16-byte loads
+// (four words each) per lane, all requested before the first gather; words
+// 0 .. nwp - 1 have sign +1, the rest -1 (pads gather zeros).
+template <int NG>
+__device__ __forceinline__ double sum_words_q(const u32x4* tq, int nwp, const lds_f64* row) {
+  u32x4 w[NG];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) w[g] = tq[g * 64];
+  double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+  for (int g = 0; g < NG; ++g)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double sg = 4 * g + j < nwp ? 1.0 : -1.0;
+      a0 = fma(sg, row[w[g][j] & 0xffffu], a0);
+      a1 = fma(sg, row[w[g][j] >> 16], a1);
+    }
+  return a0 + a1;
+}
+
+__device__ __forceinline__ double sum_words_q_n(int ng, const u32x4* tq, int nwp, const lds_f64* row) {
+  switch (ng) {
+    case 1: return sum_words_q<1>(tq, nwp, row);
+    case 2: return sum_words_q<2>(tq, nwp, row);
+    case 3: return sum_words_q<3>(tq, nwp, row);
+    case 4: return sum_words_q<4>(tq, nwp, row);
+    case 5: return sum_words_q<5>(tq, nwp, row);
+    case 6: return sum_words_q<6>(tq, nwp, row);
+    default: return 0.0;  // no words (kHalfMaxWords = 24 bounds ng by 6)
+  }
+}
+
 // k_hv_rows_half with the row's y_up (ADDUP) and x_prev values loaded
 // together with the staging of x, held in registers (one per column chunk of
 // the thread, indexed by the uniform chunk counter), so no chunk waits for
@@ -555,7 +591,9 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_held(Hv2Args a, int row
     const int64_t gi = iu * nd + ids;
     const uint32_t sdn = a.st_dn[ids];
     const double ddn = a.d_dn[ids];
-    const double pn = sum_words_n(nw, (const char*)a.dn_h16, (uint32_t)nd * 4u, (uint32_t)ids * 4u, nwp, row);
+    const double pn =
+        a.dn_h16q ? sum_words_q_n((nw + 3) >> 2, (const u32x4*)a.dn_h16q + (chunk * a.dn_hgroups) * 64 + lane, nwp, row)
+                  : sum_words_n(nw, (const char*)a.dn_h16, (uint32_t)nd * 4u, (uint32_t)ids * 4u, nwp, row);
     if (act) {
       const double xo = row[id];
       const double dg = dup + ddn + u_term2(a, sup & sdn);
@@ -1072,6 +1110,8 @@ static Hv2Args make_hv2_args(const ed_op* op) {
   a.dn_cmax_pos = op->dn.cmax_pos;
   a.dn_cmax_neg = op->dn.cmax_neg;
   a.dn_h16 = op->dn.h16;
+  a.dn_h16q = op->dn.h16q;
+  a.dn_hgroups = op->dn.hgroups;
   a.dn_hwords = op->dn.hwords_chunk;
   a.up_tgt = op->up.tgt_sorted;
   a.up_cnt_pn = op->up.cnt_pn;
