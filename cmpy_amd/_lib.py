@@ -119,6 +119,7 @@ SIGNATURES = {
     "ed_lanczos_coeffs_workspace": (C.c_int, [vp, i32, i32, P(i64)]),
     "ed_lanczos_coeffs": (C.c_int, [vp, vp, i32, i32, P(dbl), P(dbl), P(dbl), P(i32), vp, vp, i64, vp]),
     "ed_cf_eval": (C.c_int, [P(dbl), P(dbl), i32, dbl, dbl, i32, vp, i64, vp, i32, vp]),
+    "ed_tevo_eval": (C.c_int, [P(dbl), P(dbl), i32, dbl, dbl, i32, vp, i64, vp, i32, vp]),
     "ed_pole_sum": (C.c_int, [vp, vp, i64, vp, i64, vp, i32, vp]),
     "ed_lehmann_pairs": (C.c_int, [vp, vp, i64, vp, i64, dbl, dbl, vp, i64, vp, vp]),
     "ed_gemm_nt": (C.c_int, [vp, vp, vp, i64, i64, i64, vp]),

@@ -173,6 +173,48 @@ __global__ __launch_bounds__(256) void k_pole_sum(const double* __restrict__ w, 
   }
 }
 
+// Time-domain sum over the Gauss quadrature of a vector's spectral measure:
+// nodes theta[j] (Ritz values of the Lanczos tridiagonal), weights
+// vecs[j * ldc]^2 (first components of its eigenvectors, column-major):
+//   g[k] (+)= -i s norm2 sum_j w_j exp(-i s (theta_j - e0) t_k),  s = sign.
+constexpr int kTevoMaxSteps = 8192;  // the m x m eigenvectors take 512 MiB at the limit
+__global__ __launch_bounds__(256) void k_tevo_sum(const double* __restrict__ theta, const double* __restrict__ vecs,
+                                                  int m, int ldc, double norm2, double e0, int sign,
+                                                  const double* __restrict__ t, int64_t nt, double* __restrict__ g,
+                                                  int accumulate) {
+  __shared__ double sw[1024], sf[1024];
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double tk = k < nt ? t[k] : 0.0;
+  double re = 0.0, im = 0.0;
+  for (int base = 0; base < m; base += 1024) {
+    const int cnt = m - base < 1024 ? m - base : 1024;
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+      const double c0 = vecs[(int64_t)(base + i) * ldc];
+      sw[i] = c0 * c0;
+      sf[i] = sign * (theta[base + i] - e0);
+    }
+    __syncthreads();
+    for (int i = 0; i < cnt; ++i) {
+      double sn, cs;
+      sincos(sf[i] * tk, &sn, &cs);
+      re = fma(sw[i], cs, re);
+      im = fma(-sw[i], sn, im);
+    }
+  }
+  if (k < nt) {
+    // (-i s)(re + i im) = s im - i s re
+    const double gr = sign * im * norm2, gi = -sign * re * norm2;
+    if (accumulate) {
+      g[2 * k] += gr;
+      g[2 * k + 1] += gi;
+    } else {
+      g[2 * k] = gr;
+      g[2 * k + 1] = gi;
+    }
+  }
+}
+
 // Finite-T pair sum, stage 1: block (z tile, m group) -> partial[mg][z].
 constexpr int kPairGroup = 16;
 __global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ ov, const double* __restrict__ en,
@@ -346,6 +388,58 @@ int ed_cf_eval(const double* alpha, const double* beta, int32_t m, double norm2,
     hipLaunchKernelGGL(k_cf_eval, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, d, d + m, m, norm2, shift,
                        sign, z_dev, nz, g_dev, accumulate);
     if (hipGetLastError() != hipSuccess) rc = set_error(ED_ERR_HIP, "k_cf_eval launch failed");
+  }
+  if (hipStreamSynchronize(s) != hipSuccess && rc == ED_OK) rc = set_error(ED_ERR_HIP, "stream sync failed");
+  (void)hipFree(d);
+  return rc;
+}
+
+int ed_tevo_eval(const double* alpha, const double* beta, int32_t m, double norm2, double e0, int32_t sign,
+                 const double* t_dev, int64_t nt, double* g_dev, int32_t accumulate, void* stream) {
+  clear_error();
+  ED_REQUIRE(m >= 0 && m <= kTevoMaxSteps, "m=%d invalid (at most %d)", m, kTevoMaxSteps);
+  ED_REQUIRE(sign == 1 || sign == -1, "sign must be +1 or -1");
+  ED_REQUIRE(nt >= 0 && (nt == 0 || (t_dev && g_dev)), "NULL buffer");
+  ED_REQUIRE(m == 0 || alpha, "alpha is NULL");
+  ED_REQUIRE(m <= 1 || beta, "beta is NULL");
+  hipStream_t s = as_stream(stream);
+  if (nt == 0) return ED_OK;
+  if (m == 0 || norm2 == 0.0) {
+    if (!accumulate) ED_HIP(hipMemsetAsync(g_dev, 0, sizeof(double) * 2 * nt, s));
+    return ED_OK;
+  }
+  rocblas_handle h;
+  ED_TRY(get_handle(&h));
+  if (rocblas_set_stream(h, s) != rocblas_status_success) return set_error(ED_ERR_HIP, "rocblas_set_stream failed");
+  // one allocation: diagonal [m], off-diagonal [m], eigenvectors [m * m], info
+  const size_t nd = (size_t)m * (m + 2);
+  double* d = nullptr;
+  ED_HIP(hipMalloc(&d, sizeof(double) * nd + sizeof(rocblas_int)));
+  double* off = d + m;
+  double* vec = d + 2 * (size_t)m;
+  rocblas_int* info = (rocblas_int*)(d + nd);
+  std::vector<double> hd(2 * (size_t)m, 0.0);
+  for (int j = 0; j < m; ++j) hd[j] = alpha[j];
+  for (int j = 0; j + 1 < m; ++j) hd[m + j] = beta[j];
+  int rc = ED_OK;
+  rocblas_int hinfo = 0;
+  if (hipMemcpyAsync(d, hd.data(), sizeof(double) * 2 * m, hipMemcpyHostToDevice, s) != hipSuccess) {
+    rc = set_error(ED_ERR_HIP, "copying Lanczos coefficients failed");
+  } else {
+    const rocblas_status st = rocsolver_dstedc(h, rocblas_evect_tridiagonal, (rocblas_int)m, d, off, vec,
+                                               (rocblas_int)m, info);
+    if (st != rocblas_status_success) {
+      rc = set_error(ED_ERR_HIP, "rocsolver_dstedc failed with status %d", (int)st);
+    } else if (hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+               hipStreamSynchronize(s) != hipSuccess) {
+      rc = set_error(ED_ERR_HIP, "reading rocsolver info failed");
+    } else if (hinfo != 0) {
+      rc = set_error(ED_ERR_NUMERIC, "rocsolver_dstedc did not converge (info=%d)", (int)hinfo);
+    } else {
+      hipLaunchKernelGGL(k_tevo_sum, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, d, vec, m, m, norm2, e0,
+                         sign, t_dev, nt, g_dev, accumulate);
+      if (hipGetLastError() != hipSuccess) rc = set_error(ED_ERR_HIP, "k_tevo_sum launch failed");
+    }
   }
   if (hipStreamSynchronize(s) != hipSuccess && rc == ED_OK) rc = set_error(ED_ERR_HIP, "stream sync failed");
   (void)hipFree(d);

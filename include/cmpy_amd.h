@@ -209,6 +209,21 @@ int ed_cf_eval(const double* alpha, const double* beta, int32_t m, double norm2,
                int32_t sign, const double* z_dev, int64_t nz, double* g_dev, int32_t accumulate,
                void* stream);
 
+/* Real-time Green's function from Lanczos coefficients: replaces the
+ * expm_multiply propagation and overlap of gf_greater / gf_lesser
+ * (cmpy/exactdiag.py:248-301) by the Gauss quadrature of phi's spectral
+ * measure, exact for polynomials of degree 2m - 1 in H:
+ *   sign=+1: g[k] (+)= -i norm2 sum_j w_j exp(-i (theta_j - e0) t_k)
+ *            (greater part, phi = c+ psi0 in sector N+1)
+ *   sign=-1: g[k] (+)= +i norm2 sum_j w_j exp(+i (theta_j - e0) t_k)
+ *            (lesser part, phi = c psi0 in sector N-1)
+ * theta_j / w_j: eigenvalues / squared first eigenvector components of the
+ * tridiagonal (alpha[m], beta[m-1], host arrays, m <= 8192).
+ * t_dev: [nt] times; g_dev: [nt] complex interleaved.                      */
+int ed_tevo_eval(const double* alpha, const double* beta, int32_t m, double norm2, double e0,
+                 int32_t sign, const double* t_dev, int64_t nt, double* g_dev, int32_t accumulate,
+                 void* stream);
+
 /* Pole sum g[k] (+)= sum_m w_m / (z_k - p_m): the T = 0 Lehmann sum of
  * _accumulate_sum (cmpy/exactdiag.py:110-129) restricted to the ground state. */
 int ed_pole_sum(const double* w_dev, const double* p_dev, int64_t m, const double* z_dev,

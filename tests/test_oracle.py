@@ -281,6 +281,47 @@ def test_lehmann_sum_rule_and_tail():
     assert abs(g[0] * z[0] - 1.0) < 1e-6
 
 
+@pytest.mark.parametrize("u,v", [(2.0, 1.0), (5.0, 0.4)])
+def test_tevo_two_site_closed_form(u, v):
+    """g>(t) - g<(t) of the two-site SIAM equals -2i sum_k a_k cos(e_k t) with
+    the Lange poles; at t = 0 it is -i <{c, c+}> = -i."""
+    opn = O.siam_operator(1, 1, u, 0.0, [0.0], [v], u / 2)
+    e0, psi = O.groundstate(opn)
+    opp = O.siam_operator(2, 1, u, 0.0, [0.0], [v], u / 2)
+    opm = O.siam_operator(0, 1, u, 0.0, [0.0], [v], u / 2)
+    cd = O.apply_cdag(psi, 2, 1, 1, 0, O.UP)
+    cc = O.apply_c(psi, 2, 1, 1, 0, O.UP)
+    t = np.linspace(0.0, 30.0, 301)
+    g = O.tevo_part(opp, cd, e0, t, +1) - O.tevo_part(opm, cc, e0, t, -1)
+    ref = O.tevo_two_site_ref(t, u, v)
+    assert_allclose(g, ref, atol=1e-12)
+    assert abs(ref[0] + 1j) < 1e-14
+    # the hole part alone carries +i e^{+i e_k t}: a flipped sign fails this
+    a1, a2, e1, e2 = O.impurity_params_ref(u, v)
+    assert_allclose(O.tevo_part(opm, cc, e0, t, -1), 1j * (a1 * np.exp(1j * e1 * t) + a2 * np.exp(1j * e2 * t)),
+                    atol=1e-12)
+
+
+@pytest.mark.parametrize("width", [0, 6])
+def test_tevo_dense_matches_expm_multiply(width):
+    """The dense-eigh restatement equals the reference's own procedure
+    (scipy's expm_multiply interval algorithm + running phase product)."""
+    mk, op, e0, psi, cd, cc = _siam_g_pieces(7, width)
+    for sign, opx, phi in ((+1, mk(4, 3), cd), (-1, mk(2, 3), cc)):
+        times, ov = O.tevo_expm_multiply(opx, phi, e0, 0.0, 12.0, 121, sign)
+        assert_allclose(O.tevo_part(opx, phi, e0, times, sign), ov, atol=1e-9)
+
+
+def test_tevo_dense_matches_expm_multiply_hubbard():
+    ns, bonds = 6, O.chain_bonds(6, periodic=True)
+    op = O.hubbard_operator(ns, 3, 3, bonds, 3.0, 0.0, 1.0, mu=1.5)
+    e0, psi = O.groundstate(op)
+    opp = O.hubbard_operator(ns, 3, 4, bonds, 3.0, 0.0, 1.0, mu=1.5)
+    cd = O.apply_cdag(psi, ns, 3, 3, 2, O.DN, "jordan_wigner")
+    times, ov = O.tevo_expm_multiply(opp, cd, e0, -2.0, 8.0, 101, +1)
+    assert_allclose(O.tevo_part(opp, cd, e0, times, +1), ov, atol=1e-9)
+
+
 @pytest.mark.parametrize("ns", [2, 3, 4])
 def test_cdag_c_adjoint(ns):
     # cmpy/tests/test_operator.py:31-42
