@@ -10,8 +10,10 @@ written once).  The same run also
     (up to 300 steps, Ritz residual 1e-10, explicit residual reported),
   * times the CPU restatement (oracle C twin) of the same H.v on the host
     cores as the CPU baseline, and checks the GPU result against it,
-  * optionally runs the secondary workloads (12-site chain GS + CF G, batched
-    Anderson solves, bath-fit gradients) when their modules are present.
+  * runs the secondary workloads (12-site chain GS + CF G, 12-site chain
+    real-time G, finite-T Lehmann G of a 1+6-site impurity model, batched
+    Anderson solves, bath-fit gradients), each with its
+    CPU baseline, unless --no-secondary is given.
 
 Prints ONE JSON line.
 """
@@ -203,7 +205,8 @@ def main():
 
     if not args.no_secondary:
         result["secondary"] = {}
-        for name, fn in (("chain12_solves", bench_chain12), ("batched_anderson", bench_batched),
+        for name, fn in (("chain12_solves", bench_chain12), ("chain12_realtime_g", bench_tevo12),
+                         ("finite_t_lehmann", bench_finite_t), ("batched_anderson", bench_batched),
                          ("bathfit_gradients", bench_bathfit)):
             try:
                 result["secondary"][name] = fn(args, log)
@@ -322,6 +325,130 @@ def bench_chain12(args, log):
         out["cpu_baseline"] = {"value": round(1.0 / t, 4), "unit": "solves/s", "cores": 1, "kind": "port",
                                "sample": "1 solve (U = 4) by the oracle's numpy/scipy port: eigsh ground state "
                                          "+ 2 x 200 Lanczos steps + continued fractions"}
+    return out
+
+
+def bench_tevo12(args, log):
+    """Real-time G of the 12-site open chain at U = 4, sector (6,6): ground
+    state to 1e-10, then g>(t) and g<(t) of site 0 (spin up) on 1000 times in
+    [0, 20], as gf_tevo does (cmpy/exactdiag.py:248-307)."""
+    import time
+
+    import numpy as np
+    import torch
+
+    from cmpy_amd.exactdiag import gf_greater, gf_lesser, groundstate
+    from cmpy_amd.models import HubbardModel, chain_neighbors
+
+    ns, n, U = 12, 6, 4.0
+    start, stop, num = 0.0, 20.0, 1000
+    model = HubbardModel(ns, chain_neighbors(ns, periodic=False), inter=U, eps=0.0, hop=1.0, mu=0.0)
+    groundstate(model, n, n, tol=1e-10)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gs = groundstate(model, n, n, tol=1e-10)
+    times, g_gt = gf_greater(model, gs, start, stop, num)
+    _, g_lt = gf_lesser(model, gs, start, stop, num)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"metric": "solves/s", "value": round(1.0 / dt, 3), "unit": "solves/s", "dim": 853776,
+           "seconds": round(dt, 4), "times": num, "t_range": [start, stop],
+           "g_at_0": [float(g_gt[0].imag), float(g_lt[0].imag)],
+           "data": "deterministic model (no random input beyond the seeded Lanczos start)"}
+    log(f"12-site chain real-time G: {dt:.3f} s per solve")
+    if not args.no_cpu:
+        import sys
+
+        sys.path.insert(0, str(ROOT))
+        from oracle import ed_oracle as O
+
+        # bounded sample of the reference's procedure: the greater part over
+        # the first 5% of the time range (expm_multiply cost grows linearly
+        # with the range), scaled by 20 x 2 parts
+        s_stop, s_num = 1.0, 50
+        with _cpu_single_thread():
+            bonds = O.chain_bonds(ns)
+            op = O.hubbard_operator(ns, n, n, bonds, U, 0.0, 1.0)
+            t0 = time.perf_counter()
+            e0, psi = O.groundstate(op, tol=1e-10)
+            t_gs = time.perf_counter() - t0
+            phi = O.apply_cdag(psi, ns, n, n, 0, O.UP)
+            opx = O.hubbard_operator(ns, n + 1, n, bonds, U, 0.0, 1.0)
+            opx.csr()
+            t0 = time.perf_counter()
+            ts, ov = O.tevo_expm_multiply(opx, phi, e0, start, s_stop, s_num, +1)
+            t_part = time.perf_counter() - t0
+        est = t_gs + 2 * t_part * (stop - start) / (s_stop - start)
+        _, g_s = gf_greater(model, gs, start, s_stop, s_num)
+        out["parity_sample"] = {"max_abs_diff_vs_expm_multiply": float(np.abs(g_s - ov).max()),
+                                "max_abs": float(np.abs(ov).max())}
+        out["cpu_baseline"] = {
+            "value": round(1.0 / est, 5), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"the oracle's port of the reference procedure (scipy expm_multiply + overlap, 1 core): "
+                      f"ground state ({t_gs:.1f} s) and the greater part over t in [0, {s_stop}] "
+                      f"({s_num} points, {t_part:.1f} s), scaled to both parts over [0, {stop}]"}
+    return out
+
+
+def bench_finite_t(args, log):
+    """Grand-canonical finite-T Lehmann G of a 1+6-site Anderson impurity model
+    (U = 4, eps_imp = -2, mu = 0, bath from seed 0), beta = 20, G_00 up on
+    1000 frequencies in [-8, 8] + 0.05i, every sector diagonalised densely
+    (cmpy/exactdiag.py:137-245; largest sector D = 1225)."""
+    import time
+
+    import numpy as np
+    import torch
+
+    from cmpy_amd.exactdiag import gf_lehmann
+    from cmpy_amd.models import SingleImpurityAndersonModel
+
+    rng = np.random.default_rng(0)
+    nb = 6
+    eps, v = rng.uniform(-2, 2, nb), rng.uniform(0.2, 1.0, nb)
+    U, eimp, beta = 4.0, -2.0, 20.0
+    z = np.linspace(-8, 8, 1000) + 0.05j
+    model = SingleImpurityAndersonModel(U, eimp, eps, v, mu=0.0)
+    gf_lehmann(model, z[:8], beta)  # warm (kernels, solver handle)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    data = gf_lehmann(model, z, beta)
+    g = data.gf
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"metric": "solves/s", "value": round(1.0 / dt, 4), "unit": "solves/s", "seconds": round(dt, 4),
+           "sites": nb + 1, "sectors": (nb + 2) ** 2, "max_dim": 1225, "beta": beta, "frequencies": len(z),
+           "occ": data.occ, "occ_double": data.occ_double,
+           "data": "synthetic (bath from numpy default_rng(0))"}
+    log(f"finite-T Lehmann (1+{nb} sites): {dt:.3f} s per solve")
+    if not args.no_cpu:
+        import sys
+
+        sys.path.insert(0, str(ROOT))
+        from oracle import ed_oracle as O
+
+        mk = lambda nu, nd: O.siam_operator(nu, nd, U, eimp, eps, v, 0.0)
+        # bounded samples: every sector, on 10 and on 20 of the frequencies;
+        # the cost is fixed (dense eigh, overlaps) plus linear in the count
+        ts = {}
+        with _cpu_single_thread():
+            for nz in (10, 20):
+                zs = z[:: len(z) // nz][:nz]
+                t0 = time.perf_counter()
+                ref = O.gf_lehmann(mk, nb + 1, zs, beta)
+                ts[nz] = time.perf_counter() - t0
+        per_z = max(ts[20] - ts[10], 0.0) / 10
+        fixed = max(ts[10] - 10 * per_z, 0.0)
+        est = fixed + per_z * len(z)
+        zs = z[:: len(z) // 20][:20]
+        out["parity_sample"] = {"max_abs_diff": float(np.abs(g[:: len(z) // 20][:20] - ref.gf).max()),
+                                "max_abs": float(np.abs(ref.gf).max()),
+                                "occ_diff": abs(data.occ - ref.occ), "occ_double_diff": abs(data.occ_double - ref.occ_double)}
+        out["cpu_baseline"] = {
+            "value": round(1.0 / est, 5), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"the oracle's gf_lehmann (numpy eigh + vectorised pair sum, 1 core) on every sector at 10 "
+                      f"and 20 of the frequencies ({ts[10]:.1f} s, {ts[20]:.1f} s), extrapolated linearly to "
+                      f"{len(z)}"}
     return out
 
 

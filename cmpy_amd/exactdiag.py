@@ -440,7 +440,10 @@ def _tevo(model, gs, times, pos, sigma, create, nsteps, reorth, sign_mode, tol):
     op = model.hamilton_operator(n_up=sec[0], n_dn=sec[1])
     tmax = float(np.abs(times).max()) if len(times) else 0.0
     m = int(nsteps) if nsteps is not None else _tevo_nsteps(op, tmax)
-    a, b, n2 = _cf_coeffs(op, phi, m, reorth)
+    # the plain recurrence: its quadrature matches the reorthogonalised one to
+    # ~1e-14 (lost orthogonality only splits a converged node's weight over
+    # copies) at a fraction of the cost
+    a, b, n2 = _cf_coeffs(op, phi, m, False if reorth is None else reorth)
     tt = torch.from_numpy(np.ascontiguousarray(times, dtype=np.float64)).cuda()
     g = torch.zeros(2 * len(times), dtype=torch.float64, device="cuda")
     if len(a) and len(times):
@@ -461,7 +464,8 @@ def gf_greater(model, gs: EigenState, start, stop, num=1000, pos=0, sigma=UP, ns
     time; here one Lanczos run from c+ psi0 in the upper sector gives the Gauss
     quadrature of its spectral measure, and the time sum runs on the device
     (``ed_tevo_eval``).  ``nsteps`` defaults to a count that reaches FP64
-    accuracy up to max |t| (see ``_tevo_nsteps``).  Returns (times, g).
+    accuracy up to max |t| (see ``_tevo_nsteps``); ``reorth`` defaults to the
+    plain three-vector recurrence.  Returns (times, g).
     """
     times = np.linspace(start, stop, num)
     g = _tevo(model, gs, times, pos, sigma, True, nsteps, reorth, sign_mode, tol)

@@ -85,9 +85,13 @@ def test_long_times_mid_sector_vs_expm_multiply(gpu):
     times, ov = O.tevo_expm_multiply(mk(6, 5), cd, gs.energy, 0.0, 30.0, 61, +1)
     t, g = gf_greater(model, gs, 0.0, 30.0, 61)
     assert_allclose(g, ov, atol=1e-9)
-    # more steps change nothing: the default count has converged
+    # more steps change nothing: the default count has converged; and the
+    # reorthogonalised recurrence gives the same quadrature as the default
+    # plain one
     _, g2 = gf_greater(model, gs, 0.0, 30.0, 61, nsteps=800)
     assert np.abs(g2 - g).max() <= 1e-11
+    _, g3 = gf_greater(model, gs, 0.0, 30.0, 61, reorth=True)
+    assert np.abs(g3 - g).max() <= 1e-11
 
 
 def test_missing_sector_returns_zeros(gpu):
