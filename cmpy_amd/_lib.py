@@ -125,6 +125,7 @@ SIGNATURES = {
     "ed_gemm_nt": (C.c_int, [vp, vp, vp, i64, i64, i64, vp]),
     "ed_occupation": (C.c_int, [vp, i64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp]),
     "ed_dense_eigh": (C.c_int, [vp, i64, vp, vp]),
+    "ed_dense_eigh_batched": (C.c_int, [vp, i64, i64, vp, vp]),
     "ed_batch_create": (C.c_int, [P(vp), P(BatchDesc), vp]),
     "ed_batch_destroy": (C.c_int, [vp]),
     "ed_batch_config": (C.c_int, [vp, i64, i32, i32, vp]),

@@ -26,10 +26,13 @@ __global__ void k_unrank_states(uint32_t* out, int64_t count, int num_sites, int
   if (i < count) out[i] = comb_unrank(i, num_sites, n);
 }
 
-// sigma = up: one block per target row (grid-stride), lanes over columns.
+// sigma = up: one block per target row (grid-stride), lanes over columns;
+// blockIdx.y picks the vector (source stride xs, target stride ys).
 __global__ __launch_bounds__(256) void k_ladder_up(int num_sites, int n_tgt, int pos, int create, int jw,
                                                    const double* __restrict__ x, double* __restrict__ y,
-                                                   int64_t rows_tgt, int64_t cols) {
+                                                   int64_t rows_tgt, int64_t cols, int64_t xs, int64_t ys) {
+  x += (int64_t)blockIdx.y * xs;
+  y += (int64_t)blockIdx.y * ys;
   const uint32_t bit = 1u << pos;
   for (int64_t it = blockIdx.x; it < rows_tgt; it += gridDim.x) {
     const uint32_t st = comb_unrank(it, num_sites, n_tgt);
@@ -47,11 +50,14 @@ __global__ __launch_bounds__(256) void k_ladder_up(int num_sites, int n_tgt, int
   }
 }
 
-// sigma = dn: lanes over target columns, grid.y over row chunks.
+// sigma = dn: lanes over target columns, grid.y over row chunks, grid.z over
+// vectors (source stride xs, target stride ys).
 __global__ __launch_bounds__(256) void k_ladder_dn(int num_sites, int n_tgt, int pos, int create, int jw,
                                                    const double* __restrict__ x, double* __restrict__ y,
                                                    int64_t rows, int64_t cols_src, int64_t cols_tgt,
-                                                   int64_t rows_per_block) {
+                                                   int64_t rows_per_block, int64_t xs, int64_t ys) {
+  x += (int64_t)blockIdx.z * xs;
+  y += (int64_t)blockIdx.z * ys;
   const int64_t jt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (jt >= cols_tgt) return;
   const uint32_t bit = 1u << pos;
@@ -89,16 +95,22 @@ static int ladder(int num_sites, int n_up, int n_dn, int pos, int sigma, int sig
   const int64_t su = (int64_t)binom_u(num_sites, n_up), sd = (int64_t)binom_u(num_sites, n_dn);
   const int64_t tu = (int64_t)binom_u(num_sites, nu_t), td = (int64_t)binom_u(num_sites, nd_t);
   const int jw = sign_mode == ED_SIGN_JORDAN_WIGNER;
-  for (int64_t v = 0; v < nvec; ++v) {
-    const double* xv = x + v * su * sd;
-    double* yv = y + v * tu * td;
+  // vectors in launches of at most 65535 (the y / z grid limit)
+  for (int64_t v0 = 0; v0 < nvec; v0 += 65535) {
+    const unsigned nv = (unsigned)std::min<int64_t>(nvec - v0, 65535);
+    const double* xv = x + v0 * su * sd;
+    double* yv = y + v0 * tu * td;
     if (sigma == ED_UP) {
-      const unsigned g = (unsigned)std::min<int64_t>(tu, 65535);
-      hipLaunchKernelGGL(k_ladder_up, dim3(g), dim3(256), 0, s, num_sites, nu_t, pos, create, jw, xv, yv, tu, td);
+      const unsigned g = (unsigned)std::min<int64_t>(tu, std::max<int64_t>(1, 65535 / nv));
+      hipLaunchKernelGGL(k_ladder_up, dim3(g, nv), dim3(256), 0, s, num_sites, nu_t, pos, create, jw, xv, yv, tu,
+                         td, su * sd, tu * td);
     } else {
       const int64_t rpb = 64;
-      const dim3 g((unsigned)((td + 255) / 256), (unsigned)((su + rpb - 1) / rpb));
-      hipLaunchKernelGGL(k_ladder_dn, g, dim3(256), 0, s, num_sites, nd_t, pos, create, jw, xv, yv, su, sd, td, rpb);
+      const int64_t chunks = (su + rpb - 1) / rpb;
+      ED_REQUIRE(chunks <= 65535, "sector too large for the dn ladder grid");
+      const dim3 g((unsigned)((td + 255) / 256), (unsigned)chunks, nv);
+      hipLaunchKernelGGL(k_ladder_dn, g, dim3(256), 0, s, num_sites, nd_t, pos, create, jw, xv, yv, su, sd, td, rpb,
+                         su * sd, tu * td);
     }
   }
   ED_HIP(hipGetLastError());
@@ -215,31 +227,55 @@ __global__ __launch_bounds__(256) void k_tevo_sum(const double* __restrict__ the
   }
 }
 
-// Finite-T pair sum, stage 1: block (z tile, m group) -> partial[mg][z].
-constexpr int kPairGroup = 16;
+// Finite-T pair sum, stage 1: block (z tile, pair group) -> partial[group][z].
+// Pairs p = m * n + ni of the group are staged in LDS kPairTile at a time as
+// weight w = ov^2 (e^{-beta(E_n - emin)} + e^{-beta(E_m - emin)}) and pole
+// E_m - E_n, computed once per block; each lane then sums w / (z - pole) for
+// its frequency as w (dx - i y) / (dx^2 + y^2), dx = Re z - pole, with the
+// reciprocal from the hardware estimate (v_rcp_f64) refined by two Newton
+// steps r <- r + r (1 - d r), each of which squares the relative error.
+constexpr int kPairTile = 1024;
+constexpr int64_t kPairsPerGroup = 8192;
 __global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ ov, const double* __restrict__ en,
                                                int64_t n, const double* __restrict__ em, int64_t m,
                                                double beta, double emin, const double* __restrict__ z,
                                                int64_t nz, double* __restrict__ partial) {
+  __shared__ double sw[kPairTile], sp[kPairTile];
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t mg = blockIdx.y;
-  cplx acc = {0.0, 0.0};
-  if (k < nz) {
-    const cplx zz = {z[2 * k], z[2 * k + 1]};
-    const bool fin = isfinite(beta);
-    for (int64_t mi = mg * kPairGroup; mi < (mg + 1) * kPairGroup && mi < m; ++mi) {
-      const double e_m = em[mi];
-      const double xm = fin ? exp(-beta * (e_m - emin)) : 1.0;
-      for (int64_t ni = 0; ni < n; ++ni) {
-        const double o = ov[mi * n + ni];
-        const double wgt = o * o * ((fin ? exp(-beta * (en[ni] - emin)) : 1.0) + xm);
-        const cplx q = cdiv({wgt, 0.0}, {zz.re - e_m + en[ni], zz.im});
-        acc.re += q.re;
-        acc.im += q.im;
-      }
+  const int64_t grp = blockIdx.y;
+  const int64_t np = n * m;
+  const int64_t p0 = grp * kPairsPerGroup;
+  const int64_t p1 = p0 + kPairsPerGroup < np ? p0 + kPairsPerGroup : np;
+  const double zr = k < nz ? z[2 * k] : 0.0, zi = k < nz ? z[2 * k + 1] : 1.0;
+  const double zi2 = zi * zi;
+  const bool fin = isfinite(beta);
+  double re = 0.0, sr = 0.0;
+  for (int64_t t0 = p0; t0 < p1; t0 += kPairTile) {
+    const int cnt = (int)(p1 - t0 < kPairTile ? p1 - t0 : kPairTile);
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+      const int64_t p = t0 + i;
+      const int64_t mi = p / n, ni = p - mi * n;
+      const double o = ov[p];
+      const double e_m = em[mi], e_n = en[ni];
+      sw[i] = o * o * (fin ? exp(-beta * (e_n - emin)) + exp(-beta * (e_m - emin)) : 2.0);
+      sp[i] = e_m - e_n;
     }
-    partial[2 * (mg * nz + k)] = acc.re;
-    partial[2 * (mg * nz + k) + 1] = acc.im;
+    __syncthreads();
+    for (int i = 0; i < cnt; ++i) {
+      const double dx = zr - sp[i];
+      const double d = fma(dx, dx, zi2);
+      const double r0 = __builtin_amdgcn_rcp(d);
+      const double r1 = fma(r0, fma(-d, r0, 1.0), r0);
+      const double r = fma(r1, fma(-d, r1, 1.0), r1);
+      const double wr = sw[i] * r;
+      re = fma(wr, dx, re);
+      sr += wr;
+    }
+  }
+  if (k < nz) {
+    partial[2 * (grp * nz + k)] = re;
+    partial[2 * (grp * nz + k) + 1] = -zi * sr;
   }
 }
 
@@ -468,7 +504,8 @@ int ed_lehmann_pairs(const double* overlap_dev, const double* evals_dev, int64_t
   ED_REQUIRE(overlap_dev && evals_dev && evals_p1_dev && z_dev && g_dev, "NULL buffer");
   ED_REQUIRE(!std::isnan(beta) && std::isfinite(emin), "beta/emin invalid");
   hipStream_t s = as_stream(stream);
-  const int64_t ng = (m + kPairGroup - 1) / kPairGroup;
+  const int64_t ng = (n * m + kPairsPerGroup - 1) / kPairsPerGroup;
+  ED_REQUIRE(ng <= 65535, "sector pair too large for the pair-sum grid (%lld groups)", (long long)ng);
   double* part = nullptr;
   ED_HIP(hipMalloc(&part, sizeof(double) * 2 * ng * nz));
   int rc = ED_OK;
@@ -560,6 +597,42 @@ int ed_dense_eigh(double* a_dev, int64_t n, double* w_dev, void* stream) {
   }
   (void)hipFree(e);
   (void)hipFree(info);
+  return rc;
+}
+
+int ed_dense_eigh_batched(double* a_dev, int64_t n, int64_t batch, double* w_dev, void* stream) {
+  clear_error();
+  ED_REQUIRE(n >= 0 && n <= 65536, "n=%lld invalid", (long long)n);
+  ED_REQUIRE(batch >= 0 && batch <= (1 << 30), "batch=%lld invalid", (long long)batch);
+  if (n == 0 || batch == 0) return ED_OK;
+  ED_REQUIRE(a_dev && w_dev, "NULL buffer");
+  rocblas_handle h;
+  ED_TRY(get_handle(&h));
+  hipStream_t s = as_stream(stream);
+  if (rocblas_set_stream(h, s) != rocblas_status_success) return set_error(ED_ERR_HIP, "rocblas_set_stream failed");
+  double* e = nullptr;
+  ED_HIP(hipMalloc(&e, sizeof(double) * n * batch + sizeof(rocblas_int) * batch));
+  rocblas_int* info = (rocblas_int*)(e + n * batch);
+  int rc = ED_OK;
+  const rocblas_status st = rocsolver_dsyevd_strided_batched(
+      h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, a_dev, (rocblas_int)n, (rocblas_stride)(n * n),
+      w_dev, (rocblas_stride)n, e, (rocblas_stride)n, info, (rocblas_int)batch);
+  std::vector<rocblas_int> hinfo((size_t)batch, 0);
+  if (st != rocblas_status_success) {
+    rc = set_error(ED_ERR_HIP, "rocsolver_dsyevd_strided_batched failed with status %d", (int)st);
+  } else if (hipMemcpyAsync(hinfo.data(), info, sizeof(rocblas_int) * batch, hipMemcpyDeviceToHost, s) !=
+                 hipSuccess ||
+             hipStreamSynchronize(s) != hipSuccess) {
+    rc = set_error(ED_ERR_HIP, "reading rocsolver info failed");
+  } else {
+    for (int64_t k = 0; k < batch; ++k)
+      if (hinfo[k] != 0) {
+        rc = set_error(ED_ERR_NUMERIC, "rocsolver_dsyevd did not converge for matrix %lld (info=%d)", (long long)k,
+                       (int)hinfo[k]);
+        break;
+      }
+  }
+  (void)hipFree(e);
   return rc;
 }
 

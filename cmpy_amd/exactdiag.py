@@ -125,6 +125,28 @@ def _dense_eigh_device(op: SectorHamiltonOperator) -> Tuple[torch.Tensor, torch.
     return w, h.view(D, D)
 
 
+def _dense_eigh_sectors(model, sectors, cache: dict) -> None:
+    """Dense eigenpairs of every listed sector missing from ``cache``, with the
+    sectors of equal dimension diagonalised in one batched solver call."""
+    by_dim = {}
+    for sec in sectors:
+        if sec in cache:
+            continue
+        op = model.hamilton_operator(n_up=sec[0], n_dn=sec[1])
+        by_dim.setdefault(op.size, []).append((sec, op))
+    for D, group in by_dim.items():
+        if D == 0:
+            continue
+        B = len(group)
+        h = torch.empty((B, D, D), dtype=torch.float64, device="cuda")
+        for k, (_, op) in enumerate(group):
+            _lib.call("ed_op_dense", op.handle, _lib.dptr(h[k]), _lib.stream_ptr())
+        w = torch.empty((B, D), dtype=torch.float64, device="cuda")
+        _lib.call("ed_dense_eigh_batched", _lib.dptr(h), D, B, _lib.dptr(w), _lib.stream_ptr())
+        for k, (sec, _) in enumerate(group):
+            cache[sec] = (w[k], h[k])
+
+
 def compute_groundstate(model, thresh: int = 50, tol: float = 1e-10) -> EigenState:
     """Ground state over every (n_up, n_dn) sector (cmpy/exactdiag.py:24-43)."""
     gs = EigenState()
@@ -261,9 +283,11 @@ def gf_lehmann(model, z, beta, pos=0, sigma=UP, eig_cache=None, occ=True) -> Gre
     cache = eig_cache if eig_cache is not None else {}
     sign_mode = "jordan_wigner" if getattr(model, "sign_mode", "reference") == "fermion" else "reference"
 
+    pairs = [((nu, nd), upper_sector(nu, nd, sigma, ns)) for nu, nd in basis.iter_fillings()]
+    pairs = [(sec, p1) for sec, p1 in pairs if p1 is not None]
+    _dense_eigh_sectors(model, [s for pr in pairs for s in pr], cache)
+
     def solve(nu, nd):
-        if (nu, nd) not in cache:
-            cache[(nu, nd)] = _dense_eigh_device(model.hamilton_operator(n_up=nu, n_dn=nd))
         return cache[(nu, nd)]
 
     for n_up, n_dn in basis.iter_fillings():

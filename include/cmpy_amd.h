@@ -257,6 +257,12 @@ int ed_occupation(const double* vecs_dev, int64_t nvec, int num_sites, int n_up,
  * after vector: a[k*n + i] = component i of eigenvector k.                */
 int ed_dense_eigh(double* a_dev, int64_t n, double* w_dev, void* stream);
 
+/* ed_dense_eigh over `batch` matrices of one size n stored one after another
+ * (a_dev: [batch][n][n], w_dev: [batch][n]): the sectors of equal dimension
+ * that gf_lehmann diagonalises (cmpy/exactdiag.py:46-57, 215-245) in one
+ * solver call.                                                             */
+int ed_dense_eigh_batched(double* a_dev, int64_t n, int64_t batch, double* w_dev, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Batched Anderson impurity solver.  No reference counterpart: cmpy solves one
  * SingleImpurityAndersonModel per call (cmpy/models/anderson.py:163-164).
