@@ -24,7 +24,7 @@ from .models import (
 from .exactdiag import (
     EigenState, groundstate, compute_groundstate, solve_sector, GreensFunctionMeasurement,
     gf_lehmann, greens_lehmann_t0, greens_cf, iter_lanczos_coeffs, lanczos_coeffs,
-    lanczos_matrix, lanczos_ground_state, gf_greater, gf_lesser, gf_tevo,
+    lanczos_matrix, lanczos_ground_state, gf_greater, gf_lesser, gf_tevo, greens_lehmann, greens_cf_ij,
 )
 
 __version__ = "0.1.0"

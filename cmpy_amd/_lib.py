@@ -122,6 +122,7 @@ SIGNATURES = {
     "ed_tevo_eval": (C.c_int, [P(dbl), P(dbl), i32, dbl, dbl, i32, vp, i64, vp, i32, vp]),
     "ed_pole_sum": (C.c_int, [vp, vp, i64, vp, i64, vp, i32, vp]),
     "ed_lehmann_pairs": (C.c_int, [vp, vp, i64, vp, i64, dbl, dbl, vp, i64, vp, vp]),
+    "ed_lehmann_pairs_ij": (C.c_int, [vp, vp, vp, i64, vp, i64, dbl, dbl, vp, i64, vp, vp]),
     "ed_gemm_nt": (C.c_int, [vp, vp, vp, i64, i64, i64, vp]),
     "ed_occupation": (C.c_int, [vp, i64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp]),
     "ed_dense_eigh": (C.c_int, [vp, i64, vp, vp]),

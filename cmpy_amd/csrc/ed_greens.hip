@@ -234,12 +234,15 @@ __global__ __launch_bounds__(256) void k_tevo_sum(const double* __restrict__ the
 // its frequency as w (dx - i y) / (dx^2 + y^2), dx = Re z - pole, with the
 // reciprocal from the hardware estimate (v_rcp_f64) refined by two Newton
 // steps r <- r + r (1 - d r), each of which squares the relative error.
+// With a second overlap ov2 (off-diagonal G_ij) the weight takes ov * ov2
+// in place of ov^2.
 constexpr int kPairTile = 1024;
 constexpr int64_t kPairsPerGroup = 8192;
-__global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ ov, const double* __restrict__ en,
-                                               int64_t n, const double* __restrict__ em, int64_t m,
-                                               double beta, double emin, const double* __restrict__ z,
-                                               int64_t nz, double* __restrict__ partial) {
+__global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ ov, const double* __restrict__ ov2,
+                                               const double* __restrict__ en, int64_t n,
+                                               const double* __restrict__ em, int64_t m, double beta, double emin,
+                                               const double* __restrict__ z, int64_t nz,
+                                               double* __restrict__ partial) {
   __shared__ double sw[kPairTile], sp[kPairTile];
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t grp = blockIdx.y;
@@ -256,9 +259,9 @@ __global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ ov, co
     for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
       const int64_t p = t0 + i;
       const int64_t mi = p / n, ni = p - mi * n;
-      const double o = ov[p];
+      const double o = ov[p] * (ov2 ? ov2[p] : ov[p]);
       const double e_m = em[mi], e_n = en[ni];
-      sw[i] = o * o * (fin ? exp(-beta * (e_n - emin)) + exp(-beta * (e_m - emin)) : 2.0);
+      sw[i] = o * (fin ? exp(-beta * (e_n - emin)) + exp(-beta * (e_m - emin)) : 2.0);
       sp[i] = e_m - e_n;
     }
     __syncthreads();
@@ -495,27 +498,42 @@ int ed_pole_sum(const double* w_dev, const double* p_dev, int64_t m, const doubl
   return ED_OK;
 }
 
-int ed_lehmann_pairs(const double* overlap_dev, const double* evals_dev, int64_t n, const double* evals_p1_dev,
-                     int64_t m, double beta, double emin, const double* z_dev, int64_t nz, double* g_dev,
-                     void* stream) {
-  clear_error();
+static int lehmann_pairs(const double* ov, const double* ov2, const double* evals_dev, int64_t n,
+                         const double* evals_p1_dev, int64_t m, double beta, double emin, const double* z_dev,
+                         int64_t nz, double* g_dev, hipStream_t s) {
   ED_REQUIRE(n >= 0 && m >= 0 && nz >= 0, "negative size");
   if (n == 0 || m == 0 || nz == 0) return ED_OK;
-  ED_REQUIRE(overlap_dev && evals_dev && evals_p1_dev && z_dev && g_dev, "NULL buffer");
+  ED_REQUIRE(ov && evals_dev && evals_p1_dev && z_dev && g_dev, "NULL buffer");
   ED_REQUIRE(!std::isnan(beta) && std::isfinite(emin), "beta/emin invalid");
-  hipStream_t s = as_stream(stream);
   const int64_t ng = (n * m + kPairsPerGroup - 1) / kPairsPerGroup;
   ED_REQUIRE(ng <= 65535, "sector pair too large for the pair-sum grid (%lld groups)", (long long)ng);
   double* part = nullptr;
   ED_HIP(hipMalloc(&part, sizeof(double) * 2 * ng * nz));
   int rc = ED_OK;
-  hipLaunchKernelGGL(k_pairs, dim3((unsigned)((nz + 255) / 256), (unsigned)ng), dim3(256), 0, s, overlap_dev,
-                     evals_dev, n, evals_p1_dev, m, beta, emin, z_dev, nz, part);
+  hipLaunchKernelGGL(k_pairs, dim3((unsigned)((nz + 255) / 256), (unsigned)ng), dim3(256), 0, s, ov, ov2, evals_dev,
+                     n, evals_p1_dev, m, beta, emin, z_dev, nz, part);
   hipLaunchKernelGGL(k_pairs_reduce, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, part, ng, nz, g_dev);
   if (hipGetLastError() != hipSuccess) rc = set_error(ED_ERR_HIP, "pair-sum launch failed");
   if (hipStreamSynchronize(s) != hipSuccess && rc == ED_OK) rc = set_error(ED_ERR_HIP, "stream sync failed");
   (void)hipFree(part);
   return rc;
+}
+
+int ed_lehmann_pairs(const double* overlap_dev, const double* evals_dev, int64_t n, const double* evals_p1_dev,
+                     int64_t m, double beta, double emin, const double* z_dev, int64_t nz, double* g_dev,
+                     void* stream) {
+  clear_error();
+  return lehmann_pairs(overlap_dev, nullptr, evals_dev, n, evals_p1_dev, m, beta, emin, z_dev, nz, g_dev,
+                       as_stream(stream));
+}
+
+int ed_lehmann_pairs_ij(const double* overlap_i_dev, const double* overlap_j_dev, const double* evals_dev, int64_t n,
+                        const double* evals_p1_dev, int64_t m, double beta, double emin, const double* z_dev,
+                        int64_t nz, double* g_dev, void* stream) {
+  clear_error();
+  ED_REQUIRE(n == 0 || m == 0 || nz == 0 || overlap_j_dev, "NULL buffer");
+  return lehmann_pairs(overlap_i_dev, overlap_j_dev, evals_dev, n, evals_p1_dev, m, beta, emin, z_dev, nz, g_dev,
+                       as_stream(stream));
 }
 
 int ed_gemm_nt(const double* a_dev, const double* b_dev, double* c_dev, int64_t m, int64_t n, int64_t k,

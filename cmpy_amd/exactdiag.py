@@ -11,6 +11,8 @@
                               to the ground state
 * ``greens_cf``              T = 0 continued-fraction G from Lanczos
                               coefficients (no counterpart in cmpy)
+* ``greens_lehmann`` / ``greens_cf_ij``   off-diagonal G_ij (finite-T Lehmann,
+                              lehmann_full.py:19-97; T = 0 continued fractions)
 * ``gf_greater`` / ``gf_lesser`` / ``gf_tevo``   real-time G (248-307) from a
                               Lanczos quadrature of c+psi0 / c psi0
 * ``iter_lanczos_coeffs`` / ``lanczos_coeffs`` / ``lanczos_matrix`` /
@@ -38,7 +40,7 @@ __all__ = [
     "EigenState", "LanczosStats", "groundstate", "compute_groundstate", "solve_sector",
     "GreensFunctionMeasurement", "gf_lehmann", "greens_lehmann_t0", "greens_cf", "CFResult",
     "iter_lanczos_coeffs", "lanczos_coeffs", "lanczos_matrix", "lanczos_ground_state",
-    "gf_greater", "gf_lesser", "gf_tevo",
+    "gf_greater", "gf_lesser", "gf_tevo", "greens_lehmann", "greens_cf_ij", "LehmannIJ",
 ]
 
 
@@ -300,6 +302,83 @@ def gf_lehmann(model, z, beta, pos=0, sigma=UP, eig_cache=None, occ=True) -> Gre
     return data
 
 
+class LehmannIJ(NamedTuple):
+    gf: np.ndarray
+    part: float
+    gs_energy: float
+
+
+def _ij_sign_mode(model, sign_mode):
+    """Physical (anticommuting) ladder operators wherever the model's own
+    hopping carries fermionic signs: the Hubbard model, and the impurity
+    model in its fermion mode."""
+    if sign_mode is not None:
+        return sign_mode
+    if getattr(model, "sign_mode", None) == "reference":
+        return "reference"
+    return "jordan_wigner"
+
+
+def _cdag_rows(V: torch.Tensor, ns, sector, pos, sigma, sign_mode, d_p1) -> torch.Tensor:
+    C = torch.empty((V.shape[0], d_p1), dtype=torch.float64, device="cuda")
+    _lib.call("ed_apply_cdag", ns, sector[0], sector[1], pos, sigma, SIGN_MODES[sign_mode], _lib.dptr(V),
+              _lib.dptr(C), V.shape[0], _lib.stream_ptr())
+    return C
+
+
+def greens_lehmann(model, z, beta, i, j, sigma=UP, eig_cache=None, sign_mode=None) -> LehmannIJ:
+    """Finite-T off-diagonal Green's function (the sum lehmann_full.py:19-97 sets
+    out to form, ``greens_lehmann(model, z, beta, i, j)``):
+
+        G_ij(z) = (1/Z) sum_N sum_{m in N+1, n in N} <n|c_i|m><m|c+_j|n>
+                  (e^{-beta E_n} + e^{-beta E_m}) / (z - E_m + E_n)
+
+    with Z summed over every sector (the U = 0 sum is then exactly the
+    single-particle resolvent).  Every sector is diagonalised on the device
+    (sectors of one dimension in one batched call); the overlaps are two
+    device GEMMs per sector and the pair sum runs in ``ed_lehmann_pairs_ij``.
+    ``sign_mode`` defaults to Jordan-Wigner ladder operators for models with
+    fermionic hopping signs (see ``_ij_sign_mode``).
+    """
+    if not np.isfinite(beta):
+        raise ValueError("greens_lehmann needs a finite beta (greens_cf_ij gives T = 0)")
+    ns = model.num_sites
+    sign_mode = _ij_sign_mode(model, sign_mode)
+    cache = eig_cache if eig_cache is not None else {}
+    fillings = list(model.basis.iter_fillings())
+    _dense_eigh_sectors(model, fillings, cache)
+    emin = min(float(cache[sec][0][0].item()) for sec in fillings if cache[sec][0].numel())
+    part = 0.0
+    for sec in fillings:
+        w = cache[sec][0]
+        if w.numel():
+            part += float(torch.exp(-beta * (w - emin)).sum().item())
+    zt = _z_device(z)
+    g = torch.zeros(zt.numel(), dtype=torch.float64, device="cuda")
+    bb = float(beta)
+    for sec in fillings:
+        p1 = upper_sector(sec[0], sec[1], sigma, ns)
+        if p1 is None:
+            continue
+        w, V = cache[sec]
+        w1, V1 = cache[p1]
+        N, M = V.shape[0], V1.shape[0]
+        if N == 0 or M == 0:
+            continue
+        ovs = []
+        for pos in ((i,) if i == j else (i, j)):
+            C = _cdag_rows(V, ns, sec, pos, sigma, sign_mode, V1.shape[1])
+            ov = torch.empty((M, N), dtype=torch.float64, device="cuda")
+            _lib.call("ed_gemm_nt", _lib.dptr(V1), _lib.dptr(C), _lib.dptr(ov), M, N, V1.shape[1],
+                      _lib.stream_ptr())
+            ovs.append(ov)
+        _lib.call("ed_lehmann_pairs_ij", _lib.dptr(ovs[0]), _lib.dptr(ovs[-1]), _lib.dptr(w), N, _lib.dptr(w1), M,
+                  bb, emin, _lib.dptr(zt), zt.numel() // 2, _lib.dptr(g), _lib.stream_ptr())
+    gh = g.view(-1, 2).cpu().numpy()
+    gf = ((gh[:, 0] + 1j * gh[:, 1]) / part).reshape(np.shape(z))
+    return LehmannIJ(gf, part * np.exp(-beta * emin), emin)
+
+
 # =============================================================================
 # T = 0 Green's functions in a fixed sector
 # =============================================================================
@@ -432,6 +511,43 @@ def greens_cf(model, z, n_up, n_dn, pos=0, sigma=UP, nsteps=200, gs: EigenState 
     ap, bp, np_ = coeffs[True]
     ah, bh, nh = coeffs[False]
     return CFResult(gf, e0, ap, bp, np_, ah, bh, nh)
+
+
+def greens_cf_ij(model, z, n_up, n_dn, i, j, sigma=UP, nsteps=200, gs: EigenState = None, tol=1e-10,
+                 sign_mode=None, reorth: Optional[bool] = None) -> np.ndarray:
+    """T = 0 off-diagonal G_ij(z) in sector (n_up, n_dn) from continued fractions,
+    by polarisation: with c_(+-) = c_i +- c_j,
+
+        G_ij = (G[c_+] - G[c_-]) / 4,
+
+    where G[c] is the particle + hole continued fraction of the combined
+    operator (four Lanczos runs; G_ij = G_ji for a real Hamiltonian).
+    ``sign_mode`` as in :func:`greens_lehmann`."""
+    ns = model.num_sites
+    sign_mode = _ij_sign_mode(model, sign_mode)
+    e0, psi = _gs(model, n_up, n_dn, gs, tol)
+    zt = _z_device(z)
+    if i == j:
+        r = greens_cf(model, z, n_up, n_dn, pos=i, sigma=sigma, nsteps=nsteps, gs=EigenState(e0, psi, n_up, n_dn),
+                      sign_mode=sign_mode, reorth=reorth)
+        return r.gf
+    g = torch.zeros(zt.numel(), dtype=torch.float64, device="cuda")
+    for create in (True, False):
+        ai, sec = _ladder_device(psi, ns, n_up, n_dn, i, sigma, create, sign_mode)
+        if ai is None:
+            continue
+        aj, _ = _ladder_device(psi, ns, n_up, n_dn, j, sigma, create, sign_mode)
+        op = model.hamilton_operator(n_up=sec[0], n_dn=sec[1])
+        for pm in (1.0, -1.0):
+            phi = ai + pm * aj
+            a, b, n2 = _cf_coeffs(op, phi, nsteps, reorth)
+            if len(a) == 0 or n2 == 0.0:
+                continue
+            _lib.call("ed_cf_eval", a.ctypes.data_as(C.POINTER(C.c_double)),
+                      (b if len(b) else np.zeros(1)).ctypes.data_as(C.POINTER(C.c_double)), len(a), pm * n2 / 4.0,
+                      e0, 1 if create else -1, _lib.dptr(zt), zt.numel() // 2, _lib.dptr(g), 1, _lib.stream_ptr())
+    gh = g.view(-1, 2).cpu().numpy()
+    return (gh[:, 0] + 1j * gh[:, 1]).reshape(np.shape(z))
 
 
 # =============================================================================

@@ -238,6 +238,16 @@ int ed_lehmann_pairs(const double* overlap_dev, const double* evals_dev, int64_t
                      const double* evals_p1_dev, int64_t m, double beta, double emin,
                      const double* z_dev, int64_t nz, double* g_dev, void* stream);
 
+/* Off-diagonal pair sum: as ed_lehmann_pairs with |ov[m,n]|^2 replaced by
+ * ov_i[m,n] ov_j[m,n], where ov_x = V_p1^T (c+_x evecs), i.e.
+ *   g[k] += sum_{m,n} <n|c_i|m><m|c+_j|n> (e^{-beta(E_n-emin)} + e^{-beta(E_m-emin)})
+ *                     / (z_k - E_m + E_n)
+ * for real eigenvectors.  This is the sum lehmann_full.py:19-44 sets out to
+ * form (its code keeps only the c_i factor, squared).                        */
+int ed_lehmann_pairs_ij(const double* overlap_i_dev, const double* overlap_j_dev, const double* evals_dev,
+                        int64_t n, const double* evals_p1_dev, int64_t m, double beta, double emin,
+                        const double* z_dev, int64_t nz, double* g_dev, void* stream);
+
 /* C[M x N] = A B^T with A: [M][K], B: [N][K] (vectors stored one after
  * another): the overlap product evecs_p1^T (c+ evecs) of cmpy/exactdiag.py:112. */
 int ed_gemm_nt(const double* a_dev, const double* b_dev, double* c_dev, int64_t m, int64_t n,

@@ -607,6 +607,24 @@ def lehmann_t0(op_n, op_p1, op_m1, psi0, e0, cdag, c, z):
     return g
 
 
+def lehmann_t0_ij(op_p1, op_m1, e0, cdag_i, cdag_j, c_i, c_j, z):
+    """T = 0 fixed-sector G_ij: lehmann_t0 with the two operators of the
+    off-diagonal element,
+    G_ij(z) = sum_m <0|c_i|m><m|c+_j|0>/(z - E_m + E0) + sum_n <0|c+_j|n><n|c_i|0>/(z + E_n - E0),
+    real eigenvectors (so <0|c_i|m> = (V^T c+_i psi0)_m)."""
+    z = np.asarray(z, dtype=np.complex128)
+    g = np.zeros_like(z)
+    if op_p1 is not None:
+        w, v = np.linalg.eigh(op_p1.csr().toarray())
+        ov = (v.T @ cdag_i) * (v.T @ cdag_j)
+        g += (ov[None, :] / (z[:, None] - w[None, :] + e0)).sum(axis=1)
+    if op_m1 is not None:
+        w, v = np.linalg.eigh(op_m1.csr().toarray())
+        ov = (v.T @ c_i) * (v.T @ c_j)
+        g += (ov[None, :] / (z[:, None] + w[None, :] - e0)).sum(axis=1)
+    return g
+
+
 def occupation(up_states, dn_states, evals, evecs, beta, emin=0.0, pos=0, sigma=UP):
     """cmpy/exactdiag.py:60-92."""
     up = np.asarray(up_states, np.int64)
@@ -749,6 +767,59 @@ def gf_lehmann(make_op, num_sites, z, beta, pos=0, sigma=UP, occ=True):
         evals_p1, evecs_p1 = solve(*p1)
         data.accumulate(num_sites, (n_up, n_dn), p1, evals, evecs, evals_p1, evecs_p1)
     return data
+
+
+def lehmann_ij(make_op, num_sites, z, beta, i, j, sigma=UP, sign_mode="jordan_wigner"):
+    """Finite-T off-diagonal Green's function
+
+        G_ij(z) = (1/Z) sum_{N} sum_{m in N+1, n in N} <n|c_i|m><m|c+_j|n>
+                  (e^{-beta E_n} + e^{-beta E_m}) / (z - E_m + E_n),
+
+    the sum lehmann_full.py:19-97 (``greens_lehmann(model, z, beta, i, j)``)
+    sets out to form (its code keeps only the c_i factor, squared, and returns
+    the sum without 1/Z), normalised by the partition function of all sectors
+    (gf_lehmann, cmpy/exactdiag.py:228-243, leaves sectors without an upper
+    sector out of Z; with them the U = 0 sum is exactly the resolvent).
+    Energies are measured from the lowest eigenvalue of all sectors at once;
+    ``sign_mode="jordan_wigner"`` gives the physical (anticommuting) c, c+.
+    """
+    z = np.asarray(z, dtype=np.complex128)
+    fillings = list(itertools.product(range(num_sites + 1), repeat=2))
+    eig = {}
+    for nu, nd in fillings:
+        w, v = np.linalg.eigh(make_op(nu, nd).csr().toarray())
+        eig[(nu, nd)] = (w, v)
+    emin = min(float(w[0]) for w, _ in eig.values())
+    part = sum(float(np.sum(np.exp(-beta * (w - emin)))) for w, _ in eig.values()) if np.isfinite(beta) else None
+    g = np.zeros_like(z)
+    for nu, nd in fillings:
+        p1 = upper_sector(nu, nd, sigma, num_sites)
+        if p1 is None:
+            continue
+        w, v = eig[(nu, nd)]
+        w1, v1 = eig[p1]
+        ai = v1.T @ np.column_stack([apply_cdag(v[:, k], num_sites, nu, nd, i, sigma, sign_mode)
+                                     for k in range(v.shape[1])])
+        aj = v1.T @ np.column_stack([apply_cdag(v[:, k], num_sites, nu, nd, j, sigma, sign_mode)
+                                     for k in range(v.shape[1])])
+        if np.isfinite(beta):
+            wt = np.exp(-beta * (w - emin))[None, :] + np.exp(-beta * (w1 - emin))[:, None]
+        else:
+            wt = 2.0 * np.ones((len(w1), len(w)))
+        weights = ai * aj * wt
+        poles = w1[:, None] - w[None, :]
+        for k in range(len(z)):
+            g[k] += np.sum(weights / (z[k] - poles))
+    return g / part if part is not None else g
+
+
+def resolvent_ij(h, z):
+    """Non-interacting G_ij(z) = [(z - h)^{-1}]_ij for the single-particle matrix h
+    (on-site energies minus mu on the diagonal, hop_b on the bonds): at U = 0 the
+    grand-canonical Lehmann sum reduces to it at any temperature."""
+    z = np.atleast_1d(np.asarray(z, dtype=np.complex128))
+    w, v = np.linalg.eigh(h)
+    return np.einsum("ik,zk,jk->zij", v, 1.0 / (z[:, None] - w[None, :]), v)
 
 
 def gf0_lehmann(ham, z, mu=0.0, mode="diag"):
@@ -1012,4 +1083,8 @@ PARITY = {
     "Hubbard and Anderson sectors; tevo_two_site_ref at half filling",
     "tevo_expm_multiply": "tevo_part (dense eigh) and the two-site closed form",
     "tevo_two_site_ref": "impurity_params_ref poles (pinned via lehmann_t0); t = 0 value -i",
+    "lehmann_ij": "U = 0 resolvent_ij on rings and open chains at interior sites (a wrong string sign "
+    "or a dropped factor fails it); i = j equals gf_lehmann; G_ij = G_ji",
+    "resolvent_ij": "direct inversion of z - h",
+    "lehmann_t0_ij": "U = 0 closed-shell ground state: resolvent_ij at interior sites; i = j equals lehmann_t0",
 }

@@ -378,6 +378,78 @@ def test_gf_lehmann_non_interacting(ns, ref_fixtures):
     assert_allclose(g0[::50], direct, atol=1e-12)
 
 
+def _tb_matrix(ns, bonds, hop, eps):
+    h = np.diag(np.full(ns, float(eps)))
+    for i, j in bonds:
+        h[i, j] += hop
+        h[j, i] += hop
+    return h
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+@pytest.mark.parametrize("ij", [(1, 3), (0, 4), (2, 2)])
+def test_lehmann_ij_non_interacting_resolvent(periodic, ij):
+    """At U = 0 the grand-canonical G_ij is the single-particle resolvent at any
+    temperature; interior sites need the Jordan-Wigner string."""
+    ns = 6
+    bonds = O.chain_bonds(ns, periodic=periodic)
+    mk = lambda nu, nd: O.hubbard_operator(ns, nu, nd, bonds, 0.0, 0.4, 1.0, mu=0.3)
+    z = np.linspace(-4, 4, 41) + 0.1j
+    g = O.lehmann_ij(mk, ns, z, 3.0, *ij)
+    ref = O.resolvent_ij(_tb_matrix(ns, bonds, 1.0, 0.4 - 0.3), z)[:, ij[0], ij[1]]
+    assert_allclose(g, ref, atol=1e-11)
+    direct = np.linalg.inv(z[7] * np.eye(ns) - _tb_matrix(ns, bonds, 1.0, 0.1))[ij[0], ij[1]]
+    assert abs(ref[7] - direct) < 1e-12
+    if ij[0] != ij[1] and periodic:
+        # without the string the off-diagonal sum is not the resolvent
+        bad = O.lehmann_ij(mk, ns, z, 3.0, *ij, sign_mode="reference")
+        assert np.abs(bad - ref).max() > 1e-3
+
+
+@pytest.mark.parametrize("ij", [(1, 4), (0, 3), (2, 2)])
+def test_lehmann_t0_ij_closed_shell_resolvent(ij):
+    """6-site ring at half filling (closed shell): T = 0 G_ij at U = 0 is the
+    resolvent; with U the i = j case equals lehmann_t0."""
+    ns = 6
+    bonds = O.chain_bonds(ns, periodic=True)
+    z = np.linspace(-4, 4, 41) + 0.1j
+    for u in (0.0, 3.0):
+        mk = lambda nu, nd: O.hubbard_operator(ns, nu, nd, bonds, u, 0.0, 1.0, mu=u / 2)
+        op = mk(3, 3)
+        e0, psi = O.groundstate(op)
+        lad = lambda f, p: f(psi, ns, 3, 3, p, O.UP, "jordan_wigner")
+        g = O.lehmann_t0_ij(mk(4, 3), mk(2, 3), e0, lad(O.apply_cdag, ij[0]), lad(O.apply_cdag, ij[1]),
+                            lad(O.apply_c, ij[0]), lad(O.apply_c, ij[1]), z)
+        if u == 0.0:
+            ref = O.resolvent_ij(_tb_matrix(ns, bonds, 1.0, 0.0), z)[:, ij[0], ij[1]]
+            assert_allclose(g, ref, atol=1e-11)
+        elif ij[0] == ij[1]:
+            ref = O.lehmann_t0(op, mk(4, 3), mk(2, 3), psi, e0, lad(O.apply_cdag, ij[0]), lad(O.apply_c, ij[0]), z)
+            assert_allclose(g, ref, atol=1e-12)
+
+
+def test_lehmann_ij_diagonal_and_symmetry():
+    ns = 5
+    bonds = O.chain_bonds(ns, periodic=True)
+    mk = lambda nu, nd: O.hubbard_operator(ns, nu, nd, bonds, 2.5, 0.0, 1.0, mu=1.25)
+    z = np.linspace(-6, 6, 31) + 0.2j
+    # i = j is gf_lehmann with the same ladder operators (site 0 carries no
+    # string, so both sign modes agree there; at an interior site the
+    # reference's string-free c+ is a different operator), up to the
+    # partition function: gf_lehmann sums Z only over sectors that have an
+    # upper sector (cmpy/exactdiag.py:228-243), lehmann_ij over all of them
+    beta = 4.0
+    spec = {(nu, nd): np.linalg.eigvalsh(mk(nu, nd).csr().toarray())
+            for nu in range(ns + 1) for nd in range(ns + 1)}
+    emin = min(w[0] for w in spec.values())
+    z_all = sum(np.exp(-beta * (w - emin)).sum() for w in spec.values())
+    z_ref = sum(np.exp(-beta * (w - emin)).sum() for (nu, nd), w in spec.items() if nu < ns)
+    for pos, mode in ((0, "jordan_wigner"), (2, "reference")):
+        g_ref = O.gf_lehmann(mk, ns, z, beta, pos=pos, occ=False).gf
+        assert_allclose(O.lehmann_ij(mk, ns, z, beta, pos, pos, sign_mode=mode), g_ref * z_ref / z_all, atol=1e-12)
+    assert_allclose(O.lehmann_ij(mk, ns, z, beta, 1, 3), O.lehmann_ij(mk, ns, z, beta, 3, 1), atol=1e-12)
+
+
 @pytest.mark.parametrize("ns", [2, 3])
 @pytest.mark.parametrize("u", [1.0, 4.0])
 def test_gf_lehmann_atomic_and_occupation(ns, u, ref_fixtures):
