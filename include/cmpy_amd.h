@@ -11,8 +11,11 @@
  *     No C++ exception crosses this boundary.
  *   - Pointers named *_dev are device (HBM) pointers; pointers without the
  *     suffix are host pointers.  The caller owns every buffer it passes.  The
- *     library allocates only what sits behind an ed_op handle (state lists,
- *     hop tables, diagonal tables) and frees it in ed_op_destroy.
+ *     library owns what sits behind an ed_op handle (state lists, hop tables,
+ *     diagonal tables) or an ed_batch handle, freed by ed_op_destroy /
+ *     ed_batch_destroy; a few calls (continued fraction, time sum, pair sums,
+ *     dense eigensolvers, occupations) allocate scratch and free it before
+ *     returning.
  *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
  *     Device work is enqueued on it; functions that return host results
  *     (ed_lanczos_*, ed_batch_*) synchronise that stream before returning.
