@@ -189,9 +189,15 @@ def solve_sector(model, sector: Sector, cache: dict = None):
 class GreensFunctionMeasurement:
     """Accumulator of the grand-canonical Lehmann sum (cmpy/exactdiag.py:137-212).
 
-    Partition function, G(z) and occupations are accumulated relative to the
-    lowest energy seen so far; whenever a lower one appears every accumulated
-    quantity is rescaled by exp(-beta (E_old - E_new)).
+    Partition function, G(z) and occupations are accumulated relative to a
+    reference energy.  As in the reference, that is the lowest energy seen so
+    far, and whenever a lower one appears every accumulated quantity is
+    rescaled by exp(-beta (E_old - E_new)).  When the caller knows the lowest
+    energy of every sector involved beforehand (``set_reference_energy``; the
+    device path diagonalises all sectors first), that fixed value is used
+    instead: the normalised results are the same, and the weights of upper
+    sectors lying below the lowest energy seen so far can no longer overflow
+    at low temperature.
     """
 
     def __init__(self, z, beta, pos=0, sigma=UP, dtype=None, measure_occ=True):
@@ -207,10 +213,19 @@ class GreensFunctionMeasurement:
             np.stack([self.z.real.ravel(), self.z.imag.ravel()], axis=1).ravel())).cuda()
         self._occ = 0.0
         self._occ_double = 0.0
+        self._eref = None
+
+    def set_reference_energy(self, e: float) -> None:
+        """Fix the energy the weights are measured from (at most the lowest
+        eigenvalue of every sector that will be accumulated)."""
+        self._eref = float(e)
+
+    def _shift(self) -> float:
+        return self._gs_energy if self._eref is None else self._eref
 
     @property
     def part(self):
-        return self._part * np.exp(-self.beta * self._gs_energy)
+        return self._part * np.exp(-self.beta * self._shift())
 
     @property
     def gf(self):
@@ -231,7 +246,7 @@ class GreensFunctionMeasurement:
 
     def _weights(self, w: torch.Tensor) -> torch.Tensor:
         if np.isfinite(self.beta):
-            return torch.exp(-self.beta * (w - self._gs_energy))
+            return torch.exp(-self.beta * (w - self._shift()))
         return torch.ones_like(w)
 
     def accumulate(self, num_sites, sector, sector_p1, w, V, w_p1, V_p1, sign_mode="reference"):
@@ -240,7 +255,8 @@ class GreensFunctionMeasurement:
         min_energy = float(w[0].item())
         factor = 1.0
         if min_energy < self._gs_energy:
-            factor = float(np.exp(-self.beta * (self._gs_energy - min_energy)))
+            if self._eref is None:
+                factor = float(np.exp(-self.beta * (self._gs_energy - min_energy)))
             self._gs_energy = min_energy
         boltz = self._weights(w)
         self._part = self._part * factor + float(boltz.sum().item())
@@ -258,7 +274,7 @@ class GreensFunctionMeasurement:
                   _lib.stream_ptr())
         beta = float(self.beta) if np.isfinite(self.beta) else float("inf")
         _lib.call("ed_lehmann_pairs", _lib.dptr(ov), _lib.dptr(w), N, _lib.dptr(w_p1), M, beta,
-                  float(self._gs_energy), _lib.dptr(self._z_dev), self.z.size, _lib.dptr(self._gf),
+                  float(self._shift()), _lib.dptr(self._z_dev), self.z.size, _lib.dptr(self._gf),
                   _lib.stream_ptr())
         if self._measure_occ:
             out = torch.empty(2, dtype=torch.float64, device="cuda")
@@ -287,7 +303,10 @@ def gf_lehmann(model, z, beta, pos=0, sigma=UP, eig_cache=None, occ=True) -> Gre
 
     pairs = [((nu, nd), upper_sector(nu, nd, sigma, ns)) for nu, nd in basis.iter_fillings()]
     pairs = [(sec, p1) for sec, p1 in pairs if p1 is not None]
-    _dense_eigh_sectors(model, [s for pr in pairs for s in pr], cache)
+    secs = [s for pr in pairs for s in pr]
+    _dense_eigh_sectors(model, secs, cache)
+    if secs:
+        data.set_reference_energy(min(float(cache[s][0][0].item()) for s in set(secs)))
 
     def solve(nu, nd):
         return cache[(nu, nd)]

@@ -225,6 +225,25 @@ def test_gf_lehmann_finite_t(gpu, ns, ref_fixtures):
             assert abs(meas.occ_double - ref.occ_double) < 1e-10
 
 
+def test_gf_lehmann_low_temperature_no_overflow(gpu):
+    """At beta = 400 an upper sector lies far below the lowest energy the
+    reference's visiting order has seen, and its weight e^{-beta (E_m - E)}
+    would overflow; with every sector solved first the weights are measured
+    from the global minimum.  The result is then the T = 0 ground-sector G
+    (unique ground state in (1,1)).  (Z itself, e^{-beta E0} ~ e^{1600},
+    is beyond FP64 here, for the reference as well.)"""
+    from cmpy_amd import SingleImpurityAndersonModel, gf_lehmann, greens_lehmann_t0
+
+    u, v = 4.0, 0.7
+    m = SingleImpurityAndersonModel(u, 0.0, 0.0, [v], mu=u / 2)
+    z = np.linspace(-6, 6, 601) + 0.05j
+    meas = gf_lehmann(m, z, beta=400.0, occ=True)
+    assert np.isfinite(meas.gf).all()
+    assert_allclose(meas.gf, greens_lehmann_t0(m, z, 1, 1), atol=1e-10)
+    assert_allclose(meas.gf, O.impurity_gf_ref(z, u, v), atol=1e-10)
+    assert abs(meas.occ - 0.5) < 1e-10
+
+
 def test_workspace_and_numeric_errors(gpu):
     import ctypes as C
     from cmpy_amd import HubbardModel, chain_neighbors, _lib as L
