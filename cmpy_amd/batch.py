@@ -224,3 +224,57 @@ def anderson_gf(eps_bath, v, batch: AndersonBatch, wn):
     must be created with ``keep_basis=True``; one forward/backward pair at a
     time per batch (the backward uses the batch's last solve)."""
     return _AndersonG.apply(eps_bath, v, batch, np.ascontiguousarray(wn, dtype=np.float64))
+
+
+@dataclass
+class BathFit:
+    eps_bath: np.ndarray  # [M, max_bath] fitted parameters (padding entries unused)
+    v: np.ndarray
+    loss: np.ndarray      # [M] final per-model loss sum_n |G - G_target|^2
+    history: list         # total loss at every evaluation
+    evaluations: int
+
+
+def fit_bath(batch: AndersonBatch, wn, g_target, eps0=None, v0=None, max_iter: int = 100,
+             tolerance_change: float = 1e-24, history_size: int = 20) -> BathFit:
+    """Gradient-driven bath fit of every model of ``batch``: minimise
+    L = sum_models sum_n |G(i w_n; eps_bath, v) - g_target|^2 with L-BFGS
+    (strong Wolfe line search), each loss and gradient from one batched
+    forward solve and one adjoint backward (``anderson_gf``).  The batch must
+    be created with ``keep_basis=True``.  ``eps0`` / ``v0`` ([M, max_bath])
+    default to the batch's current bath.  The models' losses are independent,
+    so the joint minimisation fits each model.  G(i w_n) constrains some
+    parameter combinations only weakly (the usual ill-conditioning of
+    Matsubara-axis bath fits), so a small loss does not pin every parameter.
+
+    The reference fits the bath of its DMFT loop with derivative-free scipy
+    optimisers; this is the gradient route that the adjoint backward enables.
+    """
+    import torch
+
+    if not batch.keep_basis:
+        raise ValueError("fit_bath needs a batch created with keep_basis=True")
+    wn = np.ascontiguousarray(wn, dtype=np.float64)
+    gt = torch.as_tensor(np.asarray(g_target, dtype=np.complex128))
+    if gt.shape != (batch.num_models, len(wn)):
+        raise ValueError(f"g_target must have shape ({batch.num_models}, {len(wn)})")
+    e = torch.tensor(batch.eps_bath if eps0 is None else np.asarray(eps0, dtype=np.float64), requires_grad=True)
+    v = torch.tensor(batch.v if v0 is None else np.asarray(v0, dtype=np.float64), requires_grad=True)
+    opt = torch.optim.LBFGS([e, v], lr=1.0, max_iter=max_iter, max_eval=2 * max_iter,
+                            tolerance_grad=0.0, tolerance_change=tolerance_change, history_size=history_size,
+                            line_search_fn="strong_wolfe")
+    hist = []
+
+    def closure():
+        opt.zero_grad()
+        g = anderson_gf(e, v, batch, wn)
+        loss = (g - gt).abs().pow(2).sum()
+        loss.backward()
+        hist.append(float(loss.detach()))
+        return loss
+
+    opt.step(closure)
+    with torch.no_grad():
+        g = anderson_gf(e.detach(), v.detach(), batch, wn)
+        per_model = (g - gt).abs().pow(2).sum(dim=1).numpy()
+    return BathFit(e.detach().numpy().copy(), v.detach().numpy().copy(), per_model, hist, len(hist))
