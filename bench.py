@@ -30,6 +30,12 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 
 
+# secondary workloads, in the order they run
+SECONDARY = {"chain12_solves": "bench_chain12", "chain12_realtime_g": "bench_tevo12",
+             "finite_t_lehmann": "bench_finite_t", "batched_anderson": "bench_batched",
+             "bathfit_gradients": "bench_bathfit"}
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -43,6 +49,8 @@ def parse():
     p.add_argument("--size", choices=["4x4", "4x3"], default="4x4")
     p.add_argument("--secondary-seconds", type=float, default=10.0,
                    help="CPU seconds per secondary workload's baseline sample")
+    p.add_argument("--only-secondary", choices=sorted(SECONDARY), default=None,
+                   help="run one secondary workload alone (no headline), e.g. for a profiler")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run device copies of one sector vector (the reference kernel for "
                         "scaling the FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -76,6 +84,10 @@ def main():
     import cmpy_amd
     from cmpy_amd import SectorHamiltonOperator, groundstate
 
+    if args.only_secondary:
+        name = args.only_secondary
+        print(json.dumps({"secondary": {name: globals()[SECONDARY[name]](args, log)}}), flush=True)
+        return
     ns, n, bonds = model_spec(args.size)
     if args.copy_calibration:
         from math import comb
@@ -205,11 +217,9 @@ def main():
 
     if not args.no_secondary:
         result["secondary"] = {}
-        for name, fn in (("chain12_solves", bench_chain12), ("chain12_realtime_g", bench_tevo12),
-                         ("finite_t_lehmann", bench_finite_t), ("batched_anderson", bench_batched),
-                         ("bathfit_gradients", bench_bathfit)):
+        for name, fn_name in SECONDARY.items():
             try:
-                result["secondary"][name] = fn(args, log)
+                result["secondary"][name] = globals()[fn_name](args, log)
             except Exception as exc:  # report, never hide the headline
                 result["secondary"][name] = {"error": repr(exc)}
 
