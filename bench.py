@@ -11,9 +11,9 @@ written once).  The same run also
   * times the CPU restatement (oracle C twin) of the same H.v on the host
     cores as the CPU baseline, and checks the GPU result against it,
   * runs the secondary workloads (12-site chain GS + CF G, 12-site chain
-    real-time G, finite-T Lehmann G of a 1+6-site impurity model, batched
-    Anderson solves, bath-fit gradients), each with its
-    CPU baseline, unless --no-secondary is given.
+    real-time G, finite-T Lehmann G of a 1+6-site impurity model, off-diagonal
+    G_ij of a 7-site ring, batched Anderson solves, bath-fit gradients), each
+    with its CPU baseline, unless --no-secondary is given.
 
 Prints ONE JSON line.
 """
@@ -32,8 +32,8 @@ ROOT = Path(__file__).resolve().parent
 
 # secondary workloads, in the order they run
 SECONDARY = {"chain12_solves": "bench_chain12", "chain12_realtime_g": "bench_tevo12",
-             "finite_t_lehmann": "bench_finite_t", "batched_anderson": "bench_batched",
-             "bathfit_gradients": "bench_bathfit"}
+             "finite_t_lehmann": "bench_finite_t", "offdiagonal_gij": "bench_gij",
+             "batched_anderson": "bench_batched", "bathfit_gradients": "bench_bathfit"}
 
 
 def parse():
@@ -459,6 +459,61 @@ def bench_finite_t(args, log):
             "sample": f"the oracle's gf_lehmann (numpy eigh + vectorised pair sum, 1 core) on every sector at 10 "
                       f"and 20 of the frequencies ({ts[10]:.1f} s, {ts[20]:.1f} s), extrapolated linearly to "
                       f"{len(z)}"}
+    return out
+
+
+def bench_gij(args, log):
+    """Off-diagonal finite-T Green's functions of a 7-site Hubbard ring (U = 4,
+    mu = U/2, beta = 10): G_0j for j = 0..3 on 1000 frequencies in [-8, 8] +
+    0.05i, every sector diagonalised once and shared (lehmann_full.py:19-97)."""
+    import time
+
+    import numpy as np
+    import torch
+
+    from cmpy_amd.exactdiag import greens_lehmann
+    from cmpy_amd.models import HubbardModel, chain_neighbors
+
+    ns, U, beta = 7, 4.0, 10.0
+    z = np.linspace(-8, 8, 1000) + 0.05j
+    model = HubbardModel(ns, chain_neighbors(ns, periodic=True), inter=U, eps=0.0, hop=1.0, mu=U / 2)
+    greens_lehmann(model, z[:8], beta, 0, 1)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cache = {}
+    gs = [greens_lehmann(model, z, beta, 0, j, eig_cache=cache).gf for j in range(4)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"metric": "solves/s", "value": round(1.0 / dt, 4), "unit": "solves/s", "seconds": round(dt, 4),
+           "sites": ns, "elements": 4, "beta": beta, "frequencies": len(z), "max_dim": 1225,
+           "data": "deterministic model"}
+    log(f"G_ij (7-site ring, 4 elements): {dt:.3f} s per solve")
+    if not args.no_cpu:
+        import sys
+
+        sys.path.insert(0, str(ROOT))
+        from oracle import ed_oracle as O
+
+        bonds = O.chain_bonds(ns, periodic=True)
+        mk = lambda nu, nd: O.hubbard_operator(ns, nu, nd, bonds, U, 0.0, 1.0, mu=U / 2)
+        ts = {}
+        with _cpu_single_thread():
+            for nz in (10, 20):
+                zs = z[:: len(z) // nz][:nz]
+                t0 = time.perf_counter()
+                ref = O.lehmann_ij(mk, ns, zs, beta, 0, 1)
+                ts[nz] = time.perf_counter() - t0
+        per_z = max(ts[20] - ts[10], 0.0) / 10
+        fixed = max(ts[10] - 10 * per_z, 0.0)
+        # one diagonalisation of every sector, four elements' sums
+        est = fixed + 4 * per_z * len(z)
+        out["parity_sample"] = {"max_abs_diff": float(np.abs(gs[1][:: len(z) // 20][:20] - ref).max()),
+                                "max_abs": float(np.abs(ref).max())}
+        out["cpu_baseline"] = {
+            "value": round(1.0 / est, 5), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"the oracle's lehmann_ij (numpy eigh + pair sums, 1 core) for G_01 on 10 and 20 of the "
+                      f"frequencies ({ts[10]:.1f} s, {ts[20]:.1f} s); estimate = fixed part (eigh, overlaps) once "
+                      f"+ 4 elements x {len(z)} frequencies"}
     return out
 
 
