@@ -176,9 +176,12 @@ __global__ __launch_bounds__(kVecThreads) void k_resid(const double* __restrict_
 }
 
 // Block (c, i): partial of <V_i, w> over chunk c.
+// gate (optional): skip the whole launch unless *gate != 0 (partial
+// reorthogonalisation decides on the device whether a step needs it).
 __global__ __launch_bounds__(kVecThreads) void k_multidot(const double* __restrict__ V, int64_t D,
                                                           const double* __restrict__ w, int64_t chunk,
-                                                          double* partial, int nchunks) {
+                                                          double* partial, int nchunks, const int* gate) {
+  if (gate && *gate == 0) return;
   __shared__ double sh[kVecThreads / 64];
   const int c = blockIdx.x, i = blockIdx.y;
   const double* vi = V + (int64_t)i * D;
@@ -190,7 +193,8 @@ __global__ __launch_bounds__(kVecThreads) void k_multidot(const double* __restri
 }
 
 __global__ __launch_bounds__(kVecThreads) void k_reduce_rows(const double* __restrict__ partial, int nchunks,
-                                                             double* out) {
+                                                             double* out, const int* gate) {
+  if (gate && *gate == 0) return;
   __shared__ double sh[kVecThreads / 64];
   double acc = 0.0;
   for (int c = threadIdx.x; c < nchunks; c += blockDim.x) acc += partial[(int64_t)blockIdx.x * nchunks + c];
@@ -201,7 +205,8 @@ __global__ __launch_bounds__(kVecThreads) void k_reduce_rows(const double* __res
 // w -= sum_i coef_i V_i   (sign = -1)   or   w = sum_i coef_i V_i (sign = 0)
 __global__ __launch_bounds__(kVecThreads) void k_axpy_multi(const double* __restrict__ V, int nv, int64_t D,
                                                             const double* __restrict__ coef,
-                                                            double* __restrict__ w, int assign) {
+                                                            double* __restrict__ w, int assign, const int* gate) {
+  if (gate && *gate == 0) return;
   extern __shared__ double cs[];
   for (int i = threadIdx.x; i < nv; i += blockDim.x) cs[i] = coef[i];
   __syncthreads();
@@ -461,11 +466,80 @@ static int reorth_start(const LzLayout& L, LzCoef c, double* part, const ReorthC
   return ED_OK;
 }
 
+// Partial reorthogonalisation (H. D. Simon, Math. Comp. 42 (1984) 115-142):
+// omega[j][k] estimates <v_j, v_k> through the recurrence the Lanczos
+// vectors themselves obey,
+//   beta_{j+1} w_{j+1,k} = beta_{k+1} w_{j,k+1} + (alpha_k - alpha_j) w_{j,k}
+//                          + beta_k w_{j,k-1} - beta_j w_{j-1,k} (+ rounding term),
+// with w_{k,k} = 1 and w_{k+1,k} = eps.  When max_k |w_{j+1,k}| passes
+// sqrt(eps) the new vector is orthogonalised against all previous ones (and
+// so is the next one), and its row is reset to eps.  Rows live in om[3][M+2]
+// (row r at (r % 3)); pro[0] = orthogonalise this step, pro[1] = the next
+// step must too; bt = |w| before orthogonalisation.
+struct ProState {
+  double* om;   // [3][M + 2]
+  int* flags;   // [2]
+  double* bt;   // [1]
+  int M;
+};
+
+__global__ __launch_bounds__(256) void k_pro_init(ProState p) {
+  for (int i = threadIdx.x; i < 3 * (p.M + 2); i += blockDim.x) p.om[i] = 0.0;
+  if (threadIdx.x == 0) {
+    p.om[0] = 1.0;  // w_{0,0}
+    p.flags[0] = 0;
+    p.flags[1] = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pro_omega(ProState p, const double* __restrict__ alpha,
+                                                   const double* __restrict__ beta, int j) {
+  __shared__ double sh[256 / 64];
+  __shared__ int decide;
+  const int W = p.M + 2;
+  const double* cur = p.om + (int64_t)(j % 3) * W;
+  const double* prev = p.om + (int64_t)((j + 2) % 3) * W;
+  double* nxt = p.om + (int64_t)((j + 1) % 3) * W;
+  const double eps = 2.220446049250313e-16;
+  const double bt = *p.bt;
+  double mx = 0.0;
+  if (bt > 0.0) {
+    for (int k = threadIdx.x; k < j; k += blockDim.x) {
+      double t = beta[k + 1] * cur[k + 1] + (alpha[k] - alpha[j]) * cur[k] - beta[j] * prev[k];
+      if (k > 0) t += beta[k] * cur[k - 1];
+      t += (t >= 0.0 ? 1.0 : -1.0) * eps * (beta[k + 1] + bt) * 0.3;
+      const double o = t / bt;
+      nxt[k] = o;
+      mx = fmax(mx, fabs(o));
+    }
+  }
+  // block maximum
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < 4; ++w) m = fmax(m, sh[w]);
+    const int force = p.flags[1];
+    const int re = bt > 0.0 && (m > 1.4901161193847656e-08 || force);
+    p.flags[0] = re;
+    p.flags[1] = re && !force;
+    decide = re;
+    nxt[j] = eps;
+    nxt[j + 1] = 1.0;
+  }
+  __syncthreads();
+  if (decide)
+    for (int k = threadIdx.x; k < j; k += blockDim.x) nxt[k] = eps;
+}
+
 // Step j: w = H v_j - beta_j v_{j-1}; alpha_j = <w, v_j>; w -= alpha_j v_j;
-// two classical Gram-Schmidt passes against v_0..v_j; beta_{j+1} = |w|;
-// v_{j+1} = w / beta_{j+1}.
+// two classical Gram-Schmidt passes against v_0..v_j (every step, or, with
+// a ProState, only where the orthogonality estimate asks for them);
+// beta_{j+1} = |w|; v_{j+1} = w / beta_{j+1}.
 static int reorth_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, const ReorthCtx& r, int j,
-                       hipStream_t s, EventTimer* tm) {
+                       hipStream_t s, EventTimer* tm, const ProState* pro = nullptr) {
   const int64_t D = L.D;
   const int vb = vec_blocks(D);
   double* vj = r.V + (int64_t)j * D;
@@ -486,13 +560,20 @@ static int reorth_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, con
   }
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, np, (int)RED_SUM_TO, c, j, c.alpha + j);
   hipLaunchKernelGGL(k_update, dim3(vb), dim3(kVecThreads), 0, s, r.w, vj, D, c.alpha + j, c.misc + 1, nullptr, 0.0,
-                     nullptr);
+                     pro ? part : nullptr);
+  const int* gate = nullptr;
+  if (pro) {
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_SQRT_TO, c, 0, pro->bt);
+    hipLaunchKernelGGL(k_pro_omega, dim3(1), dim3(256), 0, s, *pro, c.alpha, c.beta, j);
+    gate = pro->flags;
+  }
   for (int pass = 0; pass < 2; ++pass) {
     hipLaunchKernelGGL(k_multidot, dim3((unsigned)r.chunks, (unsigned)(j + 1)), dim3(kVecThreads), 0, s, r.V, D, r.w,
-                       r.chunk, part, (int)r.chunks);
-    hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)(j + 1)), dim3(kVecThreads), 0, s, part, (int)r.chunks, r.coefs);
+                       r.chunk, part, (int)r.chunks, gate);
+    hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)(j + 1)), dim3(kVecThreads), 0, s, part, (int)r.chunks, r.coefs,
+                       gate);
     hipLaunchKernelGGL(k_axpy_multi, dim3(vb), dim3(kVecThreads), sizeof(double) * (j + 1), s, r.V, j + 1, D,
-                       r.coefs, r.w, 0);
+                       r.coefs, r.w, 0, gate);
   }
   hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, r.w, r.w, D, c.misc + 1, 0, part);
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_BETA, c, j, nullptr);
@@ -649,7 +730,7 @@ extern "C" int ed_lanczos_groundstate(ed_op* op, const ed_lanczos_cfg* cfg, cons
     if (cfg->want_vector) {
       ED_HIP(hipMemcpyAsync(coefs, y.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
       hipLaunchKernelGGL(k_axpy_multi, dim3(vb), dim3(kVecThreads), sizeof(double) * m, s, V, m, D, coefs,
-                         vec_ptr(base, L, M + 1), 1);
+                         vec_ptr(base, L, M + 1), 1, nullptr);
       ED_HIP(hipGetLastError());
     }
   }
@@ -689,6 +770,7 @@ extern "C" int ed_lanczos_coeffs_workspace(const ed_op* op, int32_t nsteps, int3
   clear_error();
   ED_REQUIRE(op && bytes, "NULL argument");
   ED_REQUIRE(nsteps >= 1 && nsteps <= 100000, "nsteps=%d outside [1, 100000]", nsteps);
+  ED_REQUIRE(reorth >= 0 && reorth <= 2, "reorth must be 0 (none), 1 (full) or 2 (partial)");
   ED_REQUIRE(!reorth || nsteps <= 4096, "reorth supports nsteps <= 4096 (got %d)", nsteps);
   *bytes = lz_layout(op, nsteps, reorth ? 1 : 0, 0, 0).total;
   return ED_OK;
@@ -700,6 +782,7 @@ extern "C" int ed_lanczos_coeffs(ed_op* op, const double* v0_dev, int32_t nsteps
   clear_error();
   ED_REQUIRE(op && v0_dev && alpha && beta && norm2 && steps_out && work_dev, "NULL argument");
   ED_REQUIRE(nsteps >= 1 && nsteps <= 100000, "nsteps=%d outside [1, 100000]", nsteps);
+  ED_REQUIRE(reorth >= 0 && reorth <= 2, "reorth must be 0 (none), 1 (full) or 2 (partial)");
   ED_REQUIRE(!reorth || nsteps <= 4096, "reorth supports nsteps <= 4096 (got %d)", nsteps);
   const int M = nsteps;
   const LzLayout L = lz_layout(op, M, reorth ? 1 : 0, 0, 0);
@@ -715,8 +798,28 @@ extern "C" int ed_lanczos_coeffs(ed_op* op, const double* v0_dev, int32_t nsteps
   if (reorth) {
     const ReorthCtx rc = reorth_ctx(base, L, part);
     ED_TRY(reorth_start(L, c, part, rc, v0_dev, 0, s));
-    for (int j = 0; j < M; ++j) ED_TRY(reorth_step(op, L, c, part, rc, j, s, nullptr));
-    if (basis_dev) ED_HIP(hipMemcpyAsync(basis_dev, rc.V, sizeof(double) * D * M, hipMemcpyDeviceToDevice, s));
+    ProState pro{nullptr, nullptr, nullptr, M};
+    double* pro_buf = nullptr;
+    if (reorth == 2) {
+      // omega rows, the pre-orthogonalisation norm and the two flags
+      ED_HIP(hipMalloc(&pro_buf, sizeof(double) * (3 * (size_t)(M + 2) + 2)));
+      pro.om = pro_buf;
+      pro.bt = pro_buf + 3 * (size_t)(M + 2);
+      pro.flags = reinterpret_cast<int*>(pro_buf + 3 * (size_t)(M + 2) + 1);
+      hipLaunchKernelGGL(k_pro_init, dim3(1), dim3(256), 0, s, pro);
+    }
+    int rc_step = ED_OK;
+    for (int j = 0; j < M && rc_step == ED_OK; ++j)
+      rc_step = reorth_step(op, L, c, part, rc, j, s, nullptr, reorth == 2 ? &pro : nullptr);
+    if (rc_step == ED_OK && basis_dev)
+      rc_step = hipMemcpyAsync(basis_dev, rc.V, sizeof(double) * D * M, hipMemcpyDeviceToDevice, s) == hipSuccess
+                    ? ED_OK
+                    : set_error(ED_ERR_HIP, "copying the Krylov basis failed");
+    if (pro_buf) {
+      if (hipStreamSynchronize(s) != hipSuccess && rc_step == ED_OK) rc_step = set_error(ED_ERR_HIP, "stream sync failed");
+      (void)hipFree(pro_buf);
+    }
+    if (rc_step != ED_OK) return rc_step;
   } else {
     double* u[3] = {vec_ptr(base, L, 0), vec_ptr(base, L, 1), vec_ptr(base, L, 2)};
     hipLaunchKernelGGL(k_start, dim3(vb), dim3(kVecThreads), 0, s, u[0], v0_dev, D, (uint64_t)0, part);

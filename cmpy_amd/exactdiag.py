@@ -481,32 +481,53 @@ class CFResult:
 REORTH_MAX_BYTES = 16 << 30  # Krylov basis kept for full reorthogonalisation
 
 
+PARTIAL_REORTH_MIN_SIZE = 50_000  # below this, full reorthogonalisation is as cheap
+
+
 def _default_reorth(size: int, nsteps: int) -> bool:
     return nsteps <= 4096 and 8 * size * (nsteps + 2) <= REORTH_MAX_BYTES
 
 
-def _cf_coeffs(op: SectorHamiltonOperator, phi: torch.Tensor, nsteps: int, reorth: Optional[bool] = None):
-    nsteps = int(max(1, min(nsteps, op.size)))
+def _reorth_mode(reorth, size: int, nsteps: int) -> int:
+    """0 none, 1 full (two Gram-Schmidt passes every step), 2 partial (only
+    where Simon's orthogonality estimate passes sqrt(eps)).  ``None``: partial
+    from PARTIAL_REORTH_MIN_SIZE states, full below, none when the Krylov basis
+    would exceed REORTH_MAX_BYTES."""
     if reorth is None:
-        reorth = _default_reorth(op.size, nsteps)
+        if not _default_reorth(size, nsteps):
+            return 0
+        return 2 if size >= PARTIAL_REORTH_MIN_SIZE else 1
+    if isinstance(reorth, str):
+        modes = {"none": 0, "full": 1, "partial": 2}
+        if reorth not in modes:
+            raise ValueError("reorth must be None, a bool, or one of 'none', 'full', 'partial'")
+        return modes[reorth]
+    return 1 if reorth else 0
+
+
+def _cf_coeffs(op: SectorHamiltonOperator, phi: torch.Tensor, nsteps: int, reorth=None):
+    nsteps = int(max(1, min(nsteps, op.size)))
+    mode = _reorth_mode(reorth, op.size, nsteps)
     nbytes = C.c_int64()
-    _lib.call("ed_lanczos_coeffs_workspace", op.handle, nsteps, int(bool(reorth)), C.byref(nbytes))
+    _lib.call("ed_lanczos_coeffs_workspace", op.handle, nsteps, mode, C.byref(nbytes))
     work = torch.empty(nbytes.value, dtype=torch.uint8, device="cuda")
     a = np.zeros(nsteps)
     b = np.zeros(max(nsteps, 1))
     n2 = C.c_double()
     m = C.c_int32()
-    _lib.call("ed_lanczos_coeffs", op.handle, _lib.dptr(phi), nsteps, int(bool(reorth)),
+    _lib.call("ed_lanczos_coeffs", op.handle, _lib.dptr(phi), nsteps, mode,
               a.ctypes.data_as(C.POINTER(C.c_double)), b.ctypes.data_as(C.POINTER(C.c_double)),
               C.byref(n2), C.byref(m), None, _lib.dptr(work), nbytes.value, _lib.stream_ptr())
     return a[: m.value].copy(), b[: max(m.value - 1, 0)].copy(), float(n2.value)
 
 
 def greens_cf(model, z, n_up, n_dn, pos=0, sigma=UP, nsteps=200, gs: EigenState = None, tol=1e-10,
-              sign_mode=None, reorth: Optional[bool] = None) -> CFResult:
+              sign_mode=None, reorth=None) -> CFResult:
     """T = 0 continued-fraction G from ``nsteps`` Lanczos steps started at c+psi0
-    (particle, sector N+1) and c psi0 (hole, sector N-1).  ``reorth`` defaults
-    to full reorthogonalisation whenever the Krylov basis fits in 16 GiB."""
+    (particle, sector N+1) and c psi0 (hole, sector N-1).  ``reorth`` (None,
+    bool, 'none', 'full' or 'partial') defaults to partial reorthogonalisation
+    from 50,000 states and full below, whenever the Krylov basis fits in
+    16 GiB (see ``_reorth_mode``)."""
     ns = model.num_sites
     sign_mode = _sign_mode(model, sign_mode)
     e0, psi = _gs(model, n_up, n_dn, gs, tol)

@@ -14,8 +14,8 @@
  *     library owns what sits behind an ed_op handle (state lists, hop tables,
  *     diagonal tables) or an ed_batch handle, freed by ed_op_destroy /
  *     ed_batch_destroy; a few calls (continued fraction, time sum, pair sums,
- *     dense eigensolvers, occupations) allocate scratch and free it before
- *     returning.
+ *     dense eigensolvers, occupations, partially reorthogonalised Lanczos
+ *     coefficients) allocate scratch and free it before returning.
  *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
  *     Device work is enqueued on it; functions that return host results
  *     (ed_lanczos_*, ed_batch_*) synchronise that stream before returning.
@@ -192,8 +192,12 @@ int ed_lanczos_groundstate(ed_op* op, const ed_lanczos_cfg* cfg, const double* v
  * *norm2 = |v0|^2, *steps = m <= nsteps (fewer on breakdown).  reorth = 1
  * orthogonalises every new vector against the whole basis (two Gram-Schmidt
  * passes; nsteps <= 4096), which keeps real-axis continued fractions exact up
- * to the Krylov dimension.  Optionally the normalised basis vectors are
- * written to basis_dev ([nsteps][D]).                                       */
+ * to the Krylov dimension; reorth = 2 does so only at the steps where Simon's
+ * orthogonality estimate passes sqrt(eps) (partial reorthogonalisation: the
+ * basis stays semi-orthogonal, the coefficients as accurate as their own
+ * conditioning allows; a small scratch buffer is allocated for the estimate).
+ * Optionally the normalised basis vectors are written to basis_dev
+ * ([nsteps][D]).                                                            */
 int ed_lanczos_coeffs_workspace(const ed_op* op, int32_t nsteps, int32_t reorth, int64_t* bytes);
 int ed_lanczos_coeffs(ed_op* op, const double* v0_dev, int32_t nsteps, int32_t reorth, double* alpha,
                       double* beta, double* norm2, int32_t* steps, double* basis_dev,

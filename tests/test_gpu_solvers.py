@@ -178,6 +178,41 @@ def test_cf_equals_lehmann(gpu, seed):
                     greens_lehmann_t0(model, iw, 3, 3, gs=gs), atol=G_ATOL)
 
 
+def test_cf_partial_reorth_matches_full(gpu):
+    """Partial reorthogonalisation (the default from 50,000 states) against
+    full reorthogonalisation on the 10-site chain, (6,5) sector (D = 52,920).
+    A 200-step continued fraction near the real axis is only as well defined
+    as its conditioning: perturbing the start vector by 1e-15 moves G at
+    eta = 0.05 by ~1e-8 at U = 2, while G at eta = 1 stays put.  So the test
+    holds eta = 1 to 1e-12 and eta = 0.05 to 1e-7."""
+    from cmpy_amd import HubbardModel, chain_neighbors, greens_cf, groundstate
+
+    m = HubbardModel(10, chain_neighbors(10), inter=2.0, eps=0.0, hop=1.0)
+    gs = groundstate(m, 5, 5, tol=1e-12)
+    for eta, tol in ((1.0, 1e-12), (0.05, 1e-7)):
+        z = np.linspace(-10, 10, 2048) + 1j * eta
+        gp = greens_cf(m, z, 5, 5, nsteps=200, gs=gs, reorth="partial").gf
+        gf = greens_cf(m, z, 5, 5, nsteps=200, gs=gs, reorth="full").gf
+        assert np.abs(gp - gf).max() <= tol, (eta, np.abs(gp - gf).max())
+
+
+@pytest.mark.parametrize("seed", [1, 6])
+def test_cf_partial_reorth_equals_lehmann_small(gpu, seed):
+    """Partial reorthogonalisation run past the Krylov dimension still
+    reproduces the exact (Lehmann) G of a small sector."""
+    from cmpy_amd import greens_cf, groundstate
+
+    model, mk = _siam(seed)
+    gs = groundstate(model, 3, 3, tol=1e-12)
+    z = np.linspace(-8, 8, 1000) + 0.05j
+    g = greens_cf(model, z, 3, 3, nsteps=400, gs=gs, reorth="partial").gf
+    e0, psi = O.groundstate(mk(3, 3))
+    cd = O.apply_cdag(psi, 6, 3, 3, 0, O.UP)
+    cc = O.apply_c(psi, 6, 3, 3, 0, O.UP)
+    g_ref = O.lehmann_t0(mk(3, 3), mk(4, 3), mk(2, 3), psi, e0, cd, cc, z)
+    assert_allclose(g, g_ref, rtol=0, atol=G_ATOL)
+
+
 def test_cfg2_cf_10site_vs_oracle(gpu):
     """CF at a size the oracle finishes in seconds: 10-site chain, (5,5), U=4."""
     from cmpy_amd import HubbardModel, chain_neighbors, greens_cf, groundstate
