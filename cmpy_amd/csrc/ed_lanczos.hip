@@ -534,6 +534,28 @@ __global__ __launch_bounds__(256) void k_pro_omega(ProState p, const double* __r
     for (int k = threadIdx.x; k < j; k += blockDim.x) nxt[k] = eps;
 }
 
+// Owner of a ProState's device buffer: waits for the stream, then frees.
+struct ProBuffer {
+  double* p;
+  hipStream_t s;
+  ~ProBuffer() {
+    if (p) {
+      (void)hipStreamSynchronize(s);
+      (void)hipFree(p);
+    }
+  }
+};
+
+static int pro_alloc(ProState& pro, ProBuffer& buf, hipStream_t s) {
+  ED_HIP(hipMalloc(&buf.p, sizeof(double) * (3 * (size_t)(pro.M + 2) + 2)));
+  pro.om = buf.p;
+  pro.bt = buf.p + 3 * (size_t)(pro.M + 2);
+  pro.flags = reinterpret_cast<int*>(buf.p + 3 * (size_t)(pro.M + 2) + 1);
+  hipLaunchKernelGGL(k_pro_init, dim3(1), dim3(256), 0, s, pro);
+  ED_HIP(hipGetLastError());
+  return ED_OK;
+}
+
 // Step j: w = H v_j - beta_j v_{j-1}; alpha_j = <w, v_j>; w -= alpha_j v_j;
 // two classical Gram-Schmidt passes against v_0..v_j (every step, or, with
 // a ProState, only where the orthogonality estimate asks for them);
@@ -589,6 +611,7 @@ static int validate_cfg(const ed_lanczos_cfg* cfg) {
   ED_REQUIRE(cfg->check_every >= 1, "check_every must be >= 1");
   ED_REQUIRE(cfg->tol >= 0.0 && std::isfinite(cfg->tol), "tol must be finite and >= 0");
   // the reorthogonalisation kernels stage one coefficient per basis vector in LDS
+  ED_REQUIRE(cfg->reorth >= 0 && cfg->reorth <= 2, "reorth must be 0 (none), 1 (full) or 2 (partial)");
   ED_REQUIRE(!cfg->reorth || cfg->max_iter <= 4096, "reorth supports max_iter <= 4096 (got %d)", cfg->max_iter);
   return ED_OK;
 }
@@ -707,14 +730,19 @@ extern "C" int ed_lanczos_groundstate(ed_op* op, const ed_lanczos_cfg* cfg, cons
       ED_HIP(hipGetLastError());
     }
   } else {
-    // full reorthogonalisation: V holds normalised basis vectors
+    // full or partial reorthogonalisation: V holds normalised basis vectors
     const ReorthCtx rc = reorth_ctx(base, L, part);
     double* V = rc.V;
     double* coefs = rc.coefs;
     ED_TRY(reorth_start(L, c, part, rc, v0_dev, cfg->seed, s));
+    ProState pro{nullptr, nullptr, nullptr, M};
+    ProBuffer pro_buf{nullptr, s};
+    if (cfg->reorth == 2) {
+      ED_TRY(pro_alloc(pro, pro_buf, s));
+    }
     int steps = 0;
     for (int j = 0; j < M; ++j) {
-      ED_TRY(reorth_step(op, L, c, part, rc, j, s, &tm));
+      ED_TRY(reorth_step(op, L, c, part, rc, j, s, &tm, cfg->reorth == 2 ? &pro : nullptr));
       steps = j + 1;
       const bool last = (j + 1 == M);
       const bool due = !cfg->fixed_iter && steps >= cfg->min_iter && (steps % cfg->check_every == 0);
@@ -799,27 +827,10 @@ extern "C" int ed_lanczos_coeffs(ed_op* op, const double* v0_dev, int32_t nsteps
     const ReorthCtx rc = reorth_ctx(base, L, part);
     ED_TRY(reorth_start(L, c, part, rc, v0_dev, 0, s));
     ProState pro{nullptr, nullptr, nullptr, M};
-    double* pro_buf = nullptr;
-    if (reorth == 2) {
-      // omega rows, the pre-orthogonalisation norm and the two flags
-      ED_HIP(hipMalloc(&pro_buf, sizeof(double) * (3 * (size_t)(M + 2) + 2)));
-      pro.om = pro_buf;
-      pro.bt = pro_buf + 3 * (size_t)(M + 2);
-      pro.flags = reinterpret_cast<int*>(pro_buf + 3 * (size_t)(M + 2) + 1);
-      hipLaunchKernelGGL(k_pro_init, dim3(1), dim3(256), 0, s, pro);
-    }
-    int rc_step = ED_OK;
-    for (int j = 0; j < M && rc_step == ED_OK; ++j)
-      rc_step = reorth_step(op, L, c, part, rc, j, s, nullptr, reorth == 2 ? &pro : nullptr);
-    if (rc_step == ED_OK && basis_dev)
-      rc_step = hipMemcpyAsync(basis_dev, rc.V, sizeof(double) * D * M, hipMemcpyDeviceToDevice, s) == hipSuccess
-                    ? ED_OK
-                    : set_error(ED_ERR_HIP, "copying the Krylov basis failed");
-    if (pro_buf) {
-      if (hipStreamSynchronize(s) != hipSuccess && rc_step == ED_OK) rc_step = set_error(ED_ERR_HIP, "stream sync failed");
-      (void)hipFree(pro_buf);
-    }
-    if (rc_step != ED_OK) return rc_step;
+    ProBuffer pro_buf{nullptr, s};
+    if (reorth == 2) ED_TRY(pro_alloc(pro, pro_buf, s));
+    for (int j = 0; j < M; ++j) ED_TRY(reorth_step(op, L, c, part, rc, j, s, nullptr, reorth == 2 ? &pro : nullptr));
+    if (basis_dev) ED_HIP(hipMemcpyAsync(basis_dev, rc.V, sizeof(double) * D * M, hipMemcpyDeviceToDevice, s));
   } else {
     double* u[3] = {vec_ptr(base, L, 0), vec_ptr(base, L, 1), vec_ptr(base, L, 2)};
     hipLaunchKernelGGL(k_start, dim3(vb), dim3(kVecThreads), 0, s, u[0], v0_dev, D, (uint64_t)0, part);

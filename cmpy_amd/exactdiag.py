@@ -79,8 +79,9 @@ def groundstate(model_or_op, n_up: int = None, n_dn: int = None, *, tol: float =
     """Lowest eigenpair of one sector by device Lanczos.
 
     Converged when the Ritz residual |H psi - E psi| <= tol (absolute, psi
-    normalised).  ``reorth`` defaults to full reorthogonalisation for sectors
-    up to 2e5 states and to the three-vector recurrence above.
+    normalised).  ``reorth`` (None, bool, 'none', 'full', 'partial') defaults
+    to full reorthogonalisation up to 2e5 states, partial above while the
+    Krylov basis fits in 16 GiB, and the three-vector recurrence beyond.
     Returns ``EigenState(energy, state, n_up, n_dn)``; ``state`` is a numpy array,
     or a CUDA tensor when ``device=True``.
     """
@@ -88,10 +89,14 @@ def groundstate(model_or_op, n_up: int = None, n_dn: int = None, *, tol: float =
     D = op.size
     if D == 0:
         raise ValueError("empty sector")
-    if reorth is None:
-        reorth = D <= 200_000
     mi = int(min(max_iter, D))
-    cfg = _lib.LanczosCfg(mi, int(min_iter), int(check_every), int(bool(reorth)), int(bool(want_vector)),
+    if reorth is None:
+        # full up to 200,000 states; above, partial while the basis fits
+        # (one pass instead of two for the vector), else the plain recurrence
+        mode = 1 if D <= 200_000 else (2 if _default_reorth(D, mi) else 0)
+    else:
+        mode = _reorth_mode(reorth, D, mi)
+    cfg = _lib.LanczosCfg(mi, int(min_iter), int(check_every), mode, int(bool(want_vector)),
                           int(bool(fixed_iter)), float(tol), int(seed) & 0xFFFFFFFFFFFFFFFF)
     nbytes = C.c_int64()
     _lib.call("ed_lanczos_workspace", op.handle, C.byref(cfg), C.byref(nbytes))

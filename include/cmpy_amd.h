@@ -161,7 +161,10 @@ typedef struct ed_lanczos_cfg {
   int32_t max_iter;     /* maximum Lanczos steps                              */
   int32_t min_iter;     /* no convergence test before this many steps         */
   int32_t check_every;  /* steps between convergence tests                    */
-  int32_t reorth;       /* 0: three-vector recurrence; 1: full reorthogonalisation */
+  int32_t reorth;       /* 0: three-vector recurrence (two passes for the vector);
+                           1: full reorthogonalisation; 2: partial (Simon's
+                           estimate, Gram-Schmidt only where it passes sqrt(eps));
+                           1 and 2 keep the basis, so the vector takes one pass */
   int32_t want_vector;  /* 1: also return the ground-state vector             */
   int32_t fixed_iter;   /* 1: run exactly max_iter steps (no early stop)      */
   double tol;           /* stop when the Ritz residual |H psi - E psi| <= tol  */

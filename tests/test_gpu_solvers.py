@@ -83,6 +83,23 @@ def test_cfg2_groundstate_12site_chain(gpu, u):
         assert abs(gs.energy - w[0]) <= E0_RTOL * abs(w[0])
 
 
+@pytest.mark.parametrize("u", [2.0, 8.0])
+def test_groundstate_12site_chain_partial_reorth(gpu, u):
+    """The default for this size: partial reorthogonalisation with the basis
+    kept, so psi is formed in one pass.  Same checks as the three-vector
+    recurrence: Ritz and explicit residuals, and the oracle operator's."""
+    from cmpy_amd import HubbardModel, chain_neighbors, groundstate
+
+    m = HubbardModel(12, chain_neighbors(12), inter=u, eps=0.0, hop=1.0)
+    gs, st = groundstate(m, 6, 6, tol=1e-10, return_stats=True, seed=7)
+    assert st.converged and st.residual <= 1e-10, st
+    ref = O.hubbard_operator(12, 6, 6, O.chain_bonds(12), u, 0.0, 1.0)
+    psi = gs.state / np.linalg.norm(gs.state)
+    assert np.linalg.norm(ref.matvec(psi) - gs.energy * psi) <= 2e-10
+    gs3 = groundstate(m, 6, 6, tol=1e-10, reorth=False, seed=7)
+    assert abs(gs.energy - gs3.energy) <= E0_RTOL * abs(gs3.energy)
+
+
 def test_groundstate_4x4_one_up_six_dn(gpu):
     """4x4 torus, sector (1, 6), D = 128,128: the Lanczos epilogue through the
     two-pass sweep with the half-width row tables (8008 dn states)."""
