@@ -175,21 +175,26 @@ __global__ __launch_bounds__(kVecThreads) void k_resid(const double* __restrict_
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// Block (c, i): partial of <V_i, w> over chunk c.
+// Blocks (c, i0): partials of <V_i, w> over chunk c for i = i0, i0 +
+// gridDim.y, ... < nv (a bounded grid, so a gated-off launch is cheap).
 // gate (optional): skip the whole launch unless *gate != 0 (partial
 // reorthogonalisation decides on the device whether a step needs it).
+constexpr int kMultidotRows = 32;
 __global__ __launch_bounds__(kVecThreads) void k_multidot(const double* __restrict__ V, int64_t D,
                                                           const double* __restrict__ w, int64_t chunk,
-                                                          double* partial, int nchunks, const int* gate) {
+                                                          double* partial, int nchunks, int nv, const int* gate) {
   if (gate && *gate == 0) return;
   __shared__ double sh[kVecThreads / 64];
-  const int c = blockIdx.x, i = blockIdx.y;
-  const double* vi = V + (int64_t)i * D;
+  const int c = blockIdx.x;
   const int64_t k0 = (int64_t)c * chunk, k1 = std::min<int64_t>(D, k0 + chunk);
-  double acc = 0.0;
-  for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) acc += vi[k] * w[k];
-  const double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) partial[(int64_t)i * nchunks + c] = s;
+  for (int i = blockIdx.y; i < nv; i += gridDim.y) {
+    const double* vi = V + (int64_t)i * D;
+    double acc = 0.0;
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) acc += vi[k] * w[k];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[(int64_t)i * nchunks + c] = s;
+    __syncthreads();  // sh is reused by the next vector's sum
+  }
 }
 
 __global__ __launch_bounds__(kVecThreads) void k_reduce_rows(const double* __restrict__ partial, int nchunks,
@@ -590,8 +595,8 @@ static int reorth_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, con
     gate = pro->flags;
   }
   for (int pass = 0; pass < 2; ++pass) {
-    hipLaunchKernelGGL(k_multidot, dim3((unsigned)r.chunks, (unsigned)(j + 1)), dim3(kVecThreads), 0, s, r.V, D, r.w,
-                       r.chunk, part, (int)r.chunks, gate);
+    hipLaunchKernelGGL(k_multidot, dim3((unsigned)r.chunks, (unsigned)std::min(j + 1, kMultidotRows)),
+                       dim3(kVecThreads), 0, s, r.V, D, r.w, r.chunk, part, (int)r.chunks, j + 1, gate);
     hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)(j + 1)), dim3(kVecThreads), 0, s, part, (int)r.chunks, r.coefs,
                        gate);
     hipLaunchKernelGGL(k_axpy_multi, dim3(vb), dim3(kVecThreads), sizeof(double) * (j + 1), s, r.V, j + 1, D,

@@ -1195,6 +1195,8 @@ static ColKernel col_kernel_split(int tw, int maxl) {
   return col_kernel_split_tw<32>(maxl);
 }
 
+constexpr int64_t kMinRowBlocks = 512;
+
 // Two-pass plan.  Kernels: sign-split tables when hopping is uniform and the
 // tables exist (CMPY_AMD_HV_SPLIT=0 turns them off), general packed tables
 // otherwise.  CMPY_AMD_HV_MODE: 0 single fused pass, 1 two passes, unset =
@@ -1221,6 +1223,9 @@ static TwoPassPlan plan_two_pass(const ed_op* op) {
   int R = (int)std::min<size_t>(budget / row_bytes, 32);
   if (R < 1) return p;
   R = (int)std::min<int64_t>(R, op->num_up);
+  // keep at least ~2 row blocks per CU: short rows (small num_dn) would
+  // otherwise pack so many rows per block that most of the chip idles
+  R = (int)std::max<int64_t>(1, std::min<int64_t>(R, op->num_up / kMinRowBlocks));
   p.rows_per_block = R;
   // one-row blocks with the half-width tables where at most two rows fit
   const bool half = split && R <= 2 && op->dn.h16 &&
