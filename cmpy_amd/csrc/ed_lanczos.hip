@@ -141,6 +141,36 @@ __global__ __launch_bounds__(kVecThreads) void k_update(double* __restrict__ w, 
   }
 }
 
+// k_update on pairs of doubles (16-byte loads and stores; all pointers
+// 16-byte aligned, D2 = D / 2 with D even)
+__global__ __launch_bounds__(kVecThreads) void k_update2(double2* __restrict__ w, const double2* __restrict__ u,
+                                                         int64_t D2, const double* alpha, const double* scale,
+                                                         double2* __restrict__ psi, double yj, double* partial) {
+  __shared__ double sh[kVecThreads / 64];
+  const double a = (*alpha) * (*scale);
+  const double ys = yj * (*scale);
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < D2; k += stride) {
+    const double2 uk = u[k];
+    double2 wk = w[k];
+    wk.x -= a * uk.x;
+    wk.y -= a * uk.y;
+    w[k] = wk;
+    acc += wk.x * wk.x + wk.y * wk.y;
+    if (psi) {
+      double2 pk = psi[k];
+      pk.x += ys * uk.x;
+      pk.y += ys * uk.y;
+      psi[k] = pk;
+    }
+  }
+  if (partial) {
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+}
+
 // dst = scale * src (scale from device), partial |dst|^2 optional
 __global__ __launch_bounds__(kVecThreads) void k_scale(double* dst, const double* src,
                                                        int64_t D, const double* scale, int invert,
@@ -224,6 +254,18 @@ __global__ __launch_bounds__(kVecThreads) void k_axpy_multi(const double* __rest
 }
 
 static int vec_blocks(int64_t D) { return (int)std::max<int64_t>(1, std::min<int64_t>((D + 255) / 256, 2048)); }
+
+// k_update, as 16-byte accesses where D is even and every pointer is 16-byte
+// aligned; vb partials either way.
+static void launch_update(int vb, hipStream_t s, double* w, const double* u, int64_t D, const double* alpha,
+                          const double* scale, double* psi, double yj, double* partial) {
+  const uintptr_t al = (uintptr_t)w | (uintptr_t)u | (uintptr_t)psi;
+  if (D % 2 == 0 && (al & 15) == 0)
+    hipLaunchKernelGGL(k_update2, dim3(vb), dim3(kVecThreads), 0, s, (double2*)w, (const double2*)u, D / 2, alpha,
+                       scale, (double2*)psi, yj, partial);
+  else
+    hipLaunchKernelGGL(k_update, dim3(vb), dim3(kVecThreads), 0, s, w, u, D, alpha, scale, psi, yj, partial);
+}
 
 int launch_reduce(const double* partial, int n, double* out, int kind, hipStream_t stream) {
   LzCoef c{};
@@ -436,8 +478,7 @@ static int run_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, double
   }
   if (reduce) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, np, (int)RED_ALPHA, c, j, nullptr);
   const int vb = vec_blocks(L.D);
-  hipLaunchKernelGGL(k_update, dim3(vb), dim3(kVecThreads), 0, s, w, u_cur, L.D, c.alpha + j, c.scale + j, psi, yj,
-                     reduce ? part : nullptr);
+  launch_update(vb, s, w, u_cur, L.D, c.alpha + j, c.scale + j, psi, yj, reduce ? part : nullptr);
   if (reduce) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_BETA, c, j, nullptr);
   ED_HIP(hipGetLastError());
   return ED_OK;
@@ -586,8 +627,7 @@ static int reorth_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, con
     if (b) ED_HIP(hipEventRecord(b, s));
   }
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, np, (int)RED_SUM_TO, c, j, c.alpha + j);
-  hipLaunchKernelGGL(k_update, dim3(vb), dim3(kVecThreads), 0, s, r.w, vj, D, c.alpha + j, c.misc + 1, nullptr, 0.0,
-                     pro ? part : nullptr);
+  launch_update(vb, s, r.w, vj, D, c.alpha + j, c.misc + 1, nullptr, 0.0, pro ? part : nullptr);
   const int* gate = nullptr;
   if (pro) {
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_SQRT_TO, c, 0, pro->bt);
@@ -724,8 +764,7 @@ extern "C" int ed_lanczos_groundstate(ed_op* op, const ed_lanczos_cfg* cfg, cons
           ED_TRY(run_step(op, L, c, part, u[ip], u[ic], u[iw], j, s, nullptr, psi, y[j], false));
         } else {
           // last vector: accumulate only (w is scratch)
-          hipLaunchKernelGGL(k_update, dim3(vb), dim3(kVecThreads), 0, s, u[iw], u[ic], D, c.misc + 2, c.scale + j,
-                             psi, y[j], nullptr);
+          launch_update(vb, s, u[iw], u[ic], D, c.misc + 2, c.scale + j, psi, y[j], nullptr);
         }
         const int old_p = ip;
         ip = ic;
