@@ -149,6 +149,22 @@ def test_lanczos_coeffs_match_reference_recurrence(gpu):
     assert_allclose(b, b_ref, rtol=1e-9, atol=1e-9)
 
 
+@pytest.mark.parametrize("reorth", [False, "full", "partial"])
+def test_lanczos_coeffs_breakdown_on_an_eigenvector(gpu, reorth):
+    """Started on an eigenvector the recurrence stops after one step in every
+    reorthogonalisation mode: a single coefficient, the eigenvalue."""
+    from cmpy_amd import groundstate, lanczos_coeffs
+
+    model, _ = _siam(3)
+    op = model.hamilton_operator(3, 3)
+    gs = groundstate(op, tol=1e-12, max_iter=400)
+    a, b = lanczos_coeffs(op, 20, v0=gs.state, reorth=reorth)
+    assert len(a) == 1 and len(b) == 0
+    assert abs(a[0] - gs.energy) <= 1e-10 * abs(gs.energy)
+    with pytest.raises(ValueError):
+        lanczos_coeffs(op, 20, v0=gs.state, reorth="sometimes")
+
+
 @pytest.mark.parametrize("seed", range(10))
 @pytest.mark.parametrize("sign_mode", ["reference", "fermion"])
 def test_cfg1_lehmann_t0(gpu, seed, sign_mode):
