@@ -948,6 +948,13 @@ def square_bonds(nx, ny, periodic=True):
     return sorted(bonds)
 
 
+def eigvalsh_chain(num_sites, eps, t):
+    """Eigenvalues of the open tight-binding chain (cmpy/models/tightbinding.py:16-40):
+    the symmetric tridiagonal matrix with eps on the diagonal and t beside it."""
+    diag = np.full(num_sites, float(eps)) if np.isscalar(eps) else np.asarray(eps, dtype=np.float64)
+    return la.eigvalsh_tridiagonal(diag, np.full(num_sites - 1, float(t)))
+
+
 def free_fermion_energy(num_sites, bonds, hop, eps, n_up, n_dn):
     """U = 0 closed form: sum of the lowest n_up and n_dn single-particle levels."""
     h = np.diag(np.full(num_sites, float(eps)))
@@ -1087,4 +1094,5 @@ PARITY = {
     "or a dropped factor fails it); i = j equals gf_lehmann; G_ij = G_ji",
     "resolvent_ij": "direct inversion of z - h",
     "lehmann_t0_ij": "U = 0 closed-shell ground state: resolvent_ij at interior sites; i = j equals lehmann_t0",
+    "eigvalsh_chain": "eps + 2 t cos(k pi / (N + 1)), k = 1..N (open-chain closed form)",
 }

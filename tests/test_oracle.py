@@ -148,6 +148,18 @@ def test_free_fermion_closed_form(geom, filling):
     assert abs(e0 - O.free_fermion_energy(ns, bonds, 1.0, 0.3, n_up, n_dn)) < 1e-10
 
 
+@pytest.mark.parametrize("n", [2, 5, 12])
+def test_eigvalsh_chain_closed_form(n):
+    """Open chain: eps + 2 t cos(k pi / (N + 1)); the free-fermion energy of
+    an open chain at U = 0 is the sum of the lowest levels."""
+    eps, t = 0.3, -0.7
+    k = np.arange(1, n + 1)
+    assert_allclose(O.eigvalsh_chain(n, eps, t), np.sort(eps + 2 * t * np.cos(k * np.pi / (n + 1))), atol=1e-13)
+    w = O.eigvalsh_chain(n, 0.0, 1.0)
+    nu = n // 2
+    assert abs(O.free_fermion_energy(n, O.chain_bonds(n), 1.0, 0.0, nu, nu) - 2 * w[:nu].sum()) < 1e-12
+
+
 def test_sign_width_matters_on_a_ring():
     # Dropping the sign (width 0) breaks the closed form on the 4x2 torus at (3,3).
     ns, bonds = 8, O.square_bonds(4, 2, True)
