@@ -366,20 +366,35 @@ __global__ void k_sum_serial(const double* __restrict__ partial, int64_t n, doub
   *out = s;
 }
 
-// rocBLAS handle per device (the solver library's own state).
+// rocBLAS handles, one per (device, stream): calls on different streams may
+// run concurrently from different host threads, and a handle's stream (and
+// its workspace) must not be switched under a call in flight.
 static std::mutex g_handle_mu;
-static rocblas_handle g_handles[64] = {};
+struct HandleSlot {
+  int dev;
+  hipStream_t stream;
+  rocblas_handle h;
+};
+static std::vector<HandleSlot> g_handles;
 
-static int get_handle(rocblas_handle* h) {
+static int get_handle(rocblas_handle* h, hipStream_t s) {
   int dev = 0;
   ED_HIP(hipGetDevice(&dev));
-  ED_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
   std::lock_guard<std::mutex> lk(g_handle_mu);
-  if (!g_handles[dev]) {
-    if (rocblas_create_handle(&g_handles[dev]) != rocblas_status_success)
-      return set_error(ED_ERR_HIP, "rocblas_create_handle failed");
+  for (const HandleSlot& e : g_handles)
+    if (e.dev == dev && e.stream == s) {
+      *h = e.h;
+      return ED_OK;
+    }
+  ED_REQUIRE(g_handles.size() < 256, "too many (device, stream) pairs for the solver handles");
+  rocblas_handle nh;
+  if (rocblas_create_handle(&nh) != rocblas_status_success) return set_error(ED_ERR_HIP, "rocblas_create_handle failed");
+  if (rocblas_set_stream(nh, s) != rocblas_status_success) {
+    (void)rocblas_destroy_handle(nh);
+    return set_error(ED_ERR_HIP, "rocblas_set_stream failed");
   }
-  *h = g_handles[dev];
+  g_handles.push_back({dev, s, nh});
+  *h = nh;
   return ED_OK;
 }
 
@@ -448,8 +463,7 @@ int ed_tevo_eval(const double* alpha, const double* beta, int32_t m, double norm
     return ED_OK;
   }
   rocblas_handle h;
-  ED_TRY(get_handle(&h));
-  if (rocblas_set_stream(h, s) != rocblas_status_success) return set_error(ED_ERR_HIP, "rocblas_set_stream failed");
+  ED_TRY(get_handle(&h, s));
   // one allocation: diagonal [m], off-diagonal [m], eigenvectors [m * m], info
   const size_t nd = (size_t)m * (m + 2);
   double* d = nullptr;
@@ -590,10 +604,9 @@ int ed_dense_eigh(double* a_dev, int64_t n, double* w_dev, void* stream) {
   ED_REQUIRE(n >= 0 && n <= 65536, "n=%lld invalid", (long long)n);
   if (n == 0) return ED_OK;
   ED_REQUIRE(a_dev && w_dev, "NULL buffer");
-  rocblas_handle h;
-  ED_TRY(get_handle(&h));
   hipStream_t s = as_stream(stream);
-  if (rocblas_set_stream(h, s) != rocblas_status_success) return set_error(ED_ERR_HIP, "rocblas_set_stream failed");
+  rocblas_handle h;
+  ED_TRY(get_handle(&h, s));
   double* e = nullptr;
   rocblas_int* info = nullptr;
   ED_HIP(hipMalloc(&e, sizeof(double) * n));
@@ -624,10 +637,9 @@ int ed_dense_eigh_batched(double* a_dev, int64_t n, int64_t batch, double* w_dev
   ED_REQUIRE(batch >= 0 && batch <= (1 << 30), "batch=%lld invalid", (long long)batch);
   if (n == 0 || batch == 0) return ED_OK;
   ED_REQUIRE(a_dev && w_dev, "NULL buffer");
-  rocblas_handle h;
-  ED_TRY(get_handle(&h));
   hipStream_t s = as_stream(stream);
-  if (rocblas_set_stream(h, s) != rocblas_status_success) return set_error(ED_ERR_HIP, "rocblas_set_stream failed");
+  rocblas_handle h;
+  ED_TRY(get_handle(&h, s));
   double* e = nullptr;
   ED_HIP(hipMalloc(&e, sizeof(double) * n * batch + sizeof(rocblas_int) * batch));
   rocblas_int* info = (rocblas_int*)(e + n * batch);

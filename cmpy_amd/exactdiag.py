@@ -132,26 +132,65 @@ def _dense_eigh_device(op: SectorHamiltonOperator) -> Tuple[torch.Tensor, torch.
     return w, h.view(D, D)
 
 
-def _dense_eigh_sectors(model, sectors, cache: dict) -> None:
-    """Dense eigenpairs of every listed sector missing from ``cache``, with the
-    sectors of equal dimension diagonalised in one batched solver call."""
+def _dense_eigh_sectors(model, sectors, cache: dict, streams: int = 4) -> None:
+    """Dense eigenpairs of every listed sector missing from ``cache``.  Sectors
+    of equal dimension are diagonalised in one batched solver call, and the
+    dimension groups are spread over up to ``streams`` HIP streams, one host
+    thread each (the solver's many small kernels then overlap)."""
     by_dim = {}
     for sec in sectors:
         if sec in cache:
             continue
         op = model.hamilton_operator(n_up=sec[0], n_dn=sec[1])
-        by_dim.setdefault(op.size, []).append((sec, op))
-    for D, group in by_dim.items():
-        if D == 0:
-            continue
-        B = len(group)
-        h = torch.empty((B, D, D), dtype=torch.float64, device="cuda")
-        for k, (_, op) in enumerate(group):
-            _lib.call("ed_op_dense", op.handle, _lib.dptr(h[k]), _lib.stream_ptr())
-        w = torch.empty((B, D), dtype=torch.float64, device="cuda")
-        _lib.call("ed_dense_eigh_batched", _lib.dptr(h), D, B, _lib.dptr(w), _lib.stream_ptr())
-        for k, (sec, _) in enumerate(group):
-            cache[sec] = (w[k], h[k])
+        if op.size:
+            by_dim.setdefault(op.size, []).append((sec, op))
+    if not by_dim:
+        return
+    # largest work first, dealt to the streams greedily by O(D^3) cost
+    groups = sorted(by_dim.items(), key=lambda kv: -(kv[0] ** 3) * len(kv[1]))
+    k = max(1, min(int(streams), len(groups)))
+    lanes = [[] for _ in range(k)]
+    load = [0.0] * k
+    for g in groups:
+        i = load.index(min(load))
+        lanes[i].append(g)
+        load[i] += float(g[0]) ** 3 * len(g[1])
+
+    def run(lane):
+        out = []
+        for D, group in lane:
+            B = len(group)
+            h = torch.empty((B, D, D), dtype=torch.float64, device="cuda")
+            for j, (_, op) in enumerate(group):
+                _lib.call("ed_op_dense", op.handle, _lib.dptr(h[j]), _lib.stream_ptr())
+            w = torch.empty((B, D), dtype=torch.float64, device="cuda")
+            # returns after its stream has finished (it reads the solver status)
+            _lib.call("ed_dense_eigh_batched", _lib.dptr(h), D, B, _lib.dptr(w), _lib.stream_ptr())
+            out.append((group, w, h))
+        return out
+
+    if k == 1:
+        results = [run(lanes[0])]
+    else:
+        from concurrent.futures import ThreadPoolExecutor
+
+        main = torch.cuda.current_stream()
+        side = [torch.cuda.Stream() for _ in range(k)]
+        for st in side:
+            st.wait_stream(main)
+
+        def run_on(i):
+            with torch.cuda.stream(side[i]):
+                return run(lanes[i])
+
+        with ThreadPoolExecutor(k) as ex:
+            results = list(ex.map(run_on, range(k)))
+        for st in side:
+            main.wait_stream(st)
+    for lane_out in results:
+        for group, w, h in lane_out:
+            for j, (sec, _) in enumerate(group):
+                cache[sec] = (w[j], h[j])
 
 
 def compute_groundstate(model, thresh: int = 50, tol: float = 1e-10) -> EigenState:
