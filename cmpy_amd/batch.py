@@ -231,21 +231,23 @@ class BathFit:
     eps_bath: np.ndarray  # [M, max_bath] fitted parameters (padding entries unused)
     v: np.ndarray
     loss: np.ndarray      # [M] final per-model loss sum_n |G - G_target|^2
-    history: list         # total loss at every evaluation
+    history: list         # (weighted) total loss at every evaluation
     evaluations: int
 
 
 def fit_bath(batch: AndersonBatch, wn, g_target, eps0=None, v0=None, max_iter: int = 100,
-             tolerance_change: float = 1e-24, history_size: int = 20) -> BathFit:
+             tolerance_change: float = 1e-30, history_size: int = 20, normalize: bool = True) -> BathFit:
     """Gradient-driven bath fit of every model of ``batch``: minimise
-    L = sum_models sum_n |G(i w_n; eps_bath, v) - g_target|^2 with L-BFGS
+    L = sum_models w_m sum_n |G(i w_n; eps_bath, v) - g_target|^2 with L-BFGS
     (strong Wolfe line search), each loss and gradient from one batched
     forward solve and one adjoint backward (``anderson_gf``).  The batch must
     be created with ``keep_basis=True``.  ``eps0`` / ``v0`` ([M, max_bath])
     default to the batch's current bath.  The models' losses are independent,
-    so the joint minimisation fits each model.  G(i w_n) constrains some
-    parameter combinations only weakly (the usual ill-conditioning of
-    Matsubara-axis bath fits), so a small loss does not pin every parameter.
+    so the joint minimisation fits each model; with ``normalize`` each is
+    weighted by 1 / (its loss at the start), so one badly fitting model does
+    not set the step for all.  G(i w_n) constrains some parameter
+    combinations only weakly (the usual ill-conditioning of Matsubara-axis
+    bath fits), so a small loss does not pin every parameter.
 
     The reference fits the bath of its DMFT loop with derivative-free scipy
     optimisers; this is the gradient route that the adjoint backward enables.
@@ -260,6 +262,11 @@ def fit_bath(batch: AndersonBatch, wn, g_target, eps0=None, v0=None, max_iter: i
         raise ValueError(f"g_target must have shape ({batch.num_models}, {len(wn)})")
     e = torch.tensor(batch.eps_bath if eps0 is None else np.asarray(eps0, dtype=np.float64), requires_grad=True)
     v = torch.tensor(batch.v if v0 is None else np.asarray(v0, dtype=np.float64), requires_grad=True)
+    weight = torch.ones(batch.num_models, dtype=torch.float64)
+    if normalize:
+        with torch.no_grad():
+            l0 = (anderson_gf(e.detach(), v.detach(), batch, wn) - gt).abs().pow(2).sum(dim=1)
+        weight = 1.0 / torch.clamp(l0, min=1e-300)
     opt = torch.optim.LBFGS([e, v], lr=1.0, max_iter=max_iter, max_eval=2 * max_iter,
                             tolerance_grad=0.0, tolerance_change=tolerance_change, history_size=history_size,
                             line_search_fn="strong_wolfe")
@@ -268,7 +275,7 @@ def fit_bath(batch: AndersonBatch, wn, g_target, eps0=None, v0=None, max_iter: i
     def closure():
         opt.zero_grad()
         g = anderson_gf(e, v, batch, wn)
-        loss = (g - gt).abs().pow(2).sum()
+        loss = ((g - gt).abs().pow(2).sum(dim=1) * weight).sum()
         loss.backward()
         hist.append(float(loss.detach()))
         return loss

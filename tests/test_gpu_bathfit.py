@@ -175,9 +175,9 @@ def test_fit_bath_reduces_the_loss(gpu):
     """L-BFGS on the adjoint gradients fits G(i w_n) of a known bath from a
     perturbed start.  Matsubara-axis bath fits are ill-conditioned (G at
     beta = 100 barely constrains some parameter combinations), so the test
-    asks for the loss, not for the parameters: the total misfit drops by
-    four orders of magnitude and every model's by at least one (one joint
-    L-BFGS line search serves all models)."""
+    asks for the loss, not for the parameters: with each model's loss
+    normalised by its starting value, every model's misfit drops by three
+    orders of magnitude (unnormalised, the worst model sets the steps)."""
     from cmpy_amd.batch import AndersonBatch, fit_bath, matsubara
 
     rng = np.random.default_rng(7)
@@ -194,8 +194,8 @@ def test_fit_bath_reduces_the_loss(gpu):
     g0 = AndersonBatch(u=u, eps_imp=-u / 2, eps_bath=list(e0), v=list(v0), mu=0.0).solve(wn).g
     loss0 = (np.abs(g0 - target) ** 2).sum(axis=1)
     fit = fit_bath(b, wn, target, e0, v0, max_iter=100)
-    assert fit.history[-1] < 1e-4 * fit.history[0]
-    assert np.all(fit.loss < 0.1 * loss0)
+    assert fit.history[-1] < 1e-3 * fit.history[0]
+    assert np.all(fit.loss < 1e-3 * loss0)
     with pytest.raises(ValueError):
         fit_bath(AndersonBatch(u=u, eps_imp=-u / 2, eps_bath=list(eb), v=list(v), mu=0.0), wn, target)
 
