@@ -191,6 +191,18 @@ __global__ __launch_bounds__(kVecThreads) void k_scale(double* dst, const double
   }
 }
 
+// partial |w|^2; gate (optional): skip the launch unless *gate != 0
+__global__ __launch_bounds__(kVecThreads) void k_norm2(const double* __restrict__ w, int64_t D, double* partial,
+                                                       const int* gate) {
+  if (gate && *gate == 0) return;
+  __shared__ double sh[kVecThreads / 64];
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < D; k += stride) acc += w[k] * w[k];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 // partial |w - e psi|^2
 __global__ __launch_bounds__(kVecThreads) void k_resid(const double* __restrict__ w, const double* __restrict__ psi,
                                                        int64_t D, double e, double* partial) {
@@ -520,12 +532,11 @@ static int reorth_start(const LzLayout& L, LzCoef c, double* part, const ReorthC
 // with w_{k,k} = 1 and w_{k+1,k} = eps.  When max_k |w_{j+1,k}| passes
 // sqrt(eps) the new vector is orthogonalised against all previous ones (and
 // so is the next one), and its row is reset to eps.  Rows live in om[3][M+2]
-// (row r at (r % 3)); pro[0] = orthogonalise this step, pro[1] = the next
-// step must too; bt = |w| before orthogonalisation.
+// (row r at (r % 3)); flags[0] = orthogonalise this step, flags[1] = the
+// next step must too.
 struct ProState {
   double* om;   // [3][M + 2]
   int* flags;   // [2]
-  double* bt;   // [1]
   int M;
 };
 
@@ -538,16 +549,25 @@ __global__ __launch_bounds__(256) void k_pro_init(ProState p) {
   }
 }
 
+// bt = |w| before orthogonalisation, summed here from the np partials of
+// |w|^2 that the vector update left.
 __global__ __launch_bounds__(256) void k_pro_omega(ProState p, const double* __restrict__ alpha,
-                                                   const double* __restrict__ beta, int j) {
+                                                   const double* __restrict__ beta, int j,
+                                                   const double* __restrict__ wpart, int np) {
   __shared__ double sh[256 / 64];
   __shared__ int decide;
+  __shared__ double bt_sh;
   const int W = p.M + 2;
   const double* cur = p.om + (int64_t)(j % 3) * W;
   const double* prev = p.om + (int64_t)((j + 2) % 3) * W;
   double* nxt = p.om + (int64_t)((j + 1) % 3) * W;
   const double eps = 2.220446049250313e-16;
-  const double bt = *p.bt;
+  double w2 = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) w2 += wpart[i];
+  w2 = block_sum(w2, sh);
+  if (threadIdx.x == 0) bt_sh = sqrt(fmax(w2, 0.0));
+  __syncthreads();
+  const double bt = bt_sh;
   double mx = 0.0;
   if (bt > 0.0) {
     for (int k = threadIdx.x; k < j; k += blockDim.x) {
@@ -593,10 +613,9 @@ struct ProBuffer {
 };
 
 static int pro_alloc(ProState& pro, ProBuffer& buf, hipStream_t s) {
-  ED_HIP(hipMalloc(&buf.p, sizeof(double) * (3 * (size_t)(pro.M + 2) + 2)));
+  ED_HIP(hipMalloc(&buf.p, sizeof(double) * (3 * (size_t)(pro.M + 2) + 1)));
   pro.om = buf.p;
-  pro.bt = buf.p + 3 * (size_t)(pro.M + 2);
-  pro.flags = reinterpret_cast<int*>(buf.p + 3 * (size_t)(pro.M + 2) + 1);
+  pro.flags = reinterpret_cast<int*>(buf.p + 3 * (size_t)(pro.M + 2));
   hipLaunchKernelGGL(k_pro_init, dim3(1), dim3(256), 0, s, pro);
   ED_HIP(hipGetLastError());
   return ED_OK;
@@ -630,8 +649,7 @@ static int reorth_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, con
   launch_update(vb, s, r.w, vj, D, c.alpha + j, c.misc + 1, nullptr, 0.0, pro ? part : nullptr);
   const int* gate = nullptr;
   if (pro) {
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_SQRT_TO, c, 0, pro->bt);
-    hipLaunchKernelGGL(k_pro_omega, dim3(1), dim3(256), 0, s, *pro, c.alpha, c.beta, j);
+    hipLaunchKernelGGL(k_pro_omega, dim3(1), dim3(256), 0, s, *pro, c.alpha, c.beta, j, part, vb);
     gate = pro->flags;
   }
   for (int pass = 0; pass < 2; ++pass) {
@@ -642,7 +660,9 @@ static int reorth_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, con
     hipLaunchKernelGGL(k_axpy_multi, dim3(vb), dim3(kVecThreads), sizeof(double) * (j + 1), s, r.V, j + 1, D,
                        r.coefs, r.w, 0, gate);
   }
-  hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, r.w, r.w, D, c.misc + 1, 0, part);
+  // |w|^2 after the passes; on a step without them the update's partials
+  // (already in part) are those of the same w
+  hipLaunchKernelGGL(k_norm2, dim3(vb), dim3(kVecThreads), 0, s, r.w, D, part, gate);
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_BETA, c, j, nullptr);
   hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, r.V + (int64_t)(j + 1) * D, r.w, D, c.scale + j + 1, 0,
                      nullptr);
@@ -779,7 +799,7 @@ extern "C" int ed_lanczos_groundstate(ed_op* op, const ed_lanczos_cfg* cfg, cons
     double* V = rc.V;
     double* coefs = rc.coefs;
     ED_TRY(reorth_start(L, c, part, rc, v0_dev, cfg->seed, s));
-    ProState pro{nullptr, nullptr, nullptr, M};
+    ProState pro{nullptr, nullptr, M};
     ProBuffer pro_buf{nullptr, s};
     if (cfg->reorth == 2) {
       ED_TRY(pro_alloc(pro, pro_buf, s));
@@ -870,7 +890,7 @@ extern "C" int ed_lanczos_coeffs(ed_op* op, const double* v0_dev, int32_t nsteps
   if (reorth) {
     const ReorthCtx rc = reorth_ctx(base, L, part);
     ED_TRY(reorth_start(L, c, part, rc, v0_dev, 0, s));
-    ProState pro{nullptr, nullptr, nullptr, M};
+    ProState pro{nullptr, nullptr, M};
     ProBuffer pro_buf{nullptr, s};
     if (reorth == 2) ED_TRY(pro_alloc(pro, pro_buf, s));
     for (int j = 0; j < M; ++j) ED_TRY(reorth_step(op, L, c, part, rc, j, s, nullptr, reorth == 2 ? &pro : nullptr));
