@@ -276,14 +276,18 @@ def bench_chain12(args, log):
     per_u = {}
     for U in (2.0, 4.0, 8.0):
         model = HubbardModel(ns, chain_neighbors(ns, periodic=False), inter=U, eps=0.0, hop=1.0, mu=0.0)
-        groundstate(model, n, n, tol=1e-10)  # warm (tables, kernels)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        gs, st = groundstate(model, n, n, tol=1e-10, return_stats=True)
-        cf = greens_cf(model, z, n, n, nsteps=200, gs=gs)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        per_u[U] = {"seconds": round(dt, 4), "e0": gs.energy, "gs_steps": st.iterations, "residual": st.residual}
+        greens_cf(model, z, n, n, nsteps=200, gs=groundstate(model, n, n, tol=1e-10))  # warm (tables, kernels)
+        reps = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            gs, st = groundstate(model, n, n, tol=1e-10, return_stats=True)
+            cf = greens_cf(model, z, n, n, nsteps=200, gs=gs)
+            torch.cuda.synchronize()
+            reps.append(time.perf_counter() - t0)
+        dt = float(np.median(reps))
+        per_u[U] = {"seconds": round(dt, 4), "repeats": [round(r, 4) for r in reps], "e0": gs.energy,
+                    "gs_steps": st.iterations, "residual": st.residual}
         if U == 4.0:
             # parity inputs (untimed): coefficients, and G on a broadened axis
             # (eta = 1), where 200 steps have converged whether or not the
@@ -353,16 +357,24 @@ def bench_tevo12(args, log):
     ns, n, U = 12, 6, 4.0
     start, stop, num = 0.0, 20.0, 1000
     model = HubbardModel(ns, chain_neighbors(ns, periodic=False), inter=U, eps=0.0, hop=1.0, mu=0.0)
-    groundstate(model, n, n, tol=1e-10)  # warm
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    gs = groundstate(model, n, n, tol=1e-10)
-    times, g_gt = gf_greater(model, gs, start, stop, num)
-    _, g_lt = gf_lesser(model, gs, start, stop, num)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+
+    def solve():
+        gs = groundstate(model, n, n, tol=1e-10)
+        times, g_gt = gf_greater(model, gs, start, stop, num)
+        _, g_lt = gf_lesser(model, gs, start, stop, num)
+        return gs, times, g_gt, g_lt
+
+    solve()  # warm (kernels, solver handles)
+    reps = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gs, times, g_gt, g_lt = solve()
+        torch.cuda.synchronize()
+        reps.append(time.perf_counter() - t0)
+    dt = float(np.median(reps))
     out = {"metric": "solves/s", "value": round(1.0 / dt, 3), "unit": "solves/s", "dim": 853776,
-           "seconds": round(dt, 4), "times": num, "t_range": [start, stop],
+           "seconds": round(dt, 4), "repeats": [round(r, 4) for r in reps], "times": num, "t_range": [start, stop],
            "g_at_0": [float(g_gt[0].imag), float(g_lt[0].imag)],
            "data": "deterministic model (no random input beyond the seeded Lanczos start)"}
     log(f"12-site chain real-time G: {dt:.3f} s per solve")
@@ -420,13 +432,17 @@ def bench_finite_t(args, log):
     z = np.linspace(-8, 8, 1000) + 0.05j
     model = SingleImpurityAndersonModel(U, eimp, eps, v, mu=0.0)
     gf_lehmann(model, z[:8], beta)  # warm (kernels, solver handle)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    data = gf_lehmann(model, z, beta)
-    g = data.gf
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    reps = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        data = gf_lehmann(model, z, beta)
+        g = data.gf
+        torch.cuda.synchronize()
+        reps.append(time.perf_counter() - t0)
+    dt = float(np.median(reps))
     out = {"metric": "solves/s", "value": round(1.0 / dt, 4), "unit": "solves/s", "seconds": round(dt, 4),
+           "repeats": [round(r, 4) for r in reps],
            "sites": nb + 1, "sectors": (nb + 2) ** 2, "max_dim": 1225, "beta": beta, "frequencies": len(z),
            "occ": data.occ, "occ_double": data.occ_double,
            "data": "synthetic (bath from numpy default_rng(0))"}
@@ -478,13 +494,17 @@ def bench_gij(args, log):
     z = np.linspace(-8, 8, 1000) + 0.05j
     model = HubbardModel(ns, chain_neighbors(ns, periodic=True), inter=U, eps=0.0, hop=1.0, mu=U / 2)
     greens_lehmann(model, z[:8], beta, 0, 1)  # warm
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    cache = {}
-    gs = [greens_lehmann(model, z, beta, 0, j, eig_cache=cache).gf for j in range(4)]
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    reps = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cache = {}
+        gs = [greens_lehmann(model, z, beta, 0, j, eig_cache=cache).gf for j in range(4)]
+        torch.cuda.synchronize()
+        reps.append(time.perf_counter() - t0)
+    dt = float(np.median(reps))
     out = {"metric": "solves/s", "value": round(1.0 / dt, 4), "unit": "solves/s", "seconds": round(dt, 4),
+           "repeats": [round(r, 4) for r in reps],
            "sites": ns, "elements": 4, "beta": beta, "frequencies": len(z), "max_dim": 1225,
            "data": "deterministic model"}
     log(f"G_ij (7-site ring, 4 elements): {dt:.3f} s per solve")

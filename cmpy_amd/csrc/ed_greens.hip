@@ -11,6 +11,8 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <cstdint>
+#include <cstdlib>
 #include <mutex>
 
 namespace ed {
@@ -335,6 +337,57 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const double* __restrict__ A, c
     }
 }
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// C[M x N] = A B^T as k_gemm_nt, on the FP64 matrix cores: a 64x64 output
+// tile per block, each of the four waves a 32x32 quarter of it as 2x2
+// v_mfma_f64_16x16x4 tiles.  16-wide K slabs are staged in LDS (zero-padded
+// past M, N and K).  Operand lanes: lane l gives A[l % 16][l / 16] and
+// B^T[l / 16][l % 16] of a 16x4 / 4x16 step; result element r of lane l is
+// C[l / 16 + 4 r][l % 16].
+__global__ __launch_bounds__(256) void k_gemm_nt_mfma(const double* __restrict__ A, const double* __restrict__ B,
+                                                      double* __restrict__ Cm, int64_t M, int64_t N, int64_t K) {
+  __shared__ double As[64][16 + 1];
+  __shared__ double Bs[64][16 + 1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  const int64_t m0 = (int64_t)blockIdx.y * 64, n0 = (int64_t)blockIdx.x * 64;
+  f64x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+  for (int64_t k0 = 0; k0 < K; k0 += 16) {
+    for (int i = threadIdx.x; i < 64 * 16; i += 256) {
+      const int row = i >> 4, kk = i & 15;
+      const int64_t kg = k0 + kk;
+      As[row][kk] = (kg < K && m0 + row < M) ? A[(m0 + row) * K + kg] : 0.0;
+      Bs[row][kk] = (kg < K && n0 + row < N) ? B[(n0 + row) * K + kg] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 16; ks += 4) {
+      const double a0 = As[wm + li][ks + lk], a1 = As[wm + 16 + li][ks + lk];
+      const double b0 = Bs[wn + li][ks + lk], b1 = Bs[wn + 16 + li][ks + lk];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm + ti * 16 + lk + 4 * r, col = n0 + wn + tj * 16 + li;
+        if (row < M && col < N) Cm[row * N + col] = acc[ti][tj][r];
+      }
+}
+
 // Occupation partials: block (c, k) sums w_k |V_k[i]|^2 over the indices i of
 // chunk c whose up/dn/both spins occupy `pos`.
 __global__ __launch_bounds__(256) void k_occ(const double* __restrict__ V, int64_t D, int64_t num_dn,
@@ -555,9 +608,36 @@ int ed_gemm_nt(const double* a_dev, const double* b_dev, double* c_dev, int64_t 
   clear_error();
   ED_REQUIRE(k >= 0 && m >= 0 && n >= 0, "negative size");
   if (m == 0 || n == 0) return ED_OK;
-  ED_REQUIRE(a_dev && b_dev && c_dev, "NULL buffer");
+  ED_REQUIRE(c_dev && (k == 0 || (a_dev && b_dev)), "NULL buffer");
+  ED_REQUIRE((m + 63) / 64 <= 65535 && (n + 63) / 64 <= (int64_t)INT32_MAX, "m=%lld n=%lld too large",
+             (long long)m, (long long)n);
+  hipStream_t s = as_stream(stream);
+  // CMPY_AMD_GEMM: 0 VALU kernel, 1 matrix-core kernel, 2 rocBLAS dgemm;
+  // unset: the matrix-core kernel up to kGemmLibFlops multiply-adds, rocBLAS
+  // above (measured: 1225^3 in 82 us against 258 us, 245 x 49 x 245 in 10 us
+  // against 31 us; 49 x 7 x 49 in 8 us against 10 us)
+  constexpr double kGemmLibFlops = 1 << 17;
+  const char* v = getenv("CMPY_AMD_GEMM");
+  const int mode = v && *v ? atoi(v) : ((double)m * n * k > kGemmLibFlops ? 2 : 1);
+  ED_REQUIRE(mode >= 0 && mode <= 2, "CMPY_AMD_GEMM=%d is not 0, 1 or 2", mode);
+  if (mode == 2 && k > 0) {
+    ED_REQUIRE(m <= INT32_MAX && n <= INT32_MAX && k <= INT32_MAX, "sizes beyond 32-bit BLAS indices");
+    rocblas_handle h;
+    ED_TRY(get_handle(&h, s));
+    const double one = 1.0, zero = 0.0;
+    // row-major C = A B^T is column-major C^T = B^T' A' with A', B' the
+    // column-major views (K x M, K x N) of the same memory
+    const rocblas_status st = rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)n,
+                                            (rocblas_int)m, (rocblas_int)k, &one, b_dev, (rocblas_int)k, a_dev,
+                                            (rocblas_int)k, &zero, c_dev, (rocblas_int)n);
+    if (st != rocblas_status_success) return set_error(ED_ERR_HIP, "rocblas_dgemm failed (status %d)", (int)st);
+    return ED_OK;
+  }
   const dim3 g((unsigned)((n + 63) / 64), (unsigned)((m + 63) / 64));
-  hipLaunchKernelGGL(k_gemm_nt, g, dim3(256), 0, as_stream(stream), a_dev, b_dev, c_dev, m, n, k);
+  if (mode == 0)
+    hipLaunchKernelGGL(k_gemm_nt, g, dim3(256), 0, s, a_dev, b_dev, c_dev, m, n, k);
+  else
+    hipLaunchKernelGGL(k_gemm_nt_mfma, g, dim3(256), 0, s, a_dev, b_dev, c_dev, m, n, k);
   ED_HIP(hipGetLastError());
   return ED_OK;
 }

@@ -259,7 +259,10 @@ int ed_lehmann_pairs_ij(const double* overlap_i_dev, const double* overlap_j_dev
                         const double* z_dev, int64_t nz, double* g_dev, void* stream);
 
 /* C[M x N] = A B^T with A: [M][K], B: [N][K] (vectors stored one after
- * another): the overlap product evecs_p1^T (c+ evecs) of cmpy/exactdiag.py:112. */
+ * another): the overlap product evecs_p1^T (c+ evecs) of cmpy/exactdiag.py:112.
+ * Small products run on an FP64 matrix-core kernel, large ones through
+ * rocblas_dgemm (environment CMPY_AMD_GEMM = 0 VALU kernel, 1 matrix-core
+ * kernel, 2 rocBLAS).  K = 0 gives C = 0 (a_dev, b_dev may then be NULL).   */
 int ed_gemm_nt(const double* a_dev, const double* b_dev, double* c_dev, int64_t m, int64_t n,
                int64_t k, void* stream);
 
