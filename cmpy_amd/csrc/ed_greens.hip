@@ -238,17 +238,61 @@ __global__ __launch_bounds__(256) void k_tevo_sum(const double* __restrict__ the
 // steps r <- r + r (1 - d r), each of which squares the relative error.
 // With a second overlap ov2 (off-diagonal G_ij) the weight takes ov * ov2
 // in place of ov^2.
+//
+// Screening (finite beta): a pair whose two Boltzmann factors are both below
+// e^-kPairScreen (1e-20) is skipped.  keep[0] / keep[1] hold one past the
+// last n / m whose factor reaches it, so the pairs summed are those with
+// ni < keep[0] (any mi) and those with mi < keep[1] (ni >= keep[0]).  Since
+// sum_m |<m|c+|n>|^2 <= 1, the skipped terms of one sector pair add up to at
+// most (N + M) 1e-20 / Im z before the division by Z >= 1.
 constexpr int kPairTile = 1024;
 constexpr int64_t kPairsPerGroup = 8192;
+constexpr double kPairScreen = 46.0;
+__global__ __launch_bounds__(256) void k_pairs_keep(const double* __restrict__ en, int64_t n,
+                                                    const double* __restrict__ em, int64_t m, double beta,
+                                                    double emin, int screen, int64_t* __restrict__ keep) {
+  __shared__ int64_t sh[2][4];
+  int64_t kn = 0, km = 0;
+  if (screen && isfinite(beta)) {
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+      if (beta * (en[i] - emin) <= kPairScreen) kn = i + 1;
+    for (int64_t i = threadIdx.x; i < m; i += blockDim.x)
+      if (beta * (em[i] - emin) <= kPairScreen) km = i + 1;
+  } else {
+    kn = n;
+    km = m;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    kn = max(kn, (int64_t)__shfl_down(kn, off, 64));
+    km = max(km, (int64_t)__shfl_down(km, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = kn;
+    sh[1][threadIdx.x >> 6] = km;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      kn = max(kn, sh[0][w]);
+      km = max(km, sh[1][w]);
+    }
+    keep[0] = max(kn, sh[0][0]);
+    keep[1] = max(km, sh[1][0]);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ ov, const double* __restrict__ ov2,
                                                const double* __restrict__ en, int64_t n,
                                                const double* __restrict__ em, int64_t m, double beta, double emin,
-                                               const double* __restrict__ z, int64_t nz,
-                                               double* __restrict__ partial) {
+                                               const int64_t* __restrict__ keep, const double* __restrict__ z,
+                                               int64_t nz, double* __restrict__ partial) {
   __shared__ double sw[kPairTile], sp[kPairTile];
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t grp = blockIdx.y;
-  const int64_t np = n * m;
+  const int64_t nk = keep[0], mk = keep[1], rest = n - nk;
+  const int64_t head = m * nk;
+  const int64_t np = head + mk * rest;
   const int64_t p0 = grp * kPairsPerGroup;
   const int64_t p1 = p0 + kPairsPerGroup < np ? p0 + kPairsPerGroup : np;
   const double zr = k < nz ? z[2 * k] : 0.0, zi = k < nz ? z[2 * k + 1] : 1.0;
@@ -260,8 +304,17 @@ __global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ ov, co
     __syncthreads();
     for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
       const int64_t p = t0 + i;
-      const int64_t mi = p / n, ni = p - mi * n;
-      const double o = ov[p] * (ov2 ? ov2[p] : ov[p]);
+      int64_t mi, ni;
+      if (p < head) {
+        mi = p / nk;
+        ni = p - mi * nk;
+      } else {
+        const int64_t q = p - head;
+        mi = q / rest;
+        ni = nk + (q - mi * rest);
+      }
+      const int64_t at = mi * n + ni;
+      const double o = ov[at] * (ov2 ? ov2[at] : ov[at]);
       const double e_m = em[mi], e_n = en[ni];
       sw[i] = o * (fin ? exp(-beta * (e_n - emin)) + exp(-beta * (e_m - emin)) : 2.0);
       sp[i] = e_m - e_n;
@@ -574,11 +627,16 @@ static int lehmann_pairs(const double* ov, const double* ov2, const double* eval
   ED_REQUIRE(!std::isnan(beta) && std::isfinite(emin), "beta/emin invalid");
   const int64_t ng = (n * m + kPairsPerGroup - 1) / kPairsPerGroup;
   ED_REQUIRE(ng <= 65535, "sector pair too large for the pair-sum grid (%lld groups)", (long long)ng);
+  // CMPY_AMD_PAIR_SCREEN=0 sums every pair
+  const char* sv = getenv("CMPY_AMD_PAIR_SCREEN");
+  const int screen = !(sv && *sv && atoi(sv) == 0);
   double* part = nullptr;
-  ED_HIP(hipMalloc(&part, sizeof(double) * 2 * ng * nz));
+  ED_HIP(hipMalloc(&part, sizeof(double) * 2 * ng * nz + 2 * sizeof(int64_t)));
+  int64_t* keep = reinterpret_cast<int64_t*>(part + 2 * ng * nz);
   int rc = ED_OK;
+  hipLaunchKernelGGL(k_pairs_keep, dim3(1), dim3(256), 0, s, evals_dev, n, evals_p1_dev, m, beta, emin, screen, keep);
   hipLaunchKernelGGL(k_pairs, dim3((unsigned)((nz + 255) / 256), (unsigned)ng), dim3(256), 0, s, ov, ov2, evals_dev,
-                     n, evals_p1_dev, m, beta, emin, z_dev, nz, part);
+                     n, evals_p1_dev, m, beta, emin, keep, z_dev, nz, part);
   hipLaunchKernelGGL(k_pairs_reduce, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, part, ng, nz, g_dev);
   if (hipGetLastError() != hipSuccess) rc = set_error(ED_ERR_HIP, "pair-sum launch failed");
   if (hipStreamSynchronize(s) != hipSuccess && rc == ED_OK) rc = set_error(ED_ERR_HIP, "stream sync failed");

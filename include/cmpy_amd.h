@@ -243,7 +243,10 @@ int ed_pole_sum(const double* w_dev, const double* p_dev, int64_t m, const doubl
  *   g[k] += sum_{m,n} |ov[m,n]|^2 (e^{-beta(E_n-emin)} + e^{-beta(E_m-emin)})
  *                      / (z_k - E_m + E_n)
  * with ov = V_p1^T C (overlap_dev holds ov[m][n], row-major M x N).
- * beta = +inf gives unit weights, as the reference does.                    */
+ * beta = +inf gives unit weights, as the reference does.  At finite beta a
+ * pair whose two Boltzmann factors are both below 1e-20 is skipped (for
+ * normalised overlaps the skipped terms add up to at most
+ * (N + M) 1e-20 / Im z); environment CMPY_AMD_PAIR_SCREEN=0 sums every pair. */
 int ed_lehmann_pairs(const double* overlap_dev, const double* evals_dev, int64_t n,
                      const double* evals_p1_dev, int64_t m, double beta, double emin,
                      const double* z_dev, int64_t nz, double* g_dev, void* stream);
