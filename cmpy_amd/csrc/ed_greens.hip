@@ -630,8 +630,9 @@ static int lehmann_pairs(const double* ov, const double* ov2, const double* eval
   // CMPY_AMD_PAIR_SCREEN=0 sums every pair
   const char* sv = getenv("CMPY_AMD_PAIR_SCREEN");
   const int screen = !(sv && *sv && atoi(sv) == 0);
+  // stream-ordered scratch: no device-wide synchronisation per sector pair
   double* part = nullptr;
-  ED_HIP(hipMalloc(&part, sizeof(double) * 2 * ng * nz + 2 * sizeof(int64_t)));
+  ED_HIP(hipMallocAsync((void**)&part, sizeof(double) * 2 * ng * nz + 2 * sizeof(int64_t), s));
   int64_t* keep = reinterpret_cast<int64_t*>(part + 2 * ng * nz);
   int rc = ED_OK;
   hipLaunchKernelGGL(k_pairs_keep, dim3(1), dim3(256), 0, s, evals_dev, n, evals_p1_dev, m, beta, emin, screen, keep);
@@ -639,8 +640,7 @@ static int lehmann_pairs(const double* ov, const double* ov2, const double* eval
                      n, evals_p1_dev, m, beta, emin, keep, z_dev, nz, part);
   hipLaunchKernelGGL(k_pairs_reduce, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, part, ng, nz, g_dev);
   if (hipGetLastError() != hipSuccess) rc = set_error(ED_ERR_HIP, "pair-sum launch failed");
-  if (hipStreamSynchronize(s) != hipSuccess && rc == ED_OK) rc = set_error(ED_ERR_HIP, "stream sync failed");
-  (void)hipFree(part);
+  if (hipFreeAsync(part, s) != hipSuccess && rc == ED_OK) rc = set_error(ED_ERR_HIP, "hipFreeAsync failed");
   return rc;
 }
 
@@ -717,10 +717,11 @@ int ed_occupation(const double* vecs_dev, int64_t nvec, int num_sites, int n_up,
   uint32_t *su = nullptr, *sd = nullptr;
   double* part = nullptr;
   int rc = ED_OK;
-  if (hipMalloc(&su, sizeof(uint32_t) * std::max<int64_t>(nu, 1)) != hipSuccess ||
-      hipMalloc(&sd, sizeof(uint32_t) * std::max<int64_t>(nd, 1)) != hipSuccess ||
-      hipMalloc(&part, sizeof(double) * std::max<int64_t>(nchunk * nvec, 1)) != hipSuccess) {
-    rc = set_error(ED_ERR_NOMEM, "hipMalloc failed");
+  // stream-ordered scratch, freed in stream order (no synchronisation)
+  if (hipMallocAsync((void**)&su, sizeof(uint32_t) * std::max<int64_t>(nu, 1), s) != hipSuccess ||
+      hipMallocAsync((void**)&sd, sizeof(uint32_t) * std::max<int64_t>(nd, 1), s) != hipSuccess ||
+      hipMallocAsync((void**)&part, sizeof(double) * std::max<int64_t>(nchunk * nvec, 1), s) != hipSuccess) {
+    rc = set_error(ED_ERR_NOMEM, "hipMallocAsync failed");
   } else {
     hipLaunchKernelGGL(k_unrank_states, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, s, su, nu, num_sites, n_up);
     hipLaunchKernelGGL(k_unrank_states, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s, sd, nd, num_sites, n_dn);
@@ -729,11 +730,10 @@ int ed_occupation(const double* vecs_dev, int64_t nvec, int num_sites, int n_up,
                          1u << pos, kind, w_dev, chunk, part);
     hipLaunchKernelGGL(k_sum_serial, dim3(1), dim3(64), 0, s, part, nchunk * nvec, out_dev);
     if (hipGetLastError() != hipSuccess) rc = set_error(ED_ERR_HIP, "occupation launch failed");
-    if (hipStreamSynchronize(s) != hipSuccess && rc == ED_OK) rc = set_error(ED_ERR_HIP, "stream sync failed");
   }
-  if (su) (void)hipFree(su);
-  if (sd) (void)hipFree(sd);
-  if (part) (void)hipFree(part);
+  if (su) (void)hipFreeAsync(su, s);
+  if (sd) (void)hipFreeAsync(sd, s);
+  if (part) (void)hipFreeAsync(part, s);
   return rc;
 }
 

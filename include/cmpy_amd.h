@@ -13,9 +13,11 @@
  *     suffix are host pointers.  The caller owns every buffer it passes.  The
  *     library owns what sits behind an ed_op handle (state lists, hop tables,
  *     diagonal tables) or an ed_batch handle, freed by ed_op_destroy /
- *     ed_batch_destroy; a few calls (continued fraction, time sum, pair sums,
- *     dense eigensolvers, occupations, partially reorthogonalised Lanczos
- *     coefficients) allocate scratch and free it before returning.
+ *     ed_batch_destroy; a few calls (continued fraction, time sum, dense
+ *     eigensolvers, partially reorthogonalised Lanczos coefficients) allocate
+ *     scratch and free it before returning; pair sums and occupations take
+ *     theirs from the stream-ordered allocator (hipMallocAsync/hipFreeAsync on
+ *     `stream`) and return without waiting for the device.
  *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
  *     Device work is enqueued on it; functions that return host results
  *     (ed_lanczos_*, ed_batch_*) synchronise that stream before returning.
