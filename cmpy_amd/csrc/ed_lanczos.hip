@@ -622,8 +622,10 @@ static int pro_alloc(ProState& pro, ProBuffer& buf, hipStream_t s) {
 }
 
 // Step j: w = H v_j - beta_j v_{j-1}; alpha_j = <w, v_j>; w -= alpha_j v_j;
-// two classical Gram-Schmidt passes against v_0..v_j (every step, or, with
-// a ProState, only where the orthogonality estimate asks for them);
+// classical Gram-Schmidt against v_0..v_j: two passes on every step, or,
+// with a ProState, one pass where the orthogonality estimate asks for it
+// (the basis is then semi-orthogonal, |<v_i, v_k>| <= sqrt(eps), and w's
+// components along it are O(sqrt(eps) |w|), so one pass leaves O(eps));
 // beta_{j+1} = |w|; v_{j+1} = w / beta_{j+1}.
 static int reorth_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, const ReorthCtx& r, int j,
                        hipStream_t s, EventTimer* tm, const ProState* pro = nullptr) {
@@ -652,7 +654,7 @@ static int reorth_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, con
     hipLaunchKernelGGL(k_pro_omega, dim3(1), dim3(256), 0, s, *pro, c.alpha, c.beta, j, part, vb);
     gate = pro->flags;
   }
-  for (int pass = 0; pass < 2; ++pass) {
+  for (int pass = 0; pass < (pro ? 1 : 2); ++pass) {
     hipLaunchKernelGGL(k_multidot, dim3((unsigned)r.chunks, (unsigned)std::min(j + 1, kMultidotRows)),
                        dim3(kVecThreads), 0, s, r.V, D, r.w, r.chunk, part, (int)r.chunks, j + 1, gate);
     hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)(j + 1)), dim3(kVecThreads), 0, s, part, (int)r.chunks, r.coefs,

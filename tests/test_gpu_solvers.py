@@ -355,3 +355,43 @@ def test_workspace_and_numeric_errors(gpu):
     w, v = np.linalg.eigh(op.toarray())
     gs, st = groundstate(op, v0=v[:, 0], return_stats=True)
     assert st.breakdown and abs(gs.energy - w[0]) < 1e-12
+
+
+@pytest.mark.parametrize("mode, bound", [(1, 1e-13), (2, 1e-7)])
+def test_krylov_basis_orthogonality_by_mode(gpu, mode, bound):
+    """max |V^T V - I| of a 200-step Krylov basis (10-site chain, (5,5),
+    D = 63,504; H. D. Simon, Math. Comp. 42 (1984) 115-142): full
+    reorthogonalisation keeps it at rounding level, partial
+    reorthogonalisation (one Gram-Schmidt pass where the estimate passes
+    sqrt(eps) = 1.5e-8) keeps the basis semi-orthogonal.  Without
+    reorthogonalisation the same run loses orthogonality completely, so the
+    bounds test the mode."""
+    import ctypes as C
+
+    import torch
+
+    from cmpy_amd import HubbardModel, chain_neighbors
+    from cmpy_amd import _lib as L
+
+    m = HubbardModel(10, chain_neighbors(10), inter=2.0, eps=0.0, hop=1.0)
+    op = m.hamilton_operator(5, 5)
+    D, M = op.size, 200
+    phi = torch.from_numpy(np.random.default_rng(2).uniform(-1, 1, D)).cuda()
+
+    def basis_overlap(reorth):
+        nb = C.c_int64()
+        L.call("ed_lanczos_coeffs_workspace", op.handle, M, reorth, C.byref(nb))
+        work = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+        V = torch.zeros((M, D), dtype=torch.float64, device="cuda")
+        a, b = np.zeros(M), np.zeros(M)
+        n2, steps = C.c_double(), C.c_int32()
+        L.call("ed_lanczos_coeffs", op.handle, L.dptr(phi), M, reorth, a.ctypes.data_as(C.POINTER(C.c_double)),
+               b.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n2), C.byref(steps), L.dptr(V), L.dptr(work),
+               nb.value, L.stream_ptr())
+        assert steps.value == M
+        G = torch.mm(V, V.t().contiguous()).cpu().numpy()
+        return float(np.abs(G - np.eye(M)).max())
+
+    kept, plain = basis_overlap(mode), basis_overlap(0)
+    assert kept <= bound, kept
+    assert plain > 1e-2, plain
