@@ -138,7 +138,7 @@ def _dense_eigh_sectors(model, sectors, cache: dict, streams: int = 4) -> None:
     dimension groups are spread over up to ``streams`` HIP streams, one host
     thread each (the solver's many small kernels then overlap)."""
     by_dim = {}
-    for sec in sectors:
+    for sec in dict.fromkeys(sectors):  # each sector once, in first-seen order
         if sec in cache:
             continue
         op = model.hamilton_operator(n_up=sec[0], n_dn=sec[1])
@@ -250,13 +250,14 @@ class GreensFunctionMeasurement:
         self.pos = pos
         self.sigma = sigma
         self._measure_occ = measure_occ
-        self._part = 0.0
+        # partition function and occupations stay on the device while the
+        # sectors are accumulated, so accumulation never waits for the GPU
+        self._part_dev = torch.zeros((), dtype=torch.float64, device="cuda")
+        self._occ_dev = torch.zeros(2, dtype=torch.float64, device="cuda")
         self._gs_energy = np.inf
         self._gf = torch.zeros(2 * self.z.size, dtype=torch.float64, device="cuda")
         self._z_dev = torch.from_numpy(np.ascontiguousarray(
             np.stack([self.z.real.ravel(), self.z.imag.ravel()], axis=1).ravel())).cuda()
-        self._occ = 0.0
-        self._occ_double = 0.0
         self._eref = None
 
     def set_reference_energy(self, e: float) -> None:
@@ -266,6 +267,10 @@ class GreensFunctionMeasurement:
 
     def _shift(self) -> float:
         return self._gs_energy if self._eref is None else self._eref
+
+    @property
+    def _part(self) -> float:
+        return float(self._part_dev.item())
 
     @property
     def part(self):
@@ -278,11 +283,11 @@ class GreensFunctionMeasurement:
 
     @property
     def occ(self):
-        return self._occ / self._part
+        return float(self._occ_dev[0].item()) / self._part
 
     @property
     def occ_double(self):
-        return self._occ_double / self._part
+        return float(self._occ_dev[1].item()) / self._part
 
     @property
     def gs_energy(self):
@@ -293,19 +298,23 @@ class GreensFunctionMeasurement:
             return torch.exp(-self.beta * (w - self._shift()))
         return torch.ones_like(w)
 
-    def accumulate(self, num_sites, sector, sector_p1, w, V, w_p1, V_p1, sign_mode="reference"):
+    def accumulate(self, num_sites, sector, sector_p1, w, V, w_p1, V_p1, sign_mode="reference", min_energy=None):
         """Add one (sector, upper sector) pair; w/V, w_p1/V_p1 are device eigenpairs
-        (eigenvectors as rows)."""
-        min_energy = float(w[0].item())
+        (eigenvectors as rows, eigenvalues ascending).  ``min_energy`` (w[0] as a
+        host float, if the caller has it) spares a device round trip."""
+        if min_energy is None:
+            min_energy = float(w[0].item())
         factor = 1.0
         if min_energy < self._gs_energy:
             if self._eref is None:
                 factor = float(np.exp(-self.beta * (self._gs_energy - min_energy)))
             self._gs_energy = min_energy
         boltz = self._weights(w)
-        self._part = self._part * factor + float(boltz.sum().item())
         if factor != 1.0:
+            self._part_dev.mul_(factor)
+            self._occ_dev.mul_(factor)
             self._gf.mul_(factor)
+        self._part_dev.add_(boltz.sum())
         n_up, n_dn = sector
         N = V.shape[0]
         # C = c+ V (rows), overlap[m][n] = <m_p1 | c+ | n>
@@ -327,9 +336,7 @@ class GreensFunctionMeasurement:
                       _lib.dptr(boltz), _lib.dptr(out[0:1]), _lib.stream_ptr())
             _lib.call("ed_occupation", _lib.dptr(V), N, num_sites, n_up, n_dn, self.pos, 2,
                       _lib.dptr(boltz), _lib.dptr(out[1:2]), _lib.stream_ptr())
-            o = out.cpu().numpy()
-            self._occ = self._occ * factor + float(o[0])
-            self._occ_double = self._occ_double * factor + float(o[1])
+            self._occ_dev.add_(out)
 
 
 def gf_lehmann(model, z, beta, pos=0, sigma=UP, eig_cache=None, occ=True) -> GreensFunctionMeasurement:
@@ -349,19 +356,19 @@ def gf_lehmann(model, z, beta, pos=0, sigma=UP, eig_cache=None, occ=True) -> Gre
     pairs = [(sec, p1) for sec, p1 in pairs if p1 is not None]
     secs = [s for pr in pairs for s in pr]
     _dense_eigh_sectors(model, secs, cache)
-    if secs:
-        data.set_reference_energy(min(float(cache[s][0][0].item()) for s in set(secs)))
-
-    def solve(nu, nd):
-        return cache[(nu, nd)]
+    uniq = sorted(set(secs))
+    # lowest eigenvalue of every sector, in one device-to-host copy
+    emins = dict(zip(uniq, torch.stack([cache[s][0][0] for s in uniq]).cpu().tolist())) if uniq else {}
+    if uniq:
+        data.set_reference_energy(min(emins.values()))
 
     for n_up, n_dn in basis.iter_fillings():
         p1 = upper_sector(n_up, n_dn, sigma, ns)
         if p1 is None:
             continue
-        w, V = solve(n_up, n_dn)
-        w1, V1 = solve(*p1)
-        data.accumulate(ns, (n_up, n_dn), p1, w, V, w1, V1, sign_mode)
+        w, V = cache[(n_up, n_dn)]
+        w1, V1 = cache[p1]
+        data.accumulate(ns, (n_up, n_dn), p1, w, V, w1, V1, sign_mode, min_energy=emins[(n_up, n_dn)])
     return data
 
 
@@ -410,12 +417,11 @@ def greens_lehmann(model, z, beta, i, j, sigma=UP, eig_cache=None, sign_mode=Non
     cache = eig_cache if eig_cache is not None else {}
     fillings = list(model.basis.iter_fillings())
     _dense_eigh_sectors(model, fillings, cache)
-    emin = min(float(cache[sec][0][0].item()) for sec in fillings if cache[sec][0].numel())
-    part = 0.0
-    for sec in fillings:
-        w = cache[sec][0]
-        if w.numel():
-            part += float(torch.exp(-beta * (w - emin)).sum().item())
+    # all eigenvalues in one device-to-host copy (the spectra are small)
+    spectra = [cache[sec][0] for sec in fillings if cache[sec][0].numel()]
+    evals = torch.cat(spectra).cpu().numpy()
+    emin = float(evals.min())
+    part = float(np.exp(-beta * (evals - emin)).sum())
     zt = _z_device(z)
     g = torch.zeros(zt.numel(), dtype=torch.float64, device="cuda")
     bb = float(beta)

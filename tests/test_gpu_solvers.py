@@ -332,6 +332,42 @@ def test_gf_lehmann_low_temperature_no_overflow(gpu):
     assert abs(meas.occ - 0.5) < 1e-10
 
 
+def test_measurement_running_minimum_matches_fixed_reference(gpu):
+    """GreensFunctionMeasurement in the reference's own convention (no
+    reference energy set: weights measured from the lowest energy seen so far,
+    earlier sums rescaled by exp(-beta (E_old - E_new)) when a lower one
+    appears; cmpy/exactdiag.py:197-212) gives the same G, Z, occupations and
+    ground-state energy as gf_lehmann, which fixes the global minimum first.
+    The sector order of the impurity model makes the running minimum drop
+    several times, so the rescaling is exercised."""
+    from cmpy_amd import SingleImpurityAndersonModel, gf_lehmann
+    from cmpy_amd.basis import UP, upper_sector
+    from cmpy_amd.exactdiag import GreensFunctionMeasurement, _dense_eigh_sectors
+
+    m = SingleImpurityAndersonModel(4.0, -2.0, [-1.5, -0.4, 0.3, 1.2], [0.6, 0.5, 0.7, 0.4], mu=0.0)
+    z = np.linspace(-6, 6, 201) + 0.05j
+    beta, ns = 3.0, m.num_sites
+    ref = gf_lehmann(m, z, beta, occ=True)
+    data = GreensFunctionMeasurement(z, beta, 0, UP, measure_occ=True)
+    cache, drops, seen = {}, 0, np.inf
+    for nu, nd in m.basis.iter_fillings():
+        p1 = upper_sector(nu, nd, UP, ns)
+        if p1 is None:
+            continue
+        _dense_eigh_sectors(m, [(nu, nd), p1], cache)
+        w, V = cache[(nu, nd)]
+        w1, V1 = cache[p1]
+        e = float(w[0].item())
+        drops += e < seen
+        seen = min(seen, e)
+        data.accumulate(ns, (nu, nd), p1, w, V, w1, V1)
+    assert drops >= 3
+    assert_allclose(data.gf, ref.gf, rtol=0, atol=1e-12)
+    assert abs(data.part / ref.part - 1) < 1e-12
+    assert abs(data.occ - ref.occ) < 1e-12 and abs(data.occ_double - ref.occ_double) < 1e-12
+    assert data.gs_energy == ref.gs_energy
+
+
 def test_workspace_and_numeric_errors(gpu):
     import ctypes as C
     from cmpy_amd import HubbardModel, chain_neighbors, _lib as L
