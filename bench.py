@@ -286,8 +286,14 @@ def bench_chain12(args, log):
             torch.cuda.synchronize()
             reps.append(time.perf_counter() - t0)
         dt = float(np.median(reps))
+        # H.v bytes actually streamed at 16 D each (read x, write y): the
+        # ground-state steps plus its residual check in (6,6), D = 853,776,
+        # and the particle and hole fractions in (7,6) / (5,6), D = 731,808
+        n_cf = len(cf.alpha_p) + len(cf.alpha_h)
+        hv_bytes = 16 * (853776 * (st.iterations + 1) + 731808 * n_cf)
         per_u[U] = {"seconds": round(dt, 4), "repeats": [round(r, 4) for r in reps], "e0": gs.energy,
-                    "gs_steps": st.iterations, "residual": st.residual}
+                    "gs_steps": st.iterations, "cf_steps": n_cf, "residual": st.residual,
+                    "hv_bytes": hv_bytes, "hbm_fraction": round(hv_bytes / dt / 8.0e12, 5)}
         if U == 4.0:
             # parity inputs (untimed): coefficients, and G on a broadened axis
             # (eta = 1), where 200 steps have converged whether or not the
@@ -297,6 +303,9 @@ def bench_chain12(args, log):
             g_u4_b = greens_cf(model, zb, n, n, nsteps=200, gs=gs).gf
     total = sum(v["seconds"] for v in per_u.values())
     out = {"metric": "solves/s", "value": round(len(per_u) / total, 3), "unit": "solves/s", "dim": 853776,
+           "hbm_fraction": round(sum(v["hv_bytes"] for v in per_u.values()) / total / 8.0e12, 5),
+           "hbm_fraction_note": "H.v bytes (16 D per product) per second over 8 TB/s; the 6.8 MB vectors "
+                                "are served from L2/MALL, so this workload is latency-bound",
            "per_u": {str(k): v for k, v in per_u.items()}, "cf_steps": 200, "frequencies": 2048,
            "data": "deterministic model (no random input beyond the seeded Lanczos start)"}
     log(f"12-site chain: {out['value']} solves/s")
@@ -566,6 +575,18 @@ def bench_batched(args, log):
            "ground_state_steps_mean": float(r.iterations.mean()),
            "cf_steps_mean": [float(x) for x in r.cf_steps.mean(axis=0)],
            "data": f"synthetic (random_batch seed {seed})"}
+    # algorithmic bytes: per model (GS steps * D + CF steps * D_cf) * 40 bytes
+    # (16 D for the H.v, 24 D for the vector update), particle sector
+    # (n_up + 1, n_dn) and hole sector (n_up - 1, n_dn)
+    from math import comb
+
+    ns = np.asarray(b.num_bath) + 1
+    d_p = np.array([comb(int(n), int(a) + 1) * comb(int(n), int(c)) for n, a, c in zip(ns, b.n_up, b.n_dn)])
+    d_h = np.array([comb(int(n), int(a) - 1) * comb(int(n), int(c)) if a > 0 else 0
+                    for n, a, c in zip(ns, b.n_up, b.n_dn)])
+    alg = 40.0 * float(np.sum(b.dims * r.iterations + d_p * r.cf_steps[:, 0] + d_h * r.cf_steps[:, 1]))
+    out["algorithmic_bytes"] = alg
+    out["hbm_fraction"] = round(alg / (ms_med * 1e-3) / 8.0e12, 5)
     log(f"batched Anderson: {out['value']} solves/s ({ms_med:.1f} ms per 512)")
     # plain per-model GPU path on a sample: one model, one call chain at a time
     sample = list(range(0, M, M // 16))
