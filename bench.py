@@ -215,6 +215,9 @@ def main():
         log(f"CPU {t_cpu / runs * 1e3:.1f} ms/H.v; max rel diff GPU vs CPU {rel:.2e}")
         del tw, xh, yh
 
+    if not args.no_cpu:
+        result["host"] = _host_info()
+
     if not args.no_secondary:
         result["secondary"] = {}
         for name, fn_name in SECONDARY.items():
@@ -224,6 +227,28 @@ def main():
                 result["secondary"][name] = {"error": repr(exc)}
 
     print(json.dumps(result), flush=True)
+
+
+def _host_info():
+    """CPU model, cores available to this process, and the BLAS numpy uses
+    (the context of every cpu_baseline figure)."""
+    info = {"cores_available": len(os.sched_getaffinity(0)), "cpu_model": None, "blas": None}
+    try:
+        with open("/proc/cpuinfo", encoding="utf-8") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["cpu_model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        from threadpoolctl import threadpool_info
+
+        info["blas"] = sorted({f"{p.get('internal_api')} {p.get('version')}" for p in threadpool_info()
+                               if p.get("user_api") == "blas"})
+    except ImportError:  # pragma: no cover
+        pass
+    return info
 
 
 def _cpu_single_thread():
