@@ -18,7 +18,8 @@ from .models import SingleImpurityAndersonModel
 __all__ = [
     "IterationStats", "mix_values", "self_energy", "bethe_gf_omega", "quasiparticle_weight",
     "impurity_params_ref", "impurity_gf_ref", "impurity_gf0", "compute_impurity_gf", "compute_self_energy",
-    "twosite_dmft_half_filling", "compute_lattice_greens_function",
+    "twosite_dmft_half_filling", "compute_lattice_greens_function", "bethe_gf_matsubara", "hybridization",
+    "BetheDMFT", "fit_hybridization", "bethe_dmft_bath_fit",
 ]
 
 
@@ -216,3 +217,156 @@ def compute_lattice_greens_function(z, siam, t, ref=False):
     """G_lat(z) = G_Bethe(z + mu - Sigma(z)) (cmpy/dmft/twosite.py:175-178)."""
     sigma = compute_self_energy(z, siam, ref)
     return bethe_gf_omega(z + siam.mu - sigma, t)
+
+
+# -- Matsubara-axis DMFT with a fitted exact-diagonalisation bath ---------------------
+
+
+def bethe_gf_matsubara(zeta, t):
+    """Local Green's function of the Bethe lattice (semicircular density of
+    states of half bandwidth 2t) at complex argument zeta off the real axis:
+    (zeta - sqrt(zeta - 2t) sqrt(zeta + 2t)) / (2 t^2).  The product of two
+    principal roots keeps the branch with G ~ 1/zeta at infinity everywhere
+    off the cut [-2t, 2t]; on the real axis plus i eta it equals
+    ``bethe_gf_omega`` (cmpy/dmft/utils.py:210-226)."""
+    zeta = np.asarray(zeta, dtype=np.complex128)
+    return (zeta - np.sqrt(zeta - 2 * t) * np.sqrt(zeta + 2 * t)) / (2 * t * t)
+
+
+def hybridization(z, eps_bath, v):
+    """Delta(z) = sum_k |V_k|^2 / (z - eps_k) (cmpy/models/anderson.py:127-145);
+    eps_bath / v of shape [..., nb] broadcast against z of shape [nz]."""
+    z = np.asarray(z)[..., None]
+    eps_bath = np.asarray(eps_bath, dtype=np.float64)[..., None, :]
+    v = np.asarray(v, dtype=np.float64)[..., None, :]
+    return np.sum(v * v / (z - eps_bath), axis=-1)
+
+
+class BetheDMFT:
+    """Result of ``bethe_dmft_bath_fit``: per value of U the converged bath,
+    impurity and lattice Green's functions on the Matsubara axis, the
+    self-energy, and the iteration history."""
+
+    def __init__(self, u, wn, eps_bath, v, g_imp, g_loc, sigma, stats):
+        self.u, self.wn = u, wn
+        self.eps_bath, self.v = eps_bath, v
+        self.g_imp, self.g_loc, self.sigma = g_imp, g_loc, sigma
+        self.stats = stats
+
+    @property
+    def residual(self):
+        """max_n |G_imp - G_loc| per model: how well the finite bath meets
+        the lattice self-consistency."""
+        return np.abs(self.g_imp - self.g_loc).max(axis=1)
+
+    @property
+    def quasiparticle_weight(self):
+        """Z ~ 1 / (1 - Im Sigma(i w_0) / w_0), the lowest-frequency estimate."""
+        return 1.0 / (1.0 - self.sigma[:, 0].imag / self.wn[0])
+
+
+def fit_hybridization(wn, delta_target, eps0, v0, max_iter=200):
+    """Bath parameters with Delta(i w_n) = sum_k V_k^2 / (i w_n - eps_k) closest to
+    ``delta_target`` ([M, nw]) in sum_n |.|^2, per model (L-BFGS on the
+    closed-form Delta; no impurity solve).  Returns (eps [M, nb], v [M, nb])."""
+    import torch
+
+    iw = torch.as_tensor(1j * np.asarray(wn, dtype=np.float64))
+    target = torch.as_tensor(np.asarray(delta_target, dtype=np.complex128))
+    e = torch.tensor(np.asarray(eps0, dtype=np.float64), requires_grad=True)
+    v = torch.tensor(np.asarray(v0, dtype=np.float64), requires_grad=True)
+    opt = torch.optim.LBFGS([e, v], lr=1.0, max_iter=max_iter, tolerance_grad=0.0, tolerance_change=1e-30,
+                            history_size=20, line_search_fn="strong_wolfe")
+
+    def closure():
+        opt.zero_grad()
+        d = ((v * v)[:, None, :] / (iw[None, :, None] - e[:, None, :])).sum(dim=-1)
+        loss = (d - target).abs().pow(2).sum()
+        loss.backward()
+        return loss
+
+    opt.step(closure)
+    return e.detach().numpy().copy(), np.abs(v.detach().numpy())
+
+
+def bethe_dmft_bath_fit(u, beta, num_bath=5, t=1.0, n_matsubara=128, mixing=1.0, tol=1e-4, max_iter=30,
+                        fit="delta", fit_iter=40, eps0=None, v0=None, verbose=False) -> BetheDMFT:
+    """Single-site DMFT of the half-filled Hubbard model on the Bethe lattice
+    on the Matsubara axis, with exact diagonalisation of an Anderson model of
+    ``num_bath`` bath sites as the impurity solver.
+
+    ``u`` may be a list: every value is one impurity model of a single
+    batched solve, so a whole U sweep iterates together.  Each iteration
+    solves all models (``AndersonBatch``), forms Sigma = G0^-1 - G^-1 with
+    G0^-1 = i w_n - (eps_imp - mu) - Delta, the lattice function
+    G_loc = G_Bethe(i w_n - (eps_imp - mu) - Sigma) = G_Bethe(1/G + Delta), and
+    refits the bath, mixed with the old one.  ``fit="delta"`` (the usual
+    exact-diagonalisation DMFT update): on the Bethe lattice the
+    self-consistent hybridisation is t^2 G_loc, and the bath's closed-form
+    Delta is fitted to it (``fit_hybridization``).  ``fit="g"``: the bath is
+    fitted so that the interacting impurity G itself matches G_loc
+    (``fit_bath``: L-BFGS on adjoint gradients of the exact-diagonalisation G,
+    ``fit_iter`` iterations).  That minimises the self-consistency residual
+    directly, and is meant to refine a solution of the hybridisation fit
+    (pass its bath as ``eps0`` / ``v0``): from a generic start bath the G fit
+    can settle on another solution (measured: an insulating one at U = 2).  It stops when the
+    impurity G of every model changes by less than ``tol`` (max over n) from
+    one iteration to the next, as the reference stops on the change of its
+    hybridisation (cmpy/dmft/twosite.py:144-156).  With a finite bath G cannot
+    equal G_loc exactly: the remaining max_n |G - G_loc| (the fit's
+    discretisation error) is reported per model in ``BetheDMFT.residual``.  The reference
+    iterates the two-site version on the real axis (cmpy/dmft/twosite.py:
+    103-172); this is the same self-consistency with a fitted bath of any
+    size.  Half filling: eps_imp - mu = -U/2, a particle-hole symmetric
+    start bath, and an odd ``num_bath``, so that the 1 + num_bath sites hold
+    the half-filled spin singlet ((num_bath + 1) / 2 electrons of each spin),
+    the sector the batched solver works in."""
+    from .batch import AndersonBatch, fit_bath, matsubara
+
+    us = np.atleast_1d(np.asarray(u, dtype=np.float64))
+    M, nb = len(us), int(num_bath)
+    if nb < 1 or nb > 15 or nb % 2 == 0:
+        raise ValueError("num_bath must be odd and lie in 1..15 (half-filled singlet ground state)")
+    if not 0 < mixing <= 1:
+        raise ValueError("mixing must lie in (0, 1]")
+    if fit not in ("delta", "g"):
+        raise ValueError("fit must be 'delta' or 'g'")
+    wn = matsubara(beta, n_matsubara)
+    iw = 1j * wn
+    if eps0 is None:
+        eps0 = np.linspace(-2 * t, 2 * t, nb) * (nb - 1) / nb if nb > 1 else np.zeros(1)
+    if v0 is None:
+        v0 = np.full(nb, t / np.sqrt(nb))  # sum_k V_k^2 = t^2, the Bethe lattice's hybridisation weight
+    eps = np.broadcast_to(np.asarray(eps0, dtype=np.float64), (M, nb)).copy()
+    v = np.broadcast_to(np.asarray(v0, dtype=np.float64), (M, nb)).copy()
+    level = -us / 2  # eps_imp - mu at half filling
+    # physical (anticommuting) hopping signs: with the reference's sign-free
+    # hopping the U = 0 impurity is not the free resolvent 1/(i w_n - Delta)
+    batch = AndersonBatch(us, 0.0, list(eps), list(v), mu=us / 2, sign_mode="fermion", keep_basis=True)
+    stats = IterationStats("max|dG|", "max|G-G_loc|")
+    g_prev = None
+    for it in range(max_iter + 1):
+        batch.set_bath(eps, v)
+        g = batch.solve(wn).g
+        delta = hybridization(iw, eps, v)
+        sigma = (iw[None, :] - level[:, None] - delta) - 1.0 / g
+        g_loc = bethe_gf_matsubara(1.0 / g + delta, t)
+        change = np.inf if g_prev is None else float(np.abs(g - g_prev).max())
+        stats.append(change, float(np.abs(g - g_loc).max()))
+        if verbose:
+            print(f"DMFT iteration {it}: " + stats.errorstr())
+        if change < tol:
+            stats.set_status(True, f"G changed by {change:.2e} in the last of {it} bath fits")
+            break
+        if it == max_iter:
+            stats.set_maxiter_status(max_iter)
+            break
+        g_prev = g
+        if fit == "delta":
+            e_new, v_new = fit_hybridization(wn, t * t * g_loc, eps, v)
+        else:
+            res = fit_bath(batch, wn, g_loc, eps0=eps, v0=v, max_iter=fit_iter)
+            e_new, v_new = res.eps_bath[:, :nb], np.abs(res.v[:, :nb])
+        eps = mix_values(eps, e_new, mixing)
+        v = mix_values(v, v_new, mixing)
+    return BetheDMFT(us, wn, eps, v, g, g_loc, sigma, stats)

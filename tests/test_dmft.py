@@ -74,3 +74,82 @@ def test_twosite_loop_with_device_solvers(gpu, u):
     g_ref = dmft.compute_lattice_greens_function(z, s_ref, 1.0, ref=True)
     g_dev = dmft.compute_lattice_greens_function(z, s_ref, 1.0, ref=False)
     assert np.abs(g_dev - g_ref).max() < 1e-8
+
+
+# -- Matsubara-axis DMFT with a fitted bath -------------------------------------------
+
+
+def test_bethe_gf_matsubara_branch_and_asymptotics():
+    """Same function as bethe_gf_omega (cmpy/dmft/utils.py:210-226) on the
+    real axis plus i eta; 1/zeta at large |zeta|; on the imaginary axis the
+    closed form -i (sqrt(w^2 + 4 t^2) - w) / (2 t^2)."""
+    t = 0.7
+    z = np.linspace(-3, 3, 301) + 0.05j
+    assert_allclose(dmft.bethe_gf_matsubara(z, t), dmft.bethe_gf_omega(z, t), rtol=0, atol=1e-13)
+    w = np.pi / 20 * (2 * np.arange(50) + 1)
+    assert_allclose(dmft.bethe_gf_matsubara(1j * w, t), -1j * (np.sqrt(w**2 + 4 * t * t) - w) / (2 * t * t),
+                    rtol=1e-13)
+    big = 1e4 * np.exp(1j * np.linspace(0.1, 3.0, 7))
+    assert_allclose(dmft.bethe_gf_matsubara(big, t) * big, 1.0, rtol=1e-7)
+
+
+def test_hybridization_matches_model_and_fit_recovers_bath():
+    from cmpy_amd import SingleImpurityAndersonModel
+
+    eps, v = np.array([-1.2, -0.5, 0.0, 0.5, 1.2]), np.array([0.5, 0.3, 0.2, 0.3, 0.5])
+    wn = np.pi / 50 * (2 * np.arange(64) + 1)
+    m = SingleImpurityAndersonModel(2.0, 0.0, eps, v, mu=1.0)
+    assert_allclose(dmft.hybridization(1j * wn, eps, v), m.hybridization_func(1j * wn), rtol=1e-14)
+    target = dmft.hybridization(1j * wn, eps[None], v[None])
+    e, vv = dmft.fit_hybridization(wn, target, np.array([[-1.0, -0.3, 0.0, 0.3, 1.0]]), np.full((1, 5), 0.4))
+    assert np.abs(dmft.hybridization(1j * wn, e, vv) - target).max() < 1e-4
+    assert_allclose(np.sort(e[0]), eps, atol=1e-2)
+
+
+def test_bethe_dmft_argument_checks():
+    with pytest.raises(ValueError):
+        dmft.bethe_dmft_bath_fit(2.0, 50.0, num_bath=4)
+    with pytest.raises(ValueError):
+        dmft.bethe_dmft_bath_fit(2.0, 50.0, fit="chi2")
+
+
+@pytest.mark.gpu
+def test_bethe_dmft_non_interacting_limit(gpu):
+    """At U = 0 the DMFT solution is the lattice's own G_Bethe(i w_n): the
+    loop reaches it up to the bath's discretisation error, which shrinks
+    with the bath size."""
+    err = {}
+    for nb in (5, 7):
+        r = dmft.bethe_dmft_bath_fit(0.0, 50.0, num_bath=nb, tol=1e-8, max_iter=10)
+        assert r.stats.success, r.stats.message
+        err[nb] = np.abs(r.g_imp[0] - dmft.bethe_gf_matsubara(1j * r.wn, 1.0)).max()
+        assert abs(r.quasiparticle_weight[0] - 1) < 1e-10
+    assert err[5] < 1.5e-2 and err[7] < 1e-3 and err[7] < err[5] / 5, err
+
+
+@pytest.mark.gpu
+def test_bethe_dmft_metal_and_mott_insulator(gpu):
+    """One batched loop over U = 2 and U = 7 (half bandwidth 2, beta = 50):
+    a correlated metal (Z ~ 0.74, -Im G(i w_0) near its U = 0 value) and a
+    Mott insulator (Z ~ 0, G(i w_0) ~ 0), both particle-hole symmetric."""
+    r = dmft.bethe_dmft_bath_fit([2.0, 7.0], 50.0, num_bath=5, tol=1e-4, max_iter=60)
+    assert r.stats.success, r.stats.message
+    z = r.quasiparticle_weight
+    assert 0.6 < z[0] < 0.85 and z[1] < 0.02, z
+    assert -r.g_imp[0, 0].imag > 0.9 and -r.g_imp[1, 0].imag < 0.05
+    assert np.abs(r.g_imp.real).max() < 1e-4
+    assert np.all(r.residual < 0.03)
+
+
+@pytest.mark.gpu
+def test_bethe_dmft_g_fit_refines_the_metal(gpu):
+    """Started from the hybridisation-fit solution at U = 2, the update that
+    fits the interacting G through the adjoint gradients (fit_bath) stays in
+    the same correlated metal and lowers the self-consistency residual
+    max_n |G - G_loc|."""
+    rd = dmft.bethe_dmft_bath_fit(2.0, 50.0, num_bath=5, tol=1e-5, max_iter=40)
+    rg = dmft.bethe_dmft_bath_fit(2.0, 50.0, num_bath=5, tol=1e-5, max_iter=8, fit="g", eps0=rd.eps_bath[0],
+                                  v0=rd.v[0])
+    assert rg.residual[0] < 0.7 * rd.residual[0], (rg.residual, rd.residual)
+    assert np.abs(rd.g_imp - rg.g_imp).max() < 2e-2
+    assert abs(rd.quasiparticle_weight[0] - rg.quasiparticle_weight[0]) < 2e-2
