@@ -43,7 +43,7 @@ extern "C" {
 #define ED_ERR_HIP (-2)      /* a HIP runtime call failed                     */
 #define ED_ERR_NOMEM (-3)    /* device allocation failed                      */
 #define ED_ERR_NUMERIC (-4)  /* NaN/Inf met, or a solve did not converge       */
-#define ED_ERR_LIMIT (-5)    /* size beyond what this build supports          */
+#define ED_ERR_LIMIT (-5)    /* size beyond what the library supports         */
 
 #define ED_UP 1 /* cmpy/basis.py:40 */
 #define ED_DN 2
