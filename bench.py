@@ -33,7 +33,8 @@ ROOT = Path(__file__).resolve().parent
 # secondary workloads, in the order they run
 SECONDARY = {"chain12_solves": "bench_chain12", "chain12_realtime_g": "bench_tevo12",
              "finite_t_lehmann": "bench_finite_t", "offdiagonal_gij": "bench_gij",
-             "batched_anderson": "bench_batched", "bathfit_gradients": "bench_bathfit"}
+             "batched_anderson": "bench_batched", "bathfit_gradients": "bench_bathfit",
+             "dmft_bethe_sweep": "bench_dmft"}
 
 
 def parse():
@@ -644,6 +645,76 @@ def bench_batched(args, log):
         out["cpu_baseline"] = {"value": round(done / t, 3), "unit": "solves/s", "cores": 1, "kind": "port",
                                "sample": f"{done} models of the same batch (every 7th) by the oracle's numpy/scipy "
                                          "port: ground state + continued fractions"}
+    return out
+
+
+def bench_dmft(args, log):
+    """Matsubara-axis DMFT of the half-filled Bethe-lattice Hubbard model
+    (half bandwidth 2): a sweep of 8 U values in [0.5, 7.5] iterated together,
+    5 bath sites (D = 400), beta = 50, 128 Matsubara frequencies, bath update
+    by hybridisation fit, to max |dG| < 1e-5 (cmpy/dmft/twosite.py:103-172 is
+    the reference's two-site loop).  Unit: DMFT iterations per second over a
+    fixed 30 iterations, one iteration = 8 impurity solves + the
+    self-consistency and bath fits."""
+    import time
+
+    import numpy as np
+
+    from cmpy_amd.dmft import bethe_dmft_bath_fit
+
+    us = np.linspace(0.5, 7.5, 8)
+    beta, nb, nw = 50.0, 5, 128
+    bethe_dmft_bath_fit(us, beta, num_bath=nb, n_matsubara=nw, max_iter=1)  # warm
+    t0 = time.perf_counter()
+    r = bethe_dmft_bath_fit(us, beta, num_bath=nb, n_matsubara=nw, tol=0.0, max_iter=29)
+    dt = time.perf_counter() - t0
+    its = r.stats.iteration
+    out = {"metric": "DMFT iterations/s", "value": round(its / dt, 3), "unit": "iterations/s", "models": len(us),
+           "bath_sites": nb, "beta": beta, "matsubara": nw, "iterations": its, "seconds": round(dt, 3),
+           "last_max_dG": r.stats.errors[0], "residual": [round(float(x), 5) for x in r.residual],
+           "quasiparticle_weight": [round(float(z), 5) for z in r.quasiparticle_weight],
+           "data": "deterministic (particle-hole symmetric start bath)"}
+    log(f"DMFT sweep: {its} iterations in {dt:.2f} s")
+    if not args.no_cpu:
+        import sys
+
+        sys.path.insert(0, str(ROOT))
+        from oracle import ed_oracle as O
+
+        # the oracle's impurity solves for one iteration (8 models, fermionic
+        # signs as the device loop uses), timed; parity on the converged bath
+        wn = r.wn
+        n = (nb + 1) // 2
+        with _cpu_single_thread():
+            t0 = time.perf_counter()
+            g_cpu = []
+            for i, u in enumerate(us):
+                mk = lambda nu, nd: O.siam_operator(nu, nd, u, 0.0, r.eps_bath[i], r.v[i], u / 2, width=nb + 1)
+                e0, psi = O.groundstate(mk(n, n), tol=1e-12)
+                g = np.zeros(len(wn), dtype=np.complex128)
+                for create, sign in ((True, 1), (False, -1)):
+                    phi = (O.apply_cdag if create else O.apply_c)(psi, nb + 1, n, n, 0, O.UP, "jordan_wigner")
+                    opx = mk(n + 1 if create else n - 1, n)
+                    a, b, n2, _ = O.lanczos_normalized(opx.matvec, phi, opx.size, reorth=True)
+                    g += O.cf_greens(a, b, n2, 1j * wn, e0, sign)
+                g_cpu.append(g)
+            t = time.perf_counter() - t0
+        # a model whose sector ground state is (nearly) degenerate -- the Mott
+        # side, where a bath level decouples -- has no unique T = 0 G, so the
+        # parity figure covers the models with a gap above 1e-6
+        gaps = []
+        for i, u in enumerate(us):
+            ev = np.linalg.eigvalsh(O.siam_operator(n, n, u, 0.0, r.eps_bath[i], r.v[i], u / 2,
+                                                    width=nb + 1).csr().toarray())
+            gaps.append(float(ev[1] - ev[0]))
+        diff = np.abs(np.array(g_cpu) - r.g_imp).max(axis=1)
+        ok = np.array(gaps) > 1e-6
+        out["parity_sample"] = {"max_abs_diff_g_imp": float(diff[ok].max()) if ok.any() else None,
+                                "models_compared": int(ok.sum()),
+                                "sector_gaps": [round(g, 8) for g in gaps]}
+        out["cpu_baseline"] = {"value": round(1.0 / t, 4), "unit": "iterations/s", "cores": 1, "kind": "port",
+                               "sample": f"the oracle's impurity solves of one iteration (8 models, D = 400, "
+                                         f"full-Krylov continued fractions; {t:.2f} s), bath fits not included"}
     return out
 
 
