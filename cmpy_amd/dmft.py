@@ -267,26 +267,30 @@ class BetheDMFT:
 
 def fit_hybridization(wn, delta_target, eps0, v0, max_iter=200):
     """Bath parameters with Delta(i w_n) = sum_k V_k^2 / (i w_n - eps_k) closest to
-    ``delta_target`` ([M, nw]) in sum_n |.|^2, per model (L-BFGS on the
-    closed-form Delta; no impurity solve).  Returns (eps [M, nb], v [M, nb])."""
-    import torch
+    ``delta_target`` ([M, nw]) in sum_n |.|^2, per model: L-BFGS-B (scipy) on the
+    closed-form Delta and its analytic gradient, no impurity solve.  Returns
+    (eps [M, nb], v [M, nb]) with v >= 0."""
+    from scipy.optimize import minimize
 
-    iw = torch.as_tensor(1j * np.asarray(wn, dtype=np.float64))
-    target = torch.as_tensor(np.asarray(delta_target, dtype=np.complex128))
-    e = torch.tensor(np.asarray(eps0, dtype=np.float64), requires_grad=True)
-    v = torch.tensor(np.asarray(v0, dtype=np.float64), requires_grad=True)
-    opt = torch.optim.LBFGS([e, v], lr=1.0, max_iter=max_iter, tolerance_grad=0.0, tolerance_change=1e-30,
-                            history_size=20, line_search_fn="strong_wolfe")
+    iw = 1j * np.asarray(wn, dtype=np.float64)
+    target = np.asarray(delta_target, dtype=np.complex128)
+    e0 = np.asarray(eps0, dtype=np.float64)
+    shape = e0.shape
 
-    def closure():
-        opt.zero_grad()
-        d = ((v * v)[:, None, :] / (iw[None, :, None] - e[:, None, :])).sum(dim=-1)
-        loss = (d - target).abs().pow(2).sum()
-        loss.backward()
-        return loss
+    def loss_and_grad(x):
+        e, v = x[: e0.size].reshape(shape), x[e0.size:].reshape(shape)
+        r = 1.0 / (iw[None, :, None] - e[:, None, :])          # [M, nw, nb]
+        res = (v * v)[:, None, :] * r
+        diff = res.sum(axis=-1) - target                      # [M, nw]
+        c = np.conj(diff)[..., None]
+        ge = 2.0 * np.real(c * res * r).sum(axis=1)           # d|diff|^2 / d eps_k
+        gv = 4.0 * np.real(c * v[:, None, :] * r).sum(axis=1)  # d|diff|^2 / d V_k
+        return float(np.sum(np.abs(diff) ** 2)), np.concatenate([ge.ravel(), gv.ravel()])
 
-    opt.step(closure)
-    return e.detach().numpy().copy(), np.abs(v.detach().numpy())
+    x0 = np.concatenate([e0.ravel(), np.asarray(v0, dtype=np.float64).ravel()])
+    sol = minimize(loss_and_grad, x0, jac=True, method="L-BFGS-B",
+                   options={"maxiter": int(max_iter), "ftol": 1e-16, "gtol": 1e-14, "maxcor": 20})
+    return sol.x[: e0.size].reshape(shape).copy(), np.abs(sol.x[e0.size:].reshape(shape))
 
 
 def bethe_dmft_bath_fit(u, beta, num_bath=5, t=1.0, n_matsubara=128, mixing=1.0, tol=1e-4, max_iter=30,
