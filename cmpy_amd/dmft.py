@@ -265,6 +265,20 @@ class BetheDMFT:
         return 1.0 / (1.0 - self.sigma[:, 0].imag / self.wn[0])
 
 
+def _hyb_loss_and_grad(x, iw, target, shape):
+    """L = sum |Delta(i w_n; eps, V) - target|^2 and dL/d(eps, V) for the
+    flattened parameters x = (eps, V), each of ``shape`` [M, nb]."""
+    n = int(np.prod(shape))
+    e, v = x[:n].reshape(shape), x[n:].reshape(shape)
+    r = 1.0 / (iw[None, :, None] - e[:, None, :])          # [M, nw, nb]
+    res = (v * v)[:, None, :] * r
+    diff = res.sum(axis=-1) - target                      # [M, nw]
+    c = np.conj(diff)[..., None]
+    ge = 2.0 * np.real(c * res * r).sum(axis=1)           # d|diff|^2 / d eps_k
+    gv = 4.0 * np.real(c * v[:, None, :] * r).sum(axis=1)  # d|diff|^2 / d V_k
+    return float(np.sum(np.abs(diff) ** 2)), np.concatenate([ge.ravel(), gv.ravel()])
+
+
 def fit_hybridization(wn, delta_target, eps0, v0, max_iter=200):
     """Bath parameters with Delta(i w_n) = sum_k V_k^2 / (i w_n - eps_k) closest to
     ``delta_target`` ([M, nw]) in sum_n |.|^2, per model: L-BFGS-B (scipy) on the
@@ -276,19 +290,8 @@ def fit_hybridization(wn, delta_target, eps0, v0, max_iter=200):
     target = np.asarray(delta_target, dtype=np.complex128)
     e0 = np.asarray(eps0, dtype=np.float64)
     shape = e0.shape
-
-    def loss_and_grad(x):
-        e, v = x[: e0.size].reshape(shape), x[e0.size:].reshape(shape)
-        r = 1.0 / (iw[None, :, None] - e[:, None, :])          # [M, nw, nb]
-        res = (v * v)[:, None, :] * r
-        diff = res.sum(axis=-1) - target                      # [M, nw]
-        c = np.conj(diff)[..., None]
-        ge = 2.0 * np.real(c * res * r).sum(axis=1)           # d|diff|^2 / d eps_k
-        gv = 4.0 * np.real(c * v[:, None, :] * r).sum(axis=1)  # d|diff|^2 / d V_k
-        return float(np.sum(np.abs(diff) ** 2)), np.concatenate([ge.ravel(), gv.ravel()])
-
     x0 = np.concatenate([e0.ravel(), np.asarray(v0, dtype=np.float64).ravel()])
-    sol = minimize(loss_and_grad, x0, jac=True, method="L-BFGS-B",
+    sol = minimize(_hyb_loss_and_grad, x0, args=(iw, target, shape), jac=True, method="L-BFGS-B",
                    options={"maxiter": int(max_iter), "ftol": 1e-16, "gtol": 1e-14, "maxcor": 20})
     return sol.x[: e0.size].reshape(shape).copy(), np.abs(sol.x[e0.size:].reshape(shape))
 

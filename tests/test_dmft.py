@@ -106,6 +106,21 @@ def test_hybridization_matches_model_and_fit_recovers_bath():
     assert_allclose(np.sort(e[0]), eps, atol=1e-2)
 
 
+def test_hybridization_fit_gradient_matches_central_differences():
+    from cmpy_amd.dmft import _hyb_loss_and_grad
+
+    rng = np.random.default_rng(3)
+    iw = 1j * np.pi / 20 * (2 * np.arange(40) + 1)
+    shape = (2, 3)
+    target = rng.standard_normal((2, 40)) + 1j * rng.standard_normal((2, 40))
+    x = np.concatenate([rng.uniform(-2, 2, 6), rng.uniform(0.2, 1.0, 6)])
+    _, g = _hyb_loss_and_grad(x, iw, target, shape)
+    h = 1e-6
+    fd = np.array([(_hyb_loss_and_grad(x + h * e, iw, target, shape)[0]
+                    - _hyb_loss_and_grad(x - h * e, iw, target, shape)[0]) / (2 * h) for e in np.eye(12)])
+    assert_allclose(g, fd, rtol=1e-6, atol=1e-8)
+
+
 def test_bethe_dmft_argument_checks():
     with pytest.raises(ValueError):
         dmft.bethe_dmft_bath_fit(2.0, 50.0, num_bath=4)
