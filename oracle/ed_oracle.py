@@ -1095,4 +1095,29 @@ PARITY = {
     "resolvent_ij": "direct inversion of z - h",
     "lehmann_t0_ij": "U = 0 closed-shell ground state: resolvent_ij at interior sites; i = j equals lehmann_t0",
     "eigvalsh_chain": "eps + 2 t cos(k pi / (N + 1)), k = 1..N (open-chain closed form)",
+    "binom/popcount": "through combinadic_rank and sector_states (rank = position for every ns<=12)",
+    "bit_count_width": "through hopping_sign: hand cases and the free-fermion closed forms",
+    "onsite_energy_triples/hubbard_inter_triples/hopping_triples/hubbard_triples/_concat":
+    "operators.py:337-341, 391-395, 503-507 docstring matrices; test_models_hubbard.py:14-25 "
+    "(sector (1,1)) and Hermiticity over every sector (test_models_hubbard.py:28-48)",
+    "siam_triples": "factorised operator equals the COO assembly; measured reference-convention energies",
+    "hubbard_operator/siam_operator": "COO assembly at small sizes; free-fermion closed forms; "
+    "atomic limit; SIAM reference-convention energies and the U = 0 fermion-sign closed form",
+    "coo_csr/coo_trace": "dense assembly; trace equals the sum of the factorised diagonal",
+    "groundstate": "dense eigh of the same matrix; free-fermion and atomic closed forms",
+    "lanczos_coeffs_reference/lanczos_ground_state": "tridiagonal spectrum gives the dense-eigh E0",
+    "_jw_sign": "canonical anticommutation of the Jordan-Wigner ladder operators",
+    "GreensFunctionMeasurement/_accumulate_sum": "through gf_lehmann (test_exactdiag.py:24-96)",
+    "occupation": "atomic limit per site (four-state atom, t = 0) and the U = 0 Fermi sum "
+    "sum_k |phi_k(pos)|^2 f(e_k) with uneven levels and hoppings, both spins",
+    "double_occupation": "atomic limit e^{-b(2e+U)}/Z_i; U = 0 Wick factorisation "
+    "<n_up n_dn> = <n_up><n_dn>; planted index mistakes (row-reset, dn-major, "
+    "selected-only increment) fail the Wick pin",
+    "hybridization_func": "U = 0 impurity G (lehmann_ij, itself the resolvent) equals "
+    "1/(z - (eps_imp - mu) - Delta); Delta equals the directly solved bath block",
+    "impurity_gf0": "equals the U = 0 impurity G at eps_imp = 0 and, for eps_imp != 0, that "
+    "of the reflected level -eps_imp (the sign anderson.py:160-161 writes)",
+    "impurity_params_ref": "through impurity_gf_ref (lehmann_t0 on the two-site SIAM)",
+    "chain_bonds/square_bonds/free_fermion_energy": "tight-binding levels 2t cos(2 pi k/N) "
+    "and 2t(cos kx + cos ky) summed to the free-fermion energy; bond counts per site",
 }

@@ -99,7 +99,13 @@ def test_hybridization_matches_model_and_fit_recovers_bath():
     eps, v = np.array([-1.2, -0.5, 0.0, 0.5, 1.2]), np.array([0.5, 0.3, 0.2, 0.3, 0.5])
     wn = np.pi / 50 * (2 * np.arange(64) + 1)
     m = SingleImpurityAndersonModel(2.0, 0.0, eps, v, mu=1.0)
-    assert_allclose(dmft.hybridization(1j * wn, eps, v), m.hybridization_func(1j * wn), rtol=1e-14)
+    # against the oracle's restatements, which are pinned to the U = 0 impurity G
+    from oracle import ed_oracle as O
+
+    assert_allclose(m.hybridization_func(1j * wn), O.hybridization_func(1j * wn, v, eps), rtol=1e-14)
+    assert_allclose(dmft.hybridization(1j * wn, eps, v), O.hybridization_func(1j * wn, v, eps), rtol=1e-14)
+    m2 = SingleImpurityAndersonModel(2.0, 0.6, eps, v, mu=0.4)
+    assert_allclose(m2.impurity_gf0(1j * wn), O.impurity_gf0(1j * wn, 2.0, 0.6, v, eps, 0.4), rtol=1e-14)
     target = dmft.hybridization(1j * wn, eps[None], v[None])
     e, vv = dmft.fit_hybridization(wn, target, np.array([[-1.0, -0.3, 0.0, 0.3, 1.0]]), np.full((1, 5), 0.4))
     assert np.abs(dmft.hybridization(1j * wn, e, vv) - target).max() < 1e-4

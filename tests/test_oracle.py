@@ -508,3 +508,199 @@ def test_bathfit_grad_dense_matches_finite_differences(nb, width):
     fd = np.array(fd)
     got = np.concatenate([ge, gv])
     assert np.max(np.abs(got - fd)) <= 1e-6 * np.max(np.abs(fd))
+
+
+# -- occupations, double occupation, hybridisation function -------------------------
+
+
+def _sector_spectra(make_op, ns):
+    out = {}
+    for nu in range(ns + 1):
+        for nd in range(ns + 1):
+            op = make_op(nu, nd)
+            w, v = np.linalg.eigh(op.csr().toarray())
+            out[(nu, nd)] = (op, w, v)
+    return out
+
+
+def _thermal(make_op, ns, beta, pos, dbl=None):
+    """Grand-canonical <n_up>, <n_dn>, <n_up n_dn> at `pos`: the per-sector sums
+    of occupation / double_occupation over every sector, divided by Z."""
+    dbl = dbl or O.double_occupation
+    spec = _sector_spectra(make_op, ns)
+    emin = min(w[0] for _, w, _ in spec.values())
+    z = nu_ = nd_ = d_ = 0.0
+    for op, w, v in spec.values():
+        z += np.exp(-beta * (w - emin)).sum()
+        nu_ += O.occupation(op.up, op.dn, w, v, beta, emin, pos, O.UP)
+        nd_ += O.occupation(op.up, op.dn, w, v, beta, emin, pos, O.DN)
+        d_ += dbl(op.up, op.dn, w, v, beta, emin, pos)
+    return nu_ / z, nd_ / z, d_ / z
+
+
+_ATOMIC_EPS = np.array([0.3, -0.7, 1.1])
+_ATOMIC_U = np.array([2.0, 4.0, 0.5])
+
+
+def _atomic_op(nu, nd):
+    return O.SectorOperator(3, nu, nd, [], [], _ATOMIC_EPS, _ATOMIC_U, 3)
+
+
+@pytest.mark.parametrize("beta", [0.7, 2.5])
+def test_occupations_atomic_limit(beta):
+    """t = 0: every site is an independent four-state atom with levels
+    0, eps, eps, 2 eps + U, so <n_s> = (e^{-b eps} + e^{-b(2eps+U)})/Z_i and
+    <n_up n_dn> = e^{-b(2eps+U)}/Z_i, site by site."""
+    for pos in range(3):
+        e, u = _ATOMIC_EPS[pos], _ATOMIC_U[pos]
+        w1, w2 = np.exp(-beta * e), np.exp(-beta * (2 * e + u))
+        zi = 1 + 2 * w1 + w2
+        n_up, n_dn, d = _thermal(_atomic_op, 3, beta, pos)
+        assert abs(n_up - (w1 + w2) / zi) < 1e-13
+        assert abs(n_dn - (w1 + w2) / zi) < 1e-13
+        assert abs(d - w2 / zi) < 1e-13
+
+
+def _u0_chain_op(nu, nd):
+    eps = np.array([0.4, -0.3, 0.9, -1.2])
+    bonds = O.chain_bonds(4)
+    return O.SectorOperator(4, nu, nd, bonds, [1.0, 0.6, 1.3], eps, np.zeros(4), 4)
+
+
+def _u0_chain_occ(beta, pos):
+    h = np.diag([0.4, -0.3, 0.9, -1.2])
+    for (i, j), t in zip(O.chain_bonds(4), [1.0, 0.6, 1.3]):
+        h[i, j] = h[j, i] = t
+    w, v = np.linalg.eigh(h)
+    return float(np.sum(v[pos] ** 2 / (1 + np.exp(beta * w))))
+
+
+@pytest.mark.parametrize("pos", [0, 1, 3])
+def test_occupations_u0_wick(pos):
+    """U = 0 (fermionic signs, uneven levels and hoppings): <n_s> at a site is
+    the Fermi sum over single-particle orbitals, and <n_up n_dn> factorises
+    into <n_up><n_dn> (the two spins are independent)."""
+    beta = 1.7
+    n_ref = _u0_chain_occ(beta, pos)
+    n_up, n_dn, d = _thermal(_u0_chain_op, 4, beta, pos)
+    assert abs(n_up - n_ref) < 1e-12 and abs(n_dn - n_ref) < 1e-12
+    assert abs(d - n_ref * n_ref) < 1e-12
+
+
+def _dbl_running_idx_reset(up_states, dn_states, evals, evecs, beta, emin, pos):
+    # planted mistake: the running index restarts for every up state
+    boltz = np.exp(-beta * (evals - emin))
+    occ = 0.0
+    for ui, up in enumerate(up_states):
+        for di, dn in enumerate(dn_states):
+            if up & dn & (1 << pos):
+                occ += np.sum(boltz * evecs[di, :] ** 2)
+    return occ
+
+
+def _dbl_dn_major(up_states, dn_states, evals, evecs, beta, emin, pos):
+    # planted mistake: the flat index is taken dn-major
+    boltz = np.exp(-beta * (evals - emin))
+    occ = 0.0
+    for ui, up in enumerate(up_states):
+        for di, dn in enumerate(dn_states):
+            if up & dn & (1 << pos):
+                occ += np.sum(boltz * evecs[di * len(up_states) + ui, :] ** 2)
+    return occ
+
+
+def _dbl_skip_increment(up_states, dn_states, evals, evecs, beta, emin, pos):
+    # planted mistake (the shape of the F9 race): the running index only
+    # advances on the selected states
+    boltz = np.exp(-beta * (evals - emin))
+    occ, idx = 0.0, 0
+    for up in up_states:
+        for dn in dn_states:
+            if up & dn & (1 << pos):
+                occ += np.sum(boltz * evecs[idx, :] ** 2)
+                idx += 1
+    return occ
+
+
+@pytest.mark.parametrize("bad", [_dbl_running_idx_reset, _dbl_dn_major, _dbl_skip_increment])
+def test_double_occupation_pins_catch_index_mistakes(bad):
+    """Each planted indexing mistake fails the U = 0 Wick pin."""
+    beta, pos = 1.7, 1
+    n_ref = _u0_chain_occ(beta, pos)
+    _, _, d = _thermal(_u0_chain_op, 4, beta, pos, dbl=bad)
+    assert abs(d - n_ref * n_ref) > 1e-4
+
+
+def _siam_u0(eps_imp, mu, eb, v):
+    ns = len(eb) + 1
+    return lambda nu, nd: O.siam_operator(nu, nd, 0.0, eps_imp, eb, v, mu, width=ns)
+
+
+_HYB_EB = np.array([-1.3, 0.4, 1.9])
+_HYB_V = np.array([0.7, -1.1, 0.45])
+
+
+def test_hybridization_func_is_the_u0_impurity_self_energy():
+    """U = 0 impurity model with fermionic signs: the grand-canonical impurity
+    G (lehmann_ij, pinned to the resolvent) equals 1/(z - (eps_imp - mu) - Delta(z))
+    with Delta(z) = sum_k |V_k|^2/(z - eps_k); and Delta equals the bath block
+    of the resolvent computed directly."""
+    z = np.linspace(-4, 4, 33) + 0.15j
+    eps_imp, mu = 0.6, 0.25
+    g = O.lehmann_ij(_siam_u0(eps_imp, mu, _HYB_EB, _HYB_V), 4, z, 2.0, 0, 0)
+    delta = O.hybridization_func(z, _HYB_V, _HYB_EB)
+    assert_allclose(g, 1 / (z - (eps_imp - mu) - delta), atol=1e-11)
+    direct = np.array([_HYB_V @ np.linalg.solve(zz * np.eye(3) - np.diag(_HYB_EB), _HYB_V) for zz in z])
+    assert_allclose(delta, direct, atol=1e-13)
+    # a |V| (not squared) restatement would fail
+    bad = np.sum(np.abs(_HYB_V) / (z[:, None] - _HYB_EB), axis=-1)
+    assert np.max(np.abs(bad - delta)) > 1e-2
+
+
+def test_impurity_gf0_convention():
+    """anderson.py:160-161 writes G0 = 1/(z + mu + eps_imp - Delta).  At
+    eps_imp = 0 that is the U = 0 impurity G of the model's own Hamiltonian
+    (on-site level eps_imp - mu); for eps_imp != 0 it is the U = 0 G of the
+    model with the impurity level reflected, eps_imp -> -eps_imp.  A flipped
+    mu sign or a physical-sign restatement fails one of the two checks."""
+    z = np.linspace(-4, 4, 33) + 0.15j
+    mu = 0.35
+    g_zero = O.lehmann_ij(_siam_u0(0.0, mu, _HYB_EB, _HYB_V), 4, z, 2.0, 0, 0)
+    assert_allclose(O.impurity_gf0(z, 3.0, 0.0, _HYB_V, _HYB_EB, mu), g_zero, atol=1e-11)
+    e = 0.8
+    g_refl = O.lehmann_ij(_siam_u0(-e, mu, _HYB_EB, _HYB_V), 4, z, 2.0, 0, 0)
+    assert_allclose(O.impurity_gf0(z, 3.0, e, _HYB_V, _HYB_EB, mu), g_refl, atol=1e-11)
+    g_phys = O.lehmann_ij(_siam_u0(e, mu, _HYB_EB, _HYB_V), 4, z, 2.0, 0, 0)
+    assert np.max(np.abs(O.impurity_gf0(z, 3.0, e, _HYB_V, _HYB_EB, mu) - g_phys)) > 1e-2
+    assert np.max(np.abs(O.impurity_gf0(z, 3.0, 0.0, _HYB_V, _HYB_EB, -mu) - g_zero)) > 1e-2
+
+
+def test_every_oracle_function_is_in_the_parity_table():
+    import inspect
+
+    named = set()
+    for key in O.PARITY:
+        named.update(k.strip() for k in key.split("/"))
+    public = {n for n, f in vars(O).items()
+              if (inspect.isfunction(f) or inspect.isclass(f)) and f.__module__ == O.__name__}
+    assert public - named == set()
+
+
+@pytest.mark.parametrize("nx,ny", [(4, 4), (4, 2), (3, 3)])
+def test_lattice_bonds_and_free_fermion_energy_closed_form(nx, ny):
+    """Periodic square lattice: every site has four neighbours (fewer where a
+    side has length 2) and the tight-binding levels are 2t(cos kx + cos ky);
+    the free-fermion energy is the sum of the lowest of those levels, and the
+    periodic chain's levels are 2t cos(2 pi k/N)."""
+    t, ns = 0.8, nx * ny
+    bonds = O.square_bonds(nx, ny, True)
+    assert all(i < j for i, j in bonds) and len(set(bonds)) == len(bonds)
+    per_dir = lambda n: 2 if n > 2 else 1
+    assert len(bonds) == ns * (per_dir(nx) + per_dir(ny)) // 2
+    kx, ky = np.meshgrid(2 * np.pi * np.arange(nx) / nx, 2 * np.pi * np.arange(ny) / ny)
+    lv = np.sort((2 * t * (np.cos(kx) * (nx > 2 or 0.5) + np.cos(ky) * (ny > 2 or 0.5))).ravel())
+    n = ns // 2
+    assert abs(O.free_fermion_energy(ns, bonds, t, 0.0, n, n - 1) - (lv[:n].sum() + lv[:n - 1].sum())) < 1e-12
+    k = np.arange(7)
+    lv = np.sort(2 * t * np.cos(2 * np.pi * k / 7))
+    assert abs(O.free_fermion_energy(7, O.chain_bonds(7, True), t, 0.0, 3, 2) - (lv[:3].sum() + lv[:2].sum())) < 1e-12
