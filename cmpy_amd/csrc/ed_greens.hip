@@ -700,6 +700,28 @@ int ed_gemm_nt(const double* a_dev, const double* b_dev, double* c_dev, int64_t 
   return ED_OK;
 }
 
+// Occupation sum over caller-held spin-state lists (device uint32 arrays):
+// partials per chunk and vector, then one serial sum, all stream-ordered.
+static int occupation_sum(const double* vecs_dev, int64_t nvec, const uint32_t* su, int64_t nu,
+                          const uint32_t* sd, int64_t nd, int pos, int kind, const double* w_dev, double* out_dev,
+                          hipStream_t s) {
+  const int64_t D = nu * nd;
+  const int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>((D + 4095) / 4096, 1024));
+  const int64_t chunk = (D + nchunk - 1) / nchunk;
+  double* part = nullptr;
+  if (hipMallocAsync((void**)&part, sizeof(double) * std::max<int64_t>(nchunk * nvec, 1), s) != hipSuccess)
+    return set_error(ED_ERR_NOMEM, "hipMallocAsync failed");
+  int rc = ED_OK;
+  if (nvec > 0 && D > 0)
+    hipLaunchKernelGGL(k_occ, dim3((unsigned)nchunk, (unsigned)nvec), dim3(256), 0, s, vecs_dev, D, nd, su, sd,
+                       1u << pos, kind, w_dev, chunk, part);
+  hipLaunchKernelGGL(k_sum_serial, dim3(1), dim3(64), 0, s, part, (nvec > 0 && D > 0) ? nchunk * nvec : 0,
+                     out_dev);
+  if (hipGetLastError() != hipSuccess) rc = set_error(ED_ERR_HIP, "occupation launch failed");
+  (void)hipFreeAsync(part, s);
+  return rc;
+}
+
 int ed_occupation(const double* vecs_dev, int64_t nvec, int num_sites, int n_up, int n_dn, int pos, int kind,
                   const double* w_dev, double* out_dev, void* stream) {
   clear_error();
@@ -711,30 +733,38 @@ int ed_occupation(const double* vecs_dev, int64_t nvec, int num_sites, int n_up,
   ED_REQUIRE(out_dev && (nvec == 0 || (vecs_dev && w_dev)), "NULL buffer");
   hipStream_t s = as_stream(stream);
   const int64_t nu = (int64_t)binom_u(num_sites, n_up), nd = (int64_t)binom_u(num_sites, n_dn);
-  const int64_t D = nu * nd;
-  const int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>((D + 4095) / 4096, 1024));
-  const int64_t chunk = (D + nchunk - 1) / nchunk;
   uint32_t *su = nullptr, *sd = nullptr;
-  double* part = nullptr;
   int rc = ED_OK;
   // stream-ordered scratch, freed in stream order (no synchronisation)
   if (hipMallocAsync((void**)&su, sizeof(uint32_t) * std::max<int64_t>(nu, 1), s) != hipSuccess ||
-      hipMallocAsync((void**)&sd, sizeof(uint32_t) * std::max<int64_t>(nd, 1), s) != hipSuccess ||
-      hipMallocAsync((void**)&part, sizeof(double) * std::max<int64_t>(nchunk * nvec, 1), s) != hipSuccess) {
+      hipMallocAsync((void**)&sd, sizeof(uint32_t) * std::max<int64_t>(nd, 1), s) != hipSuccess) {
     rc = set_error(ED_ERR_NOMEM, "hipMallocAsync failed");
   } else {
     hipLaunchKernelGGL(k_unrank_states, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, s, su, nu, num_sites, n_up);
     hipLaunchKernelGGL(k_unrank_states, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s, sd, nd, num_sites, n_dn);
-    if (nvec > 0)
-      hipLaunchKernelGGL(k_occ, dim3((unsigned)nchunk, (unsigned)nvec), dim3(256), 0, s, vecs_dev, D, nd, su, sd,
-                         1u << pos, kind, w_dev, chunk, part);
-    hipLaunchKernelGGL(k_sum_serial, dim3(1), dim3(64), 0, s, part, nchunk * nvec, out_dev);
-    if (hipGetLastError() != hipSuccess) rc = set_error(ED_ERR_HIP, "occupation launch failed");
+    if (hipGetLastError() != hipSuccess)
+      rc = set_error(ED_ERR_HIP, "occupation launch failed");
+    else
+      rc = occupation_sum(vecs_dev, nvec, su, nu, sd, nd, pos, kind, w_dev, out_dev, s);
   }
   if (su) (void)hipFreeAsync(su, s);
   if (sd) (void)hipFreeAsync(sd, s);
-  if (part) (void)hipFreeAsync(part, s);
   return rc;
+}
+
+int ed_occupation_states(const double* vecs_dev, int64_t nvec, const uint32_t* up_states_dev, int64_t num_up,
+                         const uint32_t* dn_states_dev, int64_t num_dn, int pos, int kind, const double* w_dev,
+                         double* out_dev, void* stream) {
+  clear_error();
+  ED_REQUIRE(num_up >= 0 && num_dn >= 0, "negative state count");
+  ED_REQUIRE(num_dn == 0 || num_up <= INT64_MAX / num_dn, "num_up * num_dn overflows");
+  ED_REQUIRE(pos >= 0 && pos < 32, "pos=%d outside [0, 32)", pos);
+  ED_REQUIRE(kind >= 0 && kind <= 2, "kind must be 0 (up), 1 (dn) or 2 (double)");
+  ED_REQUIRE(nvec >= 0 && nvec <= 65535, "nvec=%lld outside [0, 65535]", (long long)nvec);
+  const bool any = nvec > 0 && num_up > 0 && num_dn > 0;
+  ED_REQUIRE(out_dev && (!any || (vecs_dev && w_dev && up_states_dev && dn_states_dev)), "NULL buffer");
+  return occupation_sum(vecs_dev, nvec, up_states_dev, num_up, dn_states_dev, num_dn, pos, kind, w_dev, out_dev,
+                        as_stream(stream));
 }
 
 int ed_dense_eigh(double* a_dev, int64_t n, double* w_dev, void* stream) {

@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import ctypes as C
 import logging
+import warnings
 from dataclasses import dataclass
 from typing import NamedTuple, Optional, Tuple
 
@@ -41,6 +42,7 @@ __all__ = [
     "GreensFunctionMeasurement", "gf_lehmann", "greens_lehmann_t0", "greens_cf", "CFResult",
     "iter_lanczos_coeffs", "lanczos_coeffs", "lanczos_matrix", "lanczos_ground_state",
     "gf_greater", "gf_lesser", "gf_tevo", "greens_lehmann", "greens_cf_ij", "LehmannIJ",
+    "occupation_up", "occupation_dn", "occupation", "double_occupation", "accumulate_gf",
 ]
 
 
@@ -223,6 +225,127 @@ def solve_sector(model, sector: Sector, cache: dict = None):
         if cache is not None:
             cache[key] = (w, v)
     return w.cpu().numpy(), v.cpu().numpy().T
+
+
+# =============================================================================
+# Observables and the Lehmann accumulation (cmpy/exactdiag.py:60-134)
+# =============================================================================
+
+
+def _device_states(states) -> Tuple[torch.Tensor, int]:
+    a = np.asarray(states.cpu() if isinstance(states, torch.Tensor) else states).astype(np.int64).ravel()
+    if a.size and (a.min() < 0 or a.max() >= 1 << 32):
+        raise ValueError("spin states must lie in [0, 2**32)")
+    return torch.from_numpy(a.astype(np.uint32).view(np.int32)).cuda(), int(a.size)
+
+
+def _rows_device(evecs, dim: int) -> torch.Tensor:
+    """Eigenvectors given as columns (dim x N, numpy or torch) -> device rows [N][dim]."""
+    if isinstance(evecs, torch.Tensor):
+        v = evecs.to("cuda", torch.float64)
+    else:
+        v = torch.from_numpy(np.ascontiguousarray(np.asarray(evecs, dtype=np.float64))).cuda()
+    if v.dim() == 1:
+        v = v[:, None]
+    if v.dim() != 2 or v.shape[0] != dim:
+        raise ValueError(f"evecs must have {dim} rows (one per sector state), got shape {tuple(v.shape)}")
+    return v.t().contiguous()
+
+
+def _occupation_call(up_states, dn_states, evals, evecs, beta, emin, pos, kind) -> float:
+    _lib.require_gpu()
+    su, nu = _device_states(up_states)
+    sd, nd = _device_states(dn_states)
+    pos = int(pos)
+    if not 0 <= pos < 32:
+        raise ValueError(f"pos={pos} outside [0, 32)")
+    V = _rows_device(evecs, nu * nd)
+    ev = np.atleast_1d(np.asarray(evals.cpu() if isinstance(evals, torch.Tensor) else evals, np.float64))
+    if ev.size != V.shape[0]:
+        raise ValueError(f"{ev.size} eigenvalues for {V.shape[0]} eigenvectors")
+    with np.errstate(invalid="ignore", over="ignore"):
+        boltz = np.exp(-beta * (ev - emin))  # the reference's weights, NaN/inf included
+    w = torch.from_numpy(np.ascontiguousarray(boltz)).cuda()
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+    _lib.call("ed_occupation_states", _lib.dptr(V), V.shape[0], _lib.dptr(su), nu, _lib.dptr(sd), nd, pos, kind,
+              _lib.dptr(w), _lib.dptr(out), _lib.stream_ptr())
+    return float(out.item())
+
+
+def occupation_up(up_states, dn_states, evals, evecs, beta, emin, pos) -> float:
+    """sum_k e^{-beta(E_k - emin)} sum_{up state has pos} |evecs[i, k]|^2
+    (cmpy/exactdiag.py:60-72); evecs as columns, flat index up_idx*num_dn + dn_idx."""
+    return _occupation_call(up_states, dn_states, evals, evecs, beta, emin, pos, 0)
+
+
+def occupation_dn(up_states, dn_states, evals, evecs, beta, emin, pos) -> float:
+    """As :func:`occupation_up` for the dn spin (cmpy/exactdiag.py:75-85)."""
+    return _occupation_call(up_states, dn_states, evals, evecs, beta, emin, pos, 1)
+
+
+def occupation(up_states, dn_states, evals, evecs, beta, emin=0.0, pos=0, sigma=UP) -> float:
+    """cmpy/exactdiag.py:88-92."""
+    if sigma == UP:
+        return occupation_up(up_states, dn_states, evals, evecs, beta, emin, pos)
+    return occupation_dn(up_states, dn_states, evals, evecs, beta, emin, pos)
+
+
+def double_occupation(up_states, dn_states, evals, evecs, beta, emin, pos) -> float:
+    """sum_k e^{-beta(E_k - emin)} sum_{both spins have pos} |evecs[i, k]|^2
+    (cmpy/exactdiag.py:95-107), with the flat index up_idx*num_dn + dn_idx that
+    the reference's running ``idx`` stands for (it is only right serially)."""
+    return _occupation_call(up_states, dn_states, evals, evecs, beta, emin, pos, 2)
+
+
+def accumulate_gf(gf, z, cdag, evals, evecs, evals_p1, evecs_p1, beta, emin=0.0) -> None:
+    """Add the Lehmann pair sum of one (sector, upper sector) pair into ``gf`` in
+    place (cmpy/exactdiag.py:110-134):
+
+        gf += sum_{m,n} |<m|c+|n>|^2 (e^{-beta(E_n-emin)} + e^{-beta(E_m-emin)}) / (z - E_m + E_n)
+
+    ``cdag`` is a :class:`CreationOperator` (applied on the device) or any
+    operator with ``matmat``; eigenvectors are columns.  ``beta = inf`` gives
+    unit weights, as in the reference.  ``gf`` is a complex numpy array (or a
+    complex torch tensor) of the shape of ``z``.
+    """
+    _lib.require_gpu()
+    z = np.asarray(z, dtype=np.complex128)
+    ev = np.ascontiguousarray(np.asarray(evals.cpu() if isinstance(evals, torch.Tensor) else evals, np.float64))
+    ev1 = np.ascontiguousarray(np.asarray(evals_p1.cpu() if isinstance(evals_p1, torch.Tensor) else evals_p1,
+                                          np.float64))
+    n, m = ev.size, ev1.size
+    d_p1 = evecs_p1.shape[0]
+    V1 = _rows_device(evecs_p1, d_p1)
+    if V1.shape[0] != m:
+        raise ValueError(f"{m} upper-sector eigenvalues for {V1.shape[0]} eigenvectors")
+    if isinstance(cdag, CreationOperator):
+        V = _rows_device(evecs, cdag.shape[1])
+        if V.shape[0] != n:
+            raise ValueError(f"{n} eigenvalues for {V.shape[0]} eigenvectors")
+        Cm = cdag.apply_device(V)
+    else:
+        ce = np.asarray(cdag.matmat(evecs.cpu().numpy() if isinstance(evecs, torch.Tensor) else evecs))
+        if np.iscomplexobj(ce):
+            if np.any(ce.imag):
+                raise ValueError("accumulate_gf takes real eigenvectors")
+            ce = ce.real
+        Cm = _rows_device(ce, d_p1)
+        if Cm.shape[0] != n:
+            raise ValueError(f"{n} eigenvalues for {Cm.shape[0]} vectors c+ evecs")
+    ov = torch.empty((m, n), dtype=torch.float64, device="cuda")
+    _lib.call("ed_gemm_nt", _lib.dptr(V1), _lib.dptr(Cm), _lib.dptr(ov), m, n, d_p1, _lib.stream_ptr())
+    zd = torch.from_numpy(np.ascontiguousarray(np.stack([z.real.ravel(), z.imag.ravel()], axis=1).ravel())).cuda()
+    g = torch.zeros(2 * z.size, dtype=torch.float64, device="cuda")
+    b = float(beta) if np.isfinite(beta) else float("inf")
+    evd, ev1d = torch.from_numpy(ev).cuda(), torch.from_numpy(ev1).cuda()
+    _lib.call("ed_lehmann_pairs", _lib.dptr(ov), _lib.dptr(evd), n, _lib.dptr(ev1d), m, b, float(emin),
+              _lib.dptr(zd), z.size, _lib.dptr(g), _lib.stream_ptr())
+    gh = g.view(-1, 2).cpu().numpy()
+    add = (gh[:, 0] + 1j * gh[:, 1]).reshape(z.shape)
+    if isinstance(gf, torch.Tensor):
+        gf += torch.from_numpy(add).to(gf.device, gf.dtype)
+    else:
+        gf += add
 
 
 # =============================================================================
@@ -526,25 +649,46 @@ class CFResult:
     alpha_h: np.ndarray
     beta_h: np.ndarray
     norm2_h: float
+    reorth_p: str = "none"  # "none", "full" or "partial": how each fraction's basis was kept orthogonal
+    reorth_h: str = "none"
 
 
-REORTH_MAX_BYTES = 16 << 30  # Krylov basis kept for full reorthogonalisation
+REORTH_MAX_BYTES = 16 << 30  # Krylov basis always allowed for reorthogonalisation
+REORTH_FREE_FRACTION = 0.4  # ... or up to this share of the free device memory, if larger
+_MODE_NAMES = {0: "none", 1: "full", 2: "partial"}
 
 
 PARTIAL_REORTH_MIN_SIZE = 50_000  # below this, full reorthogonalisation is as cheap
 
 
+def _reorth_budget() -> int:
+    budget = REORTH_MAX_BYTES
+    if torch.cuda.is_available():
+        free, _ = torch.cuda.mem_get_info()
+        budget = max(budget, int(REORTH_FREE_FRACTION * free))
+    return budget
+
+
 def _default_reorth(size: int, nsteps: int) -> bool:
-    return nsteps <= 4096 and 8 * size * (nsteps + 2) <= REORTH_MAX_BYTES
+    need = 8 * size * (nsteps + 2)
+    return nsteps <= 4096 and (need <= REORTH_MAX_BYTES or need <= _reorth_budget())
 
 
-def _reorth_mode(reorth, size: int, nsteps: int) -> int:
+def _reorth_mode(reorth, size: int, nsteps: int, warn: bool = False) -> int:
     """0 none, 1 full (two Gram-Schmidt passes every step), 2 partial (only
     where Simon's orthogonality estimate passes sqrt(eps)).  ``None``: partial
     from PARTIAL_REORTH_MIN_SIZE states, full below, none when the Krylov basis
-    would exceed REORTH_MAX_BYTES."""
+    would exceed both REORTH_MAX_BYTES and REORTH_FREE_FRACTION of the free
+    device memory; with ``warn`` that fallback raises a RuntimeWarning, since
+    an unreorthogonalised fraction loses accuracy at small broadening."""
     if reorth is None:
         if not _default_reorth(size, nsteps):
+            if warn:
+                warnings.warn(
+                    f"continued fraction of {nsteps} steps on {size} states: the Krylov basis "
+                    f"({8 * size * (nsteps + 2) / 2**30:.1f} GiB) does not fit, so it runs without "
+                    "reorthogonalisation (expect ~1e-3 errors in G at Im z ~ 0.05; pass reorth= to override)",
+                    RuntimeWarning, stacklevel=4)
             return 0
         return 2 if size >= PARTIAL_REORTH_MIN_SIZE else 1
     if isinstance(reorth, str):
@@ -555,9 +699,11 @@ def _reorth_mode(reorth, size: int, nsteps: int) -> int:
     return 1 if reorth else 0
 
 
-def _cf_coeffs(op: SectorHamiltonOperator, phi: torch.Tensor, nsteps: int, reorth=None):
+def _cf_coeffs(op: SectorHamiltonOperator, phi: torch.Tensor, nsteps: int, reorth=None, modes: list = None):
     nsteps = int(max(1, min(nsteps, op.size)))
-    mode = _reorth_mode(reorth, op.size, nsteps)
+    mode = _reorth_mode(reorth, op.size, nsteps, warn=True)
+    if modes is not None:
+        modes.append(_MODE_NAMES[mode])
     nbytes = C.c_int64()
     _lib.call("ed_lanczos_coeffs_workspace", op.handle, nsteps, mode, C.byref(nbytes))
     work = torch.empty(nbytes.value, dtype=torch.uint8, device="cuda")
@@ -577,21 +723,25 @@ def greens_cf(model, z, n_up, n_dn, pos=0, sigma=UP, nsteps=200, gs: EigenState 
     (particle, sector N+1) and c psi0 (hole, sector N-1).  ``reorth`` (None,
     bool, 'none', 'full' or 'partial') defaults to partial reorthogonalisation
     from 50,000 states and full below, whenever the Krylov basis fits in
-    16 GiB (see ``_reorth_mode``)."""
+    16 GiB or in 40% of the free device memory; beyond that the fraction runs
+    without reorthogonalisation and a RuntimeWarning says so.  The mode used
+    is recorded in ``CFResult.reorth_p`` / ``reorth_h``."""
     ns = model.num_sites
     sign_mode = _sign_mode(model, sign_mode)
     e0, psi = _gs(model, n_up, n_dn, gs, tol)
     zt = _z_device(z)
     g = torch.zeros(zt.numel(), dtype=torch.float64, device="cuda")
-    coeffs = {}
+    coeffs, modes = {}, {}
     for create in (True, False):
         phi, sec = _ladder_device(psi, ns, n_up, n_dn, pos, sigma, create, sign_mode)
         if phi is None:
             coeffs[create] = (np.zeros(0), np.zeros(0), 0.0)
             continue
         op = model.hamilton_operator(n_up=sec[0], n_dn=sec[1])
-        a, b, n2 = _cf_coeffs(op, phi, nsteps, reorth)
+        used = []
+        a, b, n2 = _cf_coeffs(op, phi, nsteps, reorth, used)
         coeffs[create] = (a, b, n2)
+        modes[create] = used[0]
         if len(a):
             _lib.call("ed_cf_eval", a.ctypes.data_as(C.POINTER(C.c_double)),
                       (b if len(b) else np.zeros(1)).ctypes.data_as(C.POINTER(C.c_double)), len(a), n2, e0,
@@ -600,7 +750,7 @@ def greens_cf(model, z, n_up, n_dn, pos=0, sigma=UP, nsteps=200, gs: EigenState 
     gf = (gh[:, 0] + 1j * gh[:, 1]).reshape(np.shape(z))
     ap, bp, np_ = coeffs[True]
     ah, bh, nh = coeffs[False]
-    return CFResult(gf, e0, ap, bp, np_, ah, bh, nh)
+    return CFResult(gf, e0, ap, bp, np_, ah, bh, nh, modes.get(True, "none"), modes.get(False, "none"))
 
 
 def greens_cf_ij(model, z, n_up, n_dn, i, j, sigma=UP, nsteps=200, gs: EigenState = None, tol=1e-10,

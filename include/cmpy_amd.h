@@ -279,6 +279,16 @@ int ed_gemm_nt(const double* a_dev, const double* b_dev, double* c_dev, int64_t 
 int ed_occupation(const double* vecs_dev, int64_t nvec, int num_sites, int n_up, int n_dn, int pos,
                   int kind, const double* w_dev, double* out_dev, void* stream);
 
+/* The same sum over caller-given spin-state lists, the signature of
+ * occupation_up / occupation_dn / double_occupation (cmpy/exactdiag.py:60-107,
+ * which take up_states and dn_states arrays): up_states_dev [num_up] and
+ * dn_states_dev [num_dn] (device, uint32, any order), V: [nvec][num_up*num_dn]
+ * with the flat index up_idx*num_dn + dn_idx (the index the reference's
+ * running `idx` stands for, exactdiag.py:95-107).  pos < 32.                 */
+int ed_occupation_states(const double* vecs_dev, int64_t nvec, const uint32_t* up_states_dev, int64_t num_up,
+                         const uint32_t* dn_states_dev, int64_t num_dn, int pos, int kind,
+                         const double* w_dev, double* out_dev, void* stream);
+
 /* Dense symmetric eigensolver (LAPACK dsyevd semantics, ascending values):
  * replaces np.linalg.eigh of solve_sector (cmpy/exactdiag.py:46-57).
  * a_dev (n x n symmetric) is overwritten by the eigenvectors stored vector

@@ -431,3 +431,26 @@ def test_krylov_basis_orthogonality_by_mode(gpu, mode, bound):
     kept, plain = basis_overlap(mode), basis_overlap(0)
     assert kept <= bound, kept
     assert plain > 1e-2, plain
+
+
+def test_greens_cf_records_and_warns_reorth_fallback(gpu, monkeypatch):
+    """The default reorthogonalisation mode is recorded in CFResult; when the
+    Krylov basis would not fit, the fraction falls back to the plain
+    recurrence with a RuntimeWarning instead of silently."""
+    import warnings
+
+    from cmpy_amd import exactdiag as ed
+
+    model, _ = _siam(0)
+    z = np.linspace(-8, 8, 101) + 0.05j
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        r = ed.greens_cf(model, z, 3, 3, nsteps=50)
+    assert (r.reorth_p, r.reorth_h) == ("full", "full")
+    monkeypatch.setattr(ed, "REORTH_MAX_BYTES", 1024)
+    monkeypatch.setattr(ed, "REORTH_FREE_FRACTION", 0.0)
+    with pytest.warns(RuntimeWarning, match="without reorthogonalisation"):
+        r2 = ed.greens_cf(model, z, 3, 3, nsteps=50)
+    assert (r2.reorth_p, r2.reorth_h) == ("none", "none")
+    r3 = ed.greens_cf(model, z, 3, 3, nsteps=50, reorth="partial")
+    assert (r3.reorth_p, r3.reorth_h) == ("partial", "partial")
