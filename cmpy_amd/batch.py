@@ -20,6 +20,7 @@ cmpy/exactdiag.py:110-134, the same quantity as
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 from dataclasses import dataclass
 from typing import Optional, Sequence
 
@@ -42,6 +43,7 @@ class BatchResult:
     residual: np.ndarray      # [M] explicit |H psi - E0 psi|
     elapsed_ms: float         # device time of the launch
     cf_steps: Optional[np.ndarray] = None  # [M, 2] particle / hole continued-fraction steps
+    cf_capped: Optional[np.ndarray] = None  # [M, 2] True where a fraction stopped at the step cap unconverged
 
 
 def _ptr(a, t):
@@ -59,7 +61,7 @@ class AndersonBatch:
     """
 
     def __init__(self, u, eps_imp, eps_bath: Sequence, v: Sequence, mu=None, n_up=None, n_dn=None,
-                 sign_mode: str = "reference", max_iter: int = 300, cf_steps: int = 512, tol: float = 1e-10,
+                 sign_mode: str = "reference", max_iter: int = 300, cf_steps: int = 1024, tol: float = 1e-10,
                  check_every: int = 8, seed: int = 0x5EED, cf_tol: float = 1e-13, keep_basis: bool = False):
         _lib.require_gpu()
         if sign_mode not in ("reference", "fermion"):
@@ -104,6 +106,7 @@ class AndersonBatch:
                            int(check_every), int(bool(keep_basis)), float(tol), int(seed) & 0xFFFFFFFFFFFFFFFF,
                            float(cf_tol))
         self.keep_basis = bool(keep_basis)
+        self.cf_steps_cap = int(cf_steps)
         h = C.c_void_p()
         _lib.call("ed_batch_create", C.byref(h), C.byref(d), _lib.stream_ptr())
         self._h = h
@@ -148,8 +151,16 @@ class AndersonBatch:
         _lib.call("ed_batch_solve", self._h, _ptr(wn, C.c_double), nw, _ptr(e0, C.c_double),
                   _ptr(g, C.c_double), _ptr(it, C.c_int32), _ptr(res, C.c_double), C.byref(ms),
                   _lib.stream_ptr())
+        steps = it[:, 1:].copy()
+        capped = steps >= self.cf_steps_cap
+        if capped.any():
+            warnings.warn(
+                f"{int(capped.any(axis=1).sum())} of {M} models have a continued fraction stopped at the "
+                f"{self.cf_steps_cap}-step cap before G(i w_0) converged to cf_tol; their G and gradients "
+                "carry the truncation (BatchResult.cf_capped marks them; raise cf_steps)",
+                RuntimeWarning, stacklevel=2)
         return BatchResult(e0, g[..., 0] + 1j * g[..., 1], it[:, 0].copy(), res[:, 0], res[:, 1], ms.value,
-                           it[:, 1:].copy())
+                           steps, capped)
 
     def set_bath(self, eps_bath, v) -> None:
         """Replace the bath parameters ([M, num_bath] arrays or per-model lists)."""

@@ -991,8 +991,9 @@ __device__ __forceinline__ double bond_hops(const SpinList& U, const SpinList& D
   return s;
 }
 
-constexpr int kPc = 16;          // W columns per LDS chunk in the backward
-constexpr int kBwdMaxSteps = 512; // LDS chunk rows: continued-fraction steps kept
+constexpr int kPc = 16;             // W columns per LDS chunk in the backward
+constexpr int kBwdLds = 512 * kPc;  // doubles of the backward's LDS staging buffer (64 KiB)
+constexpr int kBwdMaxSteps = 1024;  // continued-fraction steps a kept basis may hold
 
 __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi, const BwdSlot& bs, double* wl) {
   const ModelParams& p = bc.params[mi];
@@ -1078,7 +1079,7 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
     {
       const int nbk = (m + 3) / 4;
       const int ntile = nbk * (nbk + 1) / 2;
-      const int ncap = (kBwdMaxSteps * kPc) / (4 * m);
+      const int ncap = kBwdLds / (4 * m);
       const int nc = ncap < nw ? (ncap > 0 ? ncap : 1) : nw;
       double* vs = wl;              // [nc][m] complex
       double* us = wl + 2 * nc * m; // [nc][m] complex
@@ -1171,22 +1172,27 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
       Xr[r] = xr;
       Xi[r] = xi;
     }
-    // P = Qt Re W with FP64 matrix cores (16 x 16 x 4): wave (wr, wc) of the
-    // workgroup accumulates the 32 x 32 block of P at rows rb = R0 + 32 wr,
-    // columns 32 wc, as 2 x 2 tiles; Re W is staged through LDS kc rows at a
-    // time.  Operand layout of the instruction: lane l holds A[l % 16][l / 16]
-    // and B[l / 16][l % 16], and its four results are D[l / 16 + 4 r][l % 16].
+    // P = Qt Re W with FP64 matrix cores (16 x 16 x 4), in groups of at most
+    // one 32-column block per wave: within a group starting at column c0,
+    // wave (wr, wc) accumulates the 32 x 32 block of P at rows
+    // rb = R0 + 32 wr, columns c0 + 32 wc, as 2 x 2 tiles; the group's
+    // columns of Re W are staged through LDS kc rows at a time.  Operand
+    // layout of the instruction: lane l holds A[l % 16][l / 16] and
+    // B[l / 16][l % 16], and its four results are D[l / 16 + 4 r][l % 16].
     // Krylov indices at or past m contribute zeros (Qt holds no data there).
-    {
-      typedef double f64x4 __attribute__((ext_vector_type(4)));
-      const int ncb = (m + 31) / 32;
-      const int wpr = (int)(g.size >> 6) / ncb;
+    typedef double f64x4 __attribute__((ext_vector_type(4)));
+    const int nwv = (int)(g.size >> 6);
+    const int ncb_all = (m + 31) / 32;
+    for (int cb0 = 0; cb0 < ncb_all; cb0 += nwv) {
+      const int ncb = ncb_all - cb0 < nwv ? ncb_all - cb0 : nwv;
+      const int c0 = 32 * cb0;
+      const int wpr = nwv / ncb;
       const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
       const int lane = threadIdx.x & 63;
       const int wr = wave / ncb, wc = wave - wr * ncb;
       const bool won = wr < wpr;
       const int ldw = 32 * ncb;
-      const int kc = ((kBwdMaxSteps * kPc) / ldw) / 4 * 4;
+      const int kc = (kBwdLds / ldw) / 4 * 4;
       for (int64_t R0 = 0; R0 < Dt; R0 += 32 * (int64_t)wpr) {
         const int64_t rb = R0 + 32 * (int64_t)wr;
         f64x4 acc[2][2];
@@ -1199,7 +1205,7 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
           __syncthreads();
           for (int t = g.rank; t < kc * ldw; t += g.size) {
             const int kk = t / ldw, c = t - kk * ldw;
-            wl[t] = (kk < kn && c < m) ? W[(int64_t)(k0 + kk) * kcap + c] : 0.0;
+            wl[t] = (kk < kn && c0 + c < m) ? W[(int64_t)(k0 + kk) * kcap + c0 + c] : 0.0;
           }
           __syncthreads();
           if (won && rb < Dt) {
@@ -1231,7 +1237,7 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
 #pragma unroll
               for (int r = 0; r < 4; ++r) {
                 const int64_t row = rb + 16 * ti + lane / 16 + 4 * r;
-                const int col = 32 * wc + 16 * tj + (lane % 16);
+                const int col = c0 + 32 * wc + 16 * tj + (lane % 16);
                 if (row < Dt && col < m) P[row * kcap + col] = acc[ti][tj][r];
               }
         }
@@ -1361,7 +1367,7 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
 __global__ __launch_bounds__(kBlock) void k_batch_backward(BwdConfig bc) {
   __shared__ double red[32];
   __shared__ int popped;
-  __shared__ double wl[kBwdMaxSteps * kPc];  // 64 KiB: a column chunk of Re W
+  __shared__ double wl[kBwdLds];  // 64 KiB: a chunk of Re W
   __shared__ double s_rows[kLdsRows];
   __shared__ uint32_t s_tgt[kLdsDnBwd];
   __shared__ double s_val[kLdsDnBwd];

@@ -329,9 +329,13 @@ typedef struct ed_batch_desc {
   int32_t max_bath;         /* row stride of eps_bath and v                     */
   int32_t sign_width;       /* 0: reference signs; >= num_sites: fermionic      */
   int32_t max_iter;         /* ground-state Lanczos steps cap (1..2000)         */
-  int32_t cf_steps;         /* continued-fraction depth cap (1..2000)           */
+  int32_t cf_steps;         /* continued-fraction depth cap (1..2000; a fraction
+                               that reaches it has not converged, and
+                               iters[3 m + 1..2] then equals cf_steps)          */
   int32_t check_every;      /* steps between ground-state convergence tests     */
-  int32_t keep_basis;       /* 1: keep psi and the Krylov bases for ed_batch_backward */
+  int32_t keep_basis;       /* 1: keep psi and the Krylov bases for ed_batch_backward
+                               (needs cf_steps <= 1024; kept bases take
+                               8 cf_steps D(N+-1) bytes per model)              */
   double tol;               /* ground-state Ritz residual tolerance              */
   uint64_t seed;            /* Lanczos start-vector seed                        */
   double cf_tol;            /* continued-fraction stop: relative change of G at
