@@ -540,7 +540,8 @@ def lanczos_normalized(matvec, v0, nsteps, reorth=False):
     v0 = np.asarray(v0, dtype=np.float64)
     norm2 = float(np.dot(v0, v0))
     v = v0 / np.sqrt(norm2)
-    vs = [v]
+    basis = np.empty((max(int(nsteps), 1), v.size))  # row j: basis vector v_j
+    basis[0] = v
     alpha, beta = [], []
     v_prev = np.zeros_like(v)
     b = 0.0
@@ -549,7 +550,7 @@ def lanczos_normalized(matvec, v0, nsteps, reorth=False):
         a = float(np.dot(w, v))
         w = w - a * v
         if reorth:
-            V = np.array(vs)
+            V = basis[: j + 1]
             w = w - V.T @ (V @ w)
             w = w - V.T @ (V @ w)
         alpha.append(a)
@@ -558,8 +559,8 @@ def lanczos_normalized(matvec, v0, nsteps, reorth=False):
             break
         beta.append(b)
         v_prev, v = v, w / b
-        vs.append(v)
-    return np.array(alpha), np.array(beta), norm2, np.array(vs).T
+        basis[j + 1] = v
+    return np.array(alpha), np.array(beta), norm2, basis[: len(alpha)].T
 
 
 def cf_greens(alpha, beta, norm2, z, shift, sign=+1):
