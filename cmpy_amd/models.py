@@ -115,9 +115,12 @@ class AbstractManyBodyModel(AbstractModel):
     device operator.
     """
 
+    OPERATOR_CACHE_SIZE = 8  # sector operators kept per model (device tables of a few sectors)
+
     def __init__(self, num_sites: Optional[int] = 0, **params):
         super().__init__(**params)
         object.__setattr__(self, "basis", Basis())
+        object.__setattr__(self, "_op_cache", OrderedDict())
         self.init_basis(num_sites)
 
     @property
@@ -158,14 +161,35 @@ class AbstractManyBodyModel(AbstractModel):
         return data, np.array([rows, cols], dtype=np.int64)
 
     def hamilton_operator(self, n_up=None, n_dn=None, sector=None, dtype=None):
-        """Matrix-free device operator of one sector (cmpy/models/abc.py:250-256)."""
+        """Matrix-free device operator of one sector (cmpy/models/abc.py:250-256).
+
+        The operator (its device tables) is cached per sector and reused while
+        the model's Hamiltonian parameters are unchanged: the key holds their
+        values, so a parameter update (or an in-place edit of a parameter
+        array) builds a new one.
+        """
         if sector is None:
             if n_up is None or n_dn is None:
                 raise ValueError("hamilton_operator needs a fixed sector (n_up, n_dn)")
             sector = self.basis.get_sector(n_up, n_dn)
         bonds, hops, eps, u, width = self.operator_spec()
-        return SectorHamiltonOperator(self.num_sites, sector.n_up, sector.n_dn, bonds, hops, eps, u,
-                                      width)
+        key = (self.num_sites, sector.n_up, sector.n_dn, int(width),
+               np.asarray(bonds, dtype=np.int64).tobytes(), np.asarray(hops, dtype=np.float64).tobytes(),
+               np.asarray(eps, dtype=np.float64).tobytes(), np.asarray(u, dtype=np.float64).tobytes())
+        cache = self._op_cache
+        op = cache.get(key)
+        if op is None:
+            op = SectorHamiltonOperator(self.num_sites, sector.n_up, sector.n_dn, bonds, hops, eps, u, width)
+            cache[key] = op
+            while len(cache) > self.OPERATOR_CACHE_SIZE:
+                cache.popitem(last=False)
+        else:
+            cache.move_to_end(key)
+        return op
+
+    def clear_operator_cache(self) -> None:
+        """Drop the cached sector operators (and their device tables)."""
+        self._op_cache.clear()
 
     def coo_operator(self, n_up=None, n_dn=None, sector=None, dtype=None):
         """cmpy's COO ``HamiltonOperator`` built from ``hamiltonian_data``."""

@@ -427,26 +427,28 @@ def apply_cdag(x, num_sites, n_up, n_dn, pos, sigma, sign_mode="reference"):
     is applied (not part of the reference; identical at pos = 0).
     """
     x = np.asarray(x)
+    tail = x.shape[1:]  # a block of vectors (columns) is mapped column by column
     up = sector_states(num_sites, n_up)
     dn = sector_states(num_sites, n_dn)
     num_up, num_dn = len(up), len(dn)
-    X = x.reshape(num_up, num_dn)
+    X = x.reshape((num_up, num_dn) + tail)
     op = 1 << pos
+    ones = (1,) * len(tail)
     if sigma == UP:
         up1 = sector_states(num_sites, n_up + 1)
-        Y = np.zeros((len(up1), num_dn), dtype=x.dtype)
+        Y = np.zeros((len(up1), num_dn) + tail, dtype=x.dtype)
         src = np.nonzero((up & op) == 0)[0]
         tgt = np.searchsorted(up1, up[src] ^ op)
         s = _jw_sign(up[src], pos) if sign_mode == "jordan_wigner" else np.ones(len(src))
-        Y[tgt, :] = s[:, None] * X[src, :]
+        Y[tgt] = s.reshape((-1, 1) + ones) * X[src]
     else:
         dn1 = sector_states(num_sites, n_dn + 1)
-        Y = np.zeros((num_up, len(dn1)), dtype=x.dtype)
+        Y = np.zeros((num_up, len(dn1)) + tail, dtype=x.dtype)
         src = np.nonzero((dn & op) == 0)[0]
         tgt = np.searchsorted(dn1, dn[src] ^ op)
         s = _jw_sign(dn[src], pos) if sign_mode == "jordan_wigner" else np.ones(len(src))
-        Y[:, tgt] = X[:, src] * s[None, :]
-    return Y.ravel()
+        Y[:, tgt] = X[:, src] * s.reshape((1, -1) + ones)
+    return Y.reshape((-1,) + tail)
 
 
 def apply_c(x, num_sites, n_up, n_dn, pos, sigma, sign_mode="reference"):
@@ -455,26 +457,28 @@ def apply_c(x, num_sites, n_up, n_dn, pos, sigma, sign_mode="reference"):
     cmpy/operators.py:679-703 and 750-791.
     """
     x = np.asarray(x)
+    tail = x.shape[1:]  # a block of vectors (columns) is mapped column by column
     up = sector_states(num_sites, n_up)
     dn = sector_states(num_sites, n_dn)
     num_up, num_dn = len(up), len(dn)
-    X = x.reshape(num_up, num_dn)
+    X = x.reshape((num_up, num_dn) + tail)
     op = 1 << pos
+    ones = (1,) * len(tail)
     if sigma == UP:
         up1 = sector_states(num_sites, n_up - 1)
-        Y = np.zeros((len(up1), num_dn), dtype=x.dtype)
+        Y = np.zeros((len(up1), num_dn) + tail, dtype=x.dtype)
         src = np.nonzero((up & op) != 0)[0]
         tgt = np.searchsorted(up1, up[src] ^ op)
         s = _jw_sign(up[src], pos) if sign_mode == "jordan_wigner" else np.ones(len(src))
-        Y[tgt, :] = s[:, None] * X[src, :]
+        Y[tgt] = s.reshape((-1, 1) + ones) * X[src]
     else:
         dn1 = sector_states(num_sites, n_dn - 1)
-        Y = np.zeros((num_up, len(dn1)), dtype=x.dtype)
+        Y = np.zeros((num_up, len(dn1)) + tail, dtype=x.dtype)
         src = np.nonzero((dn & op) != 0)[0]
         tgt = np.searchsorted(dn1, dn[src] ^ op)
         s = _jw_sign(dn[src], pos) if sign_mode == "jordan_wigner" else np.ones(len(src))
-        Y[:, tgt] = X[:, src] * s[None, :]
-    return Y.ravel()
+        Y[:, tgt] = X[:, src] * s.reshape((1, -1) + ones)
+    return Y.reshape((-1,) + tail)
 
 
 # =============================================================================
@@ -729,10 +733,7 @@ class GreensFunctionMeasurement:
         # _acc_gf (175-183)
         if factor != 1.0:
             self._gf *= factor
-        cdag_evec = np.column_stack([
-            apply_cdag(evecs[:, k], num_sites, n_up, n_dn, self.pos, self.sigma)
-            for k in range(evecs.shape[1])
-        ])
+        cdag_evec = apply_cdag(evecs, num_sites, n_up, n_dn, self.pos, self.sigma)
         _accumulate_sum(self._gf, self.z, evals, evals_p1, evecs_p1, cdag_evec,
                         self.beta, self._gs_energy)
         if self._measure_occ:
@@ -784,6 +785,12 @@ def lehmann_ij(make_op, num_sites, z, beta, i, j, sigma=UP, sign_mode="jordan_wi
     Energies are measured from the lowest eigenvalue of all sectors at once;
     ``sign_mode="jordan_wigner"`` gives the physical (anticommuting) c, c+.
     """
+    return lehmann_ij_many(make_op, num_sites, z, beta, [(i, j)], sigma, sign_mode)[0]
+
+
+def lehmann_ij_many(make_op, num_sites, z, beta, pairs, sigma=UP, sign_mode="jordan_wigner"):
+    """:func:`lehmann_ij` for several (i, j) at once, every sector diagonalised
+    once; returns one G per pair."""
     z = np.asarray(z, dtype=np.complex128)
     fillings = list(itertools.product(range(num_sites + 1), repeat=2))
     eig = {}
@@ -792,26 +799,25 @@ def lehmann_ij(make_op, num_sites, z, beta, i, j, sigma=UP, sign_mode="jordan_wi
         eig[(nu, nd)] = (w, v)
     emin = min(float(w[0]) for w, _ in eig.values())
     part = sum(float(np.sum(np.exp(-beta * (w - emin)))) for w, _ in eig.values()) if np.isfinite(beta) else None
-    g = np.zeros_like(z)
+    gs = [np.zeros_like(z) for _ in pairs]
+    sites = sorted({s for p in pairs for s in p})
     for nu, nd in fillings:
         p1 = upper_sector(nu, nd, sigma, num_sites)
         if p1 is None:
             continue
         w, v = eig[(nu, nd)]
         w1, v1 = eig[p1]
-        ai = v1.T @ np.column_stack([apply_cdag(v[:, k], num_sites, nu, nd, i, sigma, sign_mode)
-                                     for k in range(v.shape[1])])
-        aj = v1.T @ np.column_stack([apply_cdag(v[:, k], num_sites, nu, nd, j, sigma, sign_mode)
-                                     for k in range(v.shape[1])])
+        ov = {s: v1.T @ apply_cdag(v, num_sites, nu, nd, s, sigma, sign_mode) for s in sites}
         if np.isfinite(beta):
             wt = np.exp(-beta * (w - emin))[None, :] + np.exp(-beta * (w1 - emin))[:, None]
         else:
             wt = 2.0 * np.ones((len(w1), len(w)))
-        weights = ai * aj * wt
         poles = w1[:, None] - w[None, :]
-        for k in range(len(z)):
-            g[k] += np.sum(weights / (z[k] - poles))
-    return g / part if part is not None else g
+        for g, (i, j) in zip(gs, pairs):
+            weights = ov[i] * ov[j] * wt
+            for k in range(len(z)):
+                g[k] += np.sum(weights / (z[k] - poles))
+    return [g / part if part is not None else g for g in gs]
 
 
 def resolvent_ij(h, z):
@@ -890,14 +896,31 @@ def tevo_part(op, phi, e0, times, sign=+1):
     return -1j * sign * (ph @ ov)
 
 
-def tevo_expm_multiply(op, phi, e0, start, stop, num, sign=+1):
+def tevo_expm_multiply(op, phi, e0, start, stop, num, sign=+1, matvec=None):
     """The reference's own procedure (cmpy/exactdiag.py:248-299) with scipy's
     public expm_multiply: propagate phi under -i s H over linspace(start, stop,
-    num), overlap with phi, then multiply by the running phase factor."""
+    num), overlap with phi, then multiply by the running phase factor.
+
+    ``matvec`` (real x -> H x, e.g. the C twin's) makes H a LinearOperator
+    instead of the assembled CSR matrix; the trace expm_multiply needs is then
+    the sum of the operator's diagonal."""
     times, dt = np.linspace(start, stop, num, retstep=True)
-    h = (-1j * sign) * op.csr().astype(np.complex128)
+    if matvec is None:
+        h = (-1j * sign) * op.csr().astype(np.complex128)
+        trace = None
+    else:
+        def apply(x, s):
+            x = np.asarray(x)
+            y = np.asarray(matvec(np.ascontiguousarray(x.real))) + 1j * np.asarray(
+                matvec(np.ascontiguousarray(x.imag)))
+            return s * y
+
+        n = op.size
+        h = sla.LinearOperator((n, n), matvec=lambda x: apply(x.ravel(), -1j * sign),
+                               rmatvec=lambda x: apply(x.ravel(), +1j * sign), dtype=np.complex128)
+        trace = -1j * sign * float(np.sum(op.diagonal()))
     psi_t = sla.expm_multiply(h, np.asarray(phi, dtype=np.complex128), start=start, stop=stop, num=num,
-                              endpoint=True)
+                              endpoint=True, traceA=trace)
     overlaps = psi_t @ np.asarray(phi, dtype=np.complex128).conj()
     step = np.exp(+1j * sign * e0 * dt)
     factor = -1j * sign * np.exp(+1j * sign * e0 * times[0])
@@ -1078,7 +1101,8 @@ PARITY = {
     "test_models_hubbard.py:14-25",
     "coo_matvec/coo_dense": "test_models_hubbard.py:14-25; Hermiticity",
     "SectorOperator": "COO assembly at small sizes; free-fermion closed form",
-    "apply_cdag/apply_c": "test_operator.py:31-42 adjointness; Lange two-site KAT",
+    "apply_cdag/apply_c": "test_operator.py:31-42 adjointness; Lange two-site KAT; a block of "
+    "vectors maps column by column",
     "lanczos_normalized": "lanczos_coeffs_reference coefficients; dense eigh",
     "cf_greens": "lehmann_t0 equality at small size; Lange KAT",
     "lehmann_t0": "Lange two-site KAT (dmft/twosite.py:25-81); sum rule",
@@ -1091,7 +1115,7 @@ PARITY = {
     "Hubbard and Anderson sectors; tevo_two_site_ref at half filling",
     "tevo_expm_multiply": "tevo_part (dense eigh) and the two-site closed form",
     "tevo_two_site_ref": "impurity_params_ref poles (pinned via lehmann_t0); t = 0 value -i",
-    "lehmann_ij": "U = 0 resolvent_ij on rings and open chains at interior sites (a wrong string sign "
+    "lehmann_ij/lehmann_ij_many": "U = 0 resolvent_ij on rings and open chains at interior sites (a wrong string sign "
     "or a dropped factor fails it); i = j equals gf_lehmann; G_ij = G_ji",
     "resolvent_ij": "direct inversion of z - h",
     "lehmann_t0_ij": "U = 0 closed-shell ground state: resolvent_ij at interior sites; i = j equals lehmann_t0",

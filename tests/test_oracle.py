@@ -704,3 +704,27 @@ def test_lattice_bonds_and_free_fermion_energy_closed_form(nx, ny):
     k = np.arange(7)
     lv = np.sort(2 * t * np.cos(2 * np.pi * k / 7))
     assert abs(O.free_fermion_energy(7, O.chain_bonds(7, True), t, 0.0, 3, 2) - (lv[:3].sum() + lv[:2].sum())) < 1e-12
+
+
+def test_lehmann_ij_many_equals_single_pairs():
+    ns = 4
+    bonds = O.chain_bonds(ns, periodic=True)
+    mk = lambda nu, nd: O.hubbard_operator(ns, nu, nd, bonds, 2.0, 0.0, 1.0, mu=1.0)
+    z = np.linspace(-4, 4, 9) + 0.3j
+    pairs = [(0, 0), (0, 2), (1, 3)]
+    many = O.lehmann_ij_many(mk, ns, z, 3.0, pairs)
+    for (i, j), g in zip(pairs, many):
+        assert_allclose(g, O.lehmann_ij(mk, ns, z, 3.0, i, j), atol=1e-14)
+
+
+@pytest.mark.parametrize("mode", ["reference", "jordan_wigner"])
+@pytest.mark.parametrize("sigma", [O.UP, O.DN])
+def test_ladder_block_equals_columns(mode, sigma):
+    ns, nu, nd = 5, 2, 3
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((O.binom(ns, nu) * O.binom(ns, nd), 6))
+    for fn in (O.apply_cdag, O.apply_c):
+        for pos in range(ns):
+            blk = fn(X, ns, nu, nd, pos, sigma, mode)
+            cols = np.column_stack([fn(X[:, k], ns, nu, nd, pos, sigma, mode) for k in range(6)])
+            assert_array_equal(blk, cols)
