@@ -108,6 +108,8 @@ SIGNATURES = {
     "ed_op_destroy": (C.c_int, [vp]),
     "ed_op_shape": (C.c_int, [vp, P(i64), P(i64)]),
     "ed_op_matvec": (C.c_int, [vp, vp, vp, i64, vp]),
+    "ed_op_matvec_ld": (C.c_int, [vp, vp, vp, i64, i64, vp]),
+    "ed_op_vector_ld": (C.c_int, [vp, P(i64)]),
     "ed_csr_matvec": (C.c_int, [vp, vp, vp, i64, vp, vp, vp]),
     "ed_op_diagonal": (C.c_int, [vp, vp, vp]),
     "ed_op_dense": (C.c_int, [vp, vp, vp]),

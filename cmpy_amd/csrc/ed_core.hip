@@ -279,6 +279,7 @@ HvArgs make_hv_args(const ed_op* op) {
   HvArgs a;
   a.num_up = op->num_up;
   a.num_dn = op->num_dn;
+  a.ld = op->num_dn;
   a.st_up = op->st_up;
   a.st_dn = op->st_dn;
   a.d_up = op->d_up;

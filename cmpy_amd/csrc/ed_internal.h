@@ -225,6 +225,7 @@ namespace ed {
 // Plain matvec arguments (device pointers from the op).
 struct HvArgs {
   int64_t num_up, num_dn;
+  int64_t ld;  // row stride of x and y in doubles (>= num_dn)
   const uint32_t* st_up;
   const uint32_t* st_dn;
   const double* d_up;
@@ -253,10 +254,15 @@ struct HvEpilogue {
   double* partial;      // per-block partial dots (null = none)
 };
 
-// Launch helpers implemented in ed_matvec.hip.
-int launch_matvec(const ed_op* op, const double* x, double* y, const HvEpilogue& epi,
+// Launch helpers implemented in ed_matvec.hip.  Vectors are stored row by
+// row at a stride of ld >= num_dn doubles; the H.v writes zeros into y's
+// entries num_dn .. ld-1 of every row and never reads those of x.
+int launch_matvec(const ed_op* op, const double* x, double* y, int64_t ld, const HvEpilogue& epi,
                   hipStream_t stream, int* num_partials);
-int matvec_num_partials(const ed_op* op);
+int matvec_num_partials(const ed_op* op, int64_t ld);
+// Row stride the library's own vectors use: num_dn rounded up to 16 doubles
+// (128-byte rows) where the two-pass sweep runs, num_dn otherwise.
+int64_t vector_ld(const ed_op* op);
 
 // Deterministic reductions (ed_lanczos.hip).
 int launch_reduce(const double* partial, int n, double* out, int op_kind, hipStream_t stream);

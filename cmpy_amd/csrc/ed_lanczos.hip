@@ -100,23 +100,49 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-// u = v0 (or seeded uniform [-1, 1)), partial |u|^2
+// u = v0 (or seeded uniform [-1, 1)), partial |u|^2.  u is stored in rows
+// of ld >= nd doubles (zeros past nd); v0 and the seeded values are indexed
+// by the contiguous state index iu * nd + id.
 __global__ __launch_bounds__(kVecThreads) void k_start(double* __restrict__ u, const double* __restrict__ v0,
-                                                       int64_t D, uint64_t seed, double* partial) {
+                                                       int64_t nu, int64_t nd, int64_t ld, uint64_t seed,
+                                                       double* partial) {
   __shared__ double sh[kVecThreads / 64];
   double acc = 0.0;
+  const int64_t n = nu * ld;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < D; k += stride) {
-    double v;
-    if (v0)
-      v = v0[k];
-    else
-      v = (double)(splitmix64(seed ^ (0xD1B54A32D192ED03ull * (uint64_t)(k + 1))) >> 11) * 0x1.0p-53 * 2.0 - 1.0;
-    u[k] = v;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+    const int64_t iu = p / ld, id = p - iu * ld;
+    double v = 0.0;
+    if (id < nd) {
+      const int64_t k = iu * nd + id;
+      if (v0)
+        v = v0[k];
+      else
+        v = (double)(splitmix64(seed ^ (0xD1B54A32D192ED03ull * (uint64_t)(k + 1))) >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+    }
+    u[p] = v;
     acc += v * v;
   }
   const double s = block_sum(acc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// dst[v][iu * nd + id] = s * src[v * src_stride + iu * ld + id] for vectors
+// v = blockIdx.y (rows of ld doubles back to the contiguous layout); s =
+// *scale (or 1/*scale with invert), or 1 when scale is null.
+__global__ __launch_bounds__(kVecThreads) void k_compact(double* __restrict__ dst, const double* __restrict__ src,
+                                                         int64_t src_stride, int64_t nu, int64_t nd, int64_t ld,
+                                                         const double* scale, int invert) {
+  double s = scale ? *scale : 1.0;
+  if (scale && invert) s = s > 0.0 ? 1.0 / s : 0.0;
+  const int64_t D = nu * nd;
+  const double* sv = src + (int64_t)blockIdx.y * src_stride;
+  double* dv = dst + (int64_t)blockIdx.y * D;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < D; k += stride) {
+    const int64_t iu = k / nd, id = k - iu * nd;
+    dv[k] = s * sv[iu * ld + id];
+  }
 }
 
 // w -= alpha_j s_j u;  psi += yj s_j u (optional);  partial |w|^2 (optional)
@@ -393,7 +419,8 @@ void tri_lowest_vector(const double* a, const double* b, int m, double lam, std:
 // ---------------------------------------------------------------------------
 struct LzLayout {
   int64_t coef_off, part_off, vec_off, total;
-  int64_t D;
+  int64_t D;           // doubles per stored vector: nu rows of ld (zeros past nd)
+  int64_t nu, nd, ld;  // rows, states per row, row stride
   int M;
   int nvec;
   int npart;
@@ -403,9 +430,12 @@ static int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
 
 static LzLayout lz_layout(const ed_op* op, int M, int reorth, int want_vector, int basis_vectors) {
   LzLayout L{};
-  L.D = op->size;
+  L.nu = op->num_up;
+  L.nd = op->num_dn;
+  L.ld = vector_ld(op);
+  L.D = L.nu * L.ld;
   L.M = M;
-  const int P1 = matvec_num_partials(op);
+  const int P1 = matvec_num_partials(op, L.ld);
   const int P2 = vec_blocks(L.D);
   int64_t chunks = std::min<int64_t>((L.D + 4095) / 4096, 256);
   int64_t P3 = reorth ? (int64_t)(M + 1) * chunks : 0;
@@ -431,6 +461,21 @@ static LzCoef coef_ptrs(char* base, int M) {
 
 static double* vec_ptr(char* base, const LzLayout& L, int i) {
   return reinterpret_cast<double*>(base + L.vec_off + align256(sizeof(double) * L.D) * i);
+}
+
+// u = the start vector (v0 in the contiguous layout, or the seeded one) in
+// the padded layout; partial |u|^2 per block (vec_blocks(L.D) of them).
+static void launch_start(const LzLayout& L, hipStream_t s, double* u, const double* v0, uint64_t seed,
+                         double* part) {
+  hipLaunchKernelGGL(k_start, dim3(vec_blocks(L.D)), dim3(kVecThreads), 0, s, u, v0, L.nu, L.nd, L.ld, seed, part);
+}
+
+// nvec vectors stored in the padded layout (src_stride doubles apart) into
+// the contiguous layout, scaled by *scale (1/*scale with invert; 1 if null).
+static void launch_compact(const LzLayout& L, hipStream_t s, double* dst, const double* src, int64_t src_stride,
+                           int nvec, const double* scale, int invert) {
+  hipLaunchKernelGGL(k_compact, dim3(vec_blocks(L.nu * L.nd), (unsigned)nvec), dim3(kVecThreads), 0, s, dst, src,
+                     src_stride, L.nu, L.nd, L.ld, scale, invert);
 }
 
 __global__ void k_init_misc(double* misc) {
@@ -478,12 +523,7 @@ static int run_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, double
     if (a) ED_HIP(hipEventRecord(a, s));
   }
   int np = 0;
-  if (!reduce) {
-    // keep the epilogue path (sx/cprev) without partials
-    ED_TRY(launch_matvec(op, u_cur, w, e, s, &np));
-  } else {
-    ED_TRY(launch_matvec(op, u_cur, w, e, s, &np));
-  }
+  ED_TRY(launch_matvec(op, u_cur, w, L.ld, e, s, &np));
   if (tm && tm->on) {
     hipEvent_t b = tm->next();
     if (b) ED_HIP(hipEventRecord(b, s));
@@ -517,7 +557,7 @@ static ReorthCtx reorth_ctx(char* base, const LzLayout& L, double* part) {
 static int reorth_start(const LzLayout& L, LzCoef c, double* part, const ReorthCtx& r, const double* v0,
                         uint64_t seed, hipStream_t s) {
   const int vb = vec_blocks(L.D);
-  hipLaunchKernelGGL(k_start, dim3(vb), dim3(kVecThreads), 0, s, r.w, v0, L.D, seed, part);
+  launch_start(L, s, r.w, v0, seed, part);
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_NORM0, c, 0, nullptr);
   hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, r.V, r.w, L.D, c.scale, 0, nullptr);
   ED_HIP(hipGetLastError());
@@ -642,7 +682,7 @@ static int reorth_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, con
     if (a) ED_HIP(hipEventRecord(a, s));
   }
   int np = 0;
-  ED_TRY(launch_matvec(op, vj, r.w, e, s, &np));
+  ED_TRY(launch_matvec(op, vj, r.w, L.ld, e, s, &np));
   if (tm && tm->on) {
     hipEvent_t b = tm->next();
     if (b) ED_HIP(hipEventRecord(b, s));
@@ -751,7 +791,7 @@ extern "C" int ed_lanczos_groundstate(ed_op* op, const ed_lanczos_cfg* cfg, cons
   if (!cfg->reorth) {
     double* u[3] = {vec_ptr(base, L, 0), vec_ptr(base, L, 1), vec_ptr(base, L, 2)};
     double* psi = cfg->want_vector ? vec_ptr(base, L, 3) : nullptr;
-    hipLaunchKernelGGL(k_start, dim3(vb), dim3(kVecThreads), 0, s, u[0], v0_dev, D, cfg->seed, part);
+    launch_start(L, s, u[0], v0_dev, cfg->seed, part);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_NORM0, c, 0, nullptr);
     ED_HIP(hipGetLastError());
     int ip = 2, ic = 0, iw = 1;  // u_prev, u_cur, w
@@ -777,7 +817,7 @@ extern "C" int ed_lanczos_groundstate(ed_op* op, const ed_lanczos_cfg* cfg, cons
     if (cfg->want_vector) {
       // second pass: regenerate u_0..u_{m-1} and accumulate psi = sum_j y_j s_j u_j
       ED_HIP(hipMemsetAsync(psi, 0, sizeof(double) * D, s));
-      hipLaunchKernelGGL(k_start, dim3(vb), dim3(kVecThreads), 0, s, u[0], v0_dev, D, cfg->seed, part);
+      launch_start(L, s, u[0], v0_dev, cfg->seed, part);
       ip = 2;
       ic = 0;
       iw = 1;
@@ -835,16 +875,18 @@ extern "C" int ed_lanczos_groundstate(ed_op* op, const ed_lanczos_cfg* cfg, cons
   res->breakdown = breakdown ? 1 : 0;
 
   if (cfg->want_vector) {
-    // normalise psi into psi_dev and compute the explicit residual |H psi - e0 psi|
+    // normalise psi (in place, padded layout), compute the explicit residual
+    // |H psi - e0 psi|, and write psi into psi_dev in the contiguous layout
     double* psi = cfg->reorth ? vec_ptr(base, L, M + 1) : vec_ptr(base, L, 3);
-    hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, psi_dev, psi, D, c.misc + 1, 0, part);
+    hipLaunchKernelGGL(k_norm2, dim3(vb), dim3(kVecThreads), 0, s, psi, D, part, nullptr);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_SQRT_TO, c, 0, c.misc + 4);
-    hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, psi_dev, psi_dev, D, c.misc + 4, 1, nullptr);
-    double* w = cfg->reorth ? vec_ptr(base, L, 0) : vec_ptr(base, L, 0);
+    hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, psi, psi, D, c.misc + 4, 1, nullptr);
+    double* w = vec_ptr(base, L, 0);
     HvEpilogue none{nullptr, nullptr, nullptr, nullptr};
-    ED_TRY(launch_matvec(op, psi_dev, w, none, s, nullptr));
-    hipLaunchKernelGGL(k_resid, dim3(vb), dim3(kVecThreads), 0, s, w, psi_dev, D, e0, part);
+    ED_TRY(launch_matvec(op, psi, w, L.ld, none, s, nullptr));
+    hipLaunchKernelGGL(k_resid, dim3(vb), dim3(kVecThreads), 0, s, w, psi, D, e0, part);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_SQRT_TO, c, 0, c.misc + 5);
+    launch_compact(L, s, psi_dev, psi, 0, 1, nullptr, 0);
     ED_HIP(hipGetLastError());
     double r[2];
     ED_HIP(hipMemcpyAsync(r, c.misc + 4, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
@@ -896,17 +938,16 @@ extern "C" int ed_lanczos_coeffs(ed_op* op, const double* v0_dev, int32_t nsteps
     ProBuffer pro_buf{nullptr, s};
     if (reorth == 2) ED_TRY(pro_alloc(pro, pro_buf, s));
     for (int j = 0; j < M; ++j) ED_TRY(reorth_step(op, L, c, part, rc, j, s, nullptr, reorth == 2 ? &pro : nullptr));
-    if (basis_dev) ED_HIP(hipMemcpyAsync(basis_dev, rc.V, sizeof(double) * D * M, hipMemcpyDeviceToDevice, s));
+    // basis vector j is at rc.V + j D
+    if (basis_dev) launch_compact(L, s, basis_dev, rc.V, D, M, nullptr, 0);
   } else {
     double* u[3] = {vec_ptr(base, L, 0), vec_ptr(base, L, 1), vec_ptr(base, L, 2)};
-    hipLaunchKernelGGL(k_start, dim3(vb), dim3(kVecThreads), 0, s, u[0], v0_dev, D, (uint64_t)0, part);
+    launch_start(L, s, u[0], v0_dev, 0, part);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_NORM0, c, 0, nullptr);
     ED_HIP(hipGetLastError());
     int ip = 2, ic = 0, iw = 1;
     for (int j = 0; j < M; ++j) {
-      if (basis_dev)
-        hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, basis_dev + (int64_t)j * D, u[ic], D,
-                           c.scale + j, 0, nullptr);
+      if (basis_dev) launch_compact(L, s, basis_dev + (int64_t)j * L.nu * L.nd, u[ic], 0, 1, c.scale + j, 0);
       ED_TRY(run_step(op, L, c, part, u[ip], u[ic], u[iw], j, s, nullptr, nullptr, 0.0, true));
       const int old_p = ip;
       ip = ic;

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void k_hv(HvArgs a, const double* __restrict__
     ct = b % nct;
     rb = b / nct;
   }
-  const int64_t nd = a.num_dn;
+  const int64_t nd = a.num_dn, ld = a.ld;
   const int64_t id = ct * 256 + threadIdx.x;
   const int64_t r0 = rb * RB;
   const bool act = id < nd;
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void k_hv(HvArgs a, const double* __restrict__
     for (int r = 0; r < RB; ++r) {
       const int64_t iu = r0 + r;
       if (iu < a.num_up) {
-        xo[r] = x[iu * nd + id];
+        xo[r] = x[iu * ld + id];
         acc[r] = (a.d_up[iu] + ddn + u_term_dev(a, a.st_up[iu] & sdn)) * xo[r];
       } else {
         xo[r] = 0.0;
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void k_hv(HvArgs a, const double* __restrict__
       const double v = a.dn_val[(int64_t)k * nd + id];
 #pragma unroll
       for (int r = 0; r < RB; ++r)
-        if (r0 + r < a.num_up) acc[r] += v * x[(r0 + r) * nd + c];
+        if (r0 + r < a.num_up) acc[r] += v * x[(r0 + r) * ld + c];
     }
     // up hops: uniform list per row, coalesced source rows
 #pragma unroll
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void k_hv(HvArgs a, const double* __restrict__
         const int cu = a.up_cnt[iu];
         const int32_t* cp = a.up_col + iu * a.up_w;
         const double* vp = a.up_val + iu * a.up_w;
-        for (int k = 0; k < cu; ++k) acc[r] += vp[k] * x[(int64_t)cp[k] * nd + id];
+        for (int k = 0; k < cu; ++k) acc[r] += vp[k] * x[(int64_t)cp[k] * ld + id];
       }
     }
     if (LZ) {
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void k_hv(HvArgs a, const double* __restrict__
       for (int r = 0; r < RB; ++r) {
         const int64_t iu = r0 + r;
         if (iu < a.num_up) {
-          const int64_t k = iu * nd + id;
+          const int64_t k = iu * ld + id;
           double v = sx * acc[r];
           if (e.xprev) v -= cp * e.xprev[k];
           y[k] = v;
@@ -115,9 +115,14 @@ __global__ __launch_bounds__(256) void k_hv(HvArgs a, const double* __restrict__
 #pragma unroll
       for (int r = 0; r < RB; ++r) {
         const int64_t iu = r0 + r;
-        if (iu < a.num_up) y[iu * nd + id] = acc[r];
+        if (iu < a.num_up) y[iu * ld + id] = acc[r];
       }
     }
+  } else if (id < ld) {
+    // row padding: zeros
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+      if (r0 + r < a.num_up) y[(r0 + r) * ld + id] = 0.0;
   }
   if (LZ && e.partial) {
     const double s = block_sum_256(dot, red);
@@ -140,6 +145,7 @@ __global__ __launch_bounds__(256) void k_hv(HvArgs a, const double* __restrict__
 // ---------------------------------------------------------------------------
 struct Hv2Args {
   int64_t nu, nd;
+  int64_t ld;  // row stride of x and y in doubles (>= nd)
   const uint32_t* st_up;
   const uint32_t* st_dn;
   const double* d_up;
@@ -201,6 +207,16 @@ __device__ __forceinline__ double block_sum_n(double v, P red) {
   return r;
 }
 
+// Zeros into the row padding (entries nd .. ld-1) of rows r0 .. r0+R-1 of y.
+__device__ __forceinline__ void zero_row_pads(double* y, int64_t r0, int R, int64_t nd, int64_t ld) {
+  const int64_t np = ld - nd;
+  if (np <= 0) return;
+  for (int64_t k = threadIdx.x; k < (int64_t)R * np; k += blockDim.x) {
+    const int64_t r = k / np;
+    y[(r0 + r) * ld + nd + (k - r * np)] = 0.0;
+  }
+}
+
 // Load from an address that is the same for every lane of the wave, through
 // the scalar cache (the tables it reads are never written by these kernels).
 template <typename T>
@@ -255,12 +271,16 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows(Hv2Args a, int rows_per
                                                          double* __restrict__ y, HvEpilogue e) {
   extern __shared__ double lds[];
   __shared__ double red[kRowThreads / 64];
-  const int64_t nd = a.nd;
+  const int64_t nd = a.nd, ld = a.ld;
   const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
   const int R = (int)(a.nu - r0 < rows_per_block ? a.nu - r0 : rows_per_block);
   double* hv = lds + (int64_t)rows_per_block * nd;
-  stage_rows<kRowThreads, 8>(lds, x + r0 * nd, (int64_t)R * nd);
+  if (ld == nd)
+    stage_rows<kRowThreads, 8>(lds, x + r0 * nd, (int64_t)R * nd);
+  else
+    for (int r = 0; r < R; ++r) stage_rows<kRowThreads, 8>(lds + (int64_t)r * nd, x + (r0 + r) * ld, nd);
   for (int b = threadIdx.x; b < a.nb; b += kRowThreads) hv[b] = a.hop[b];
+  zero_row_pads(y, r0, R, nd, ld);
   __syncthreads();
   const double sx = e.sx ? *e.sx : 1.0;
   const double cp = e.cprev ? *e.cprev : 0.0;
@@ -277,7 +297,7 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows(Hv2Args a, int rows_per
     const int cmax = wave_max_int(cnt);
     for (int r = 0; r < R; ++r) {
       const int64_t iu = r0 + r;
-      const int64_t gi = iu * nd + id;
+      const int64_t gi = iu * ld + id;
       const double xp = (e.xprev && act) ? e.xprev[gi] : 0.0;
       const double* row = lds + (int64_t)r * nd;
       const double xo = act ? row[ids] : 0.0;
@@ -354,12 +374,13 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_off(Hv2Args a, int rows
                                                              double* __restrict__ y, HvEpilogue e) {
   extern __shared__ double lds[];
   __shared__ double red[kRowThreads / 64];
-  const int64_t nd = a.nd;
+  const int64_t nd = a.nd, ld = a.ld;
   const uint32_t stride = (uint32_t)nd + 1u;  // doubles per staged row
   const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
   const int R = (int)(a.nu - r0 < rows_per_block ? a.nu - r0 : rows_per_block);
-  for (int r = 0; r < R; ++r) stage_rows<kRowThreads, 8>(lds + (int64_t)r * stride, x + (r0 + r) * nd, nd);
+  for (int r = 0; r < R; ++r) stage_rows<kRowThreads, 8>(lds + (int64_t)r * stride, x + (r0 + r) * ld, nd);
   if ((int)threadIdx.x < R) lds[(int64_t)threadIdx.x * stride + nd] = 0.0;
+  zero_row_pads(y, r0, R, nd, ld);
   __syncthreads();
   const double sx = e.sx ? *e.sx : 1.0;
   const double cp = e.cprev ? *e.cprev : 0.0;
@@ -380,8 +401,8 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_off(Hv2Args a, int rows
       double xp[RB], yu[RB];
 #pragma unroll
       for (int r = 0; r < RB; ++r) {
-        xp[r] = (e.xprev && act && rg + r < R) ? e.xprev[(r0 + rg + r) * nd + id] : 0.0;
-        yu[r] = (ADDUP && act && rg + r < R) ? __builtin_nontemporal_load(y + (r0 + rg + r) * nd + id) : 0.0;
+        xp[r] = (e.xprev && act && rg + r < R) ? e.xprev[(r0 + rg + r) * ld + id] : 0.0;
+        yu[r] = (ADDUP && act && rg + r < R) ? __builtin_nontemporal_load(y + (r0 + rg + r) * ld + id) : 0.0;
       }
       double accp[RB], accn[RB];
 #pragma unroll
@@ -395,7 +416,7 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_off(Hv2Args a, int rows
           const double xo = lds[(int64_t)(rg + r) * stride + id];
           const double dg = a.d_up[iu] + ddn + u_term2(a, a.st_up[iu] & sdn);
           const double v = sx * (dg * xo + t * (accp[r] - accn[r])) + yu[r] - cp * xp[r];
-          __builtin_nontemporal_store(v, y + iu * nd + id);
+          __builtin_nontemporal_store(v, y + iu * ld + id);
           dot += v * xo;
         }
       }
@@ -459,11 +480,12 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_half(Hv2Args a, int row
   extern __shared__ double dyn_lds[];
   (void)dyn_lds;
   lds_f64* row = (lds_f64*)(uintptr_t)0;
-  const int64_t nd = a.nd;
+  const int64_t nd = a.nd, ld = a.ld;
   lds_f64* red = row + nd + 32;
   const int64_t iu = blockIdx.x;
-  stage_rows<kRowThreads, 16>(row, x + iu * nd, nd);
+  stage_rows<kRowThreads, 16>(row, x + iu * ld, nd);
   if (threadIdx.x < 32) row[nd + threadIdx.x] = 0.0;
+  zero_row_pads(y, iu, 1, nd, ld);
   __syncthreads();
   const double sx = e.sx ? *e.sx : 1.0;
   const double cp = e.cprev ? *e.cprev : 0.0;
@@ -480,7 +502,7 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_half(Hv2Args a, int row
     const int64_t chunk = c0 >> 6;
     const int nwp = uniform_load(a.dn_hwords + 2 * chunk);
     const int nw = uniform_load(a.dn_hwords + 2 * chunk + 1);
-    const int64_t gi = iu * nd + ids;
+    const int64_t gi = iu * ld + ids;
     const double xp = e.xprev ? e.xprev[gi] : 0.0;
     const double yu = ADDUP ? __builtin_nontemporal_load(y + gi) : 0.0;
     const uint32_t sdn = a.st_dn[ids];
@@ -548,7 +570,7 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_held(Hv2Args a, int row
   extern __shared__ double dyn_lds[];
   (void)dyn_lds;
   lds_f64* row = (lds_f64*)(uintptr_t)0;
-  const int64_t nd = a.nd;
+  const int64_t nd = a.nd, ld = a.ld;
   lds_f64* red = row + nd + 32;
   const int64_t iu = blockIdx.x;
   const int lane = threadIdx.x & 63;
@@ -564,13 +586,14 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_held(Hv2Args a, int row
     ph[k] = 0.0;
     if (c0 < nd) {
       const int64_t id = c0 + lane;
-      const int64_t gi = iu * nd + (id < nd ? id : c0);
+      const int64_t gi = iu * ld + (id < nd ? id : c0);
       if (ADDUP) held[k] = __builtin_nontemporal_load(y + gi);
       if (e.xprev) ph[k] = e.xprev[gi];
     }
   }
-  stage_rows<kRowThreads, 16>(row, x + iu * nd, nd);
+  stage_rows<kRowThreads, 16>(row, x + iu * ld, nd);
   if (threadIdx.x < 32) row[nd + threadIdx.x] = 0.0;
+  zero_row_pads(y, iu, 1, nd, ld);
 #pragma unroll
   for (int k = 0; k < kRowHeld; ++k) held[k] = fma(-cp, ph[k], held[k]);
   __syncthreads();
@@ -588,7 +611,7 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_held(Hv2Args a, int row
     const int64_t chunk = c0 >> 6;
     const int nwp = uniform_load(a.dn_hwords + 2 * chunk);
     const int nw = uniform_load(a.dn_hwords + 2 * chunk + 1);
-    const int64_t gi = iu * nd + ids;
+    const int64_t gi = iu * ld + ids;
     const uint32_t sdn = a.st_dn[ids];
     const double ddn = a.d_dn[ids];
     const double pn =
@@ -625,7 +648,7 @@ __global__ __launch_bounds__(256) void k_hv_cols(Hv2Args a, const double* __rest
   const int64_t xcd = b & 7, j = b >> 3;
   const int64_t tile = (j / nrc) * 8 + xcd;
   const int64_t rc = j % nrc;
-  const int64_t nd = a.nd;
+  const int64_t nd = a.nd, ld = a.ld;
   const int64_t row0 = rc * rch;
   for (int i = threadIdx.x; i < a.nb; i += 256) hv[i] = a.hop[i];
   for (int i = threadIdx.x; i < rch * MAXH; i += 256) {
@@ -652,7 +675,7 @@ __global__ __launch_bounds__(256) void k_hv_cols(Hv2Args a, const double* __rest
       const uint32_t* ent = sent + it * MAXH;
       // the output's read-modify-write operands are requested before the gathers
       const bool wr = act && iu < a.nu;
-      const int64_t gw = wr ? iu * nd + id : ids;
+      const int64_t gw = wr ? iu * ld + id : ids;
       const double yold = __builtin_nontemporal_load(y + gw);
       const double xown = x[gw];
       double acc = 0.0;
@@ -668,7 +691,7 @@ __global__ __launch_bounds__(256) void k_hv_cols(Hv2Args a, const double* __rest
 #pragma unroll
         for (int q = 0; q < B; ++q) {
           const bool ok = k0 + q < cnt;
-          g[q] = x[ok ? (int64_t)hop_target(en[q]) * nd + ids : ids];
+          g[q] = x[ok ? (int64_t)hop_target(en[q]) * ld + ids : ids];
         }
 #pragma unroll
         for (int q = 0; q < B; ++q) acc += (k0 + q < cnt ? hv_val<UNI>(en[q], t0, hv) : 0.0) * g[q];
@@ -705,7 +728,7 @@ __global__ __launch_bounds__(256) void k_hv_cols_split(Hv2Args a, const double* 
   const int64_t xcd = b & 7, j = b >> 3;
   const int64_t tile = (j / nrc) * 8 + xcd;
   const int64_t rc = j % nrc;
-  const int64_t nd = a.nd;
+  const int64_t nd = a.nd, ld = a.ld;
   const int64_t row0 = rc * rch;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -716,7 +739,7 @@ __global__ __launch_bounds__(256) void k_hv_cols_split(Hv2Args a, const double* 
     const bool act = col < nd;
     const int64_t colc = act ? col : tile * TW;
     const uint32_t laneoff = (uint32_t)colc * 8u;
-    const uint32_t nd8 = (uint32_t)nd * 8u;
+    const uint32_t nd8 = (uint32_t)ld * 8u;  // bytes per row of x
     const char* xb = (const char*)x;
     const double sx = e.sx ? *e.sx : 1.0;
     const double st = sx * a.hop[0];
@@ -724,7 +747,7 @@ __global__ __launch_bounds__(256) void k_hv_cols_split(Hv2Args a, const double* 
     for (int it = wave; it < rch; it += 4) {
       const int64_t iu = row0 + it;
       if (iu >= a.nu) break;
-      const int64_t gw = iu * nd + colc;
+      const int64_t gw = iu * ld + colc;
       double yold = 0.0, xown = 0.0;
       if (wr) {
         yold = __builtin_nontemporal_load(y + gw);
@@ -889,10 +912,10 @@ __global__ __launch_bounds__(256) void k_hv_cols_pipe(Hv2Args a, const double* _
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   double dot = 0.0;
   if (tile < ntiles) {
-    const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(x, (uint32_t)(a.nu * a.nd * 8));
+    const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(x, (uint32_t)(a.nu * a.ld * 8));
     const double st = (e.sx ? *e.sx : 1.0) * a.hop[0];
     const PairLane L = pair_lane<TW>(tile, a.nd);
-    const uint32_t nd8 = (uint32_t)a.nd * 8u;
+    const uint32_t nd8 = (uint32_t)a.ld * 8u;  // bytes per row of x
     const int64_t rend = row0 + rch < a.nu ? row0 + rch : a.nu;
     int64_t iu = row0 + wave;
     PairList nxt;
@@ -905,7 +928,7 @@ __global__ __launch_bounds__(256) void k_hv_cols_pipe(Hv2Args a, const double* _
       const PairList cur = nxt;
       const int64_t inx = iu + 4;
       const int pp = (int)(cur.hdr & 0xffu), npr = (int)((cur.hdr >> 8) & 0xffu);
-      const int64_t gw = iu * a.nd + L.colc;
+      const int64_t gw = iu * a.ld + L.colc;
       double yold = 0.0, xown = 0.0;
       if (L.wr) {
         yold = __builtin_nontemporal_load(y + gw);
@@ -944,10 +967,10 @@ __global__ __launch_bounds__(256) void k_hv_cols_first(Hv2Args a, const double* 
   const int64_t row0 = (j % nrc) * rch;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (tile >= ntiles) return;
-  const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(x, (uint32_t)(a.nu * a.nd * 8));
+  const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(x, (uint32_t)(a.nu * a.ld * 8));
   const double st = (e.sx ? *e.sx : 1.0) * a.hop[0];
   const PairLane L = pair_lane<TW>(tile, a.nd);
-  const uint32_t nd8 = (uint32_t)a.nd * 8u;
+  const uint32_t nd8 = (uint32_t)a.ld * 8u;  // bytes per row of x
   const int64_t rend = row0 + rch < a.nu ? row0 + rch : a.nu;
   int64_t iu = row0 + wave;
   PairList nxt;
@@ -966,7 +989,7 @@ __global__ __launch_bounds__(256) void k_hv_cols_first(Hv2Args a, const double* 
     });
     if (TW == 32) acc += __shfl_xor(acc, 32, 64);
     pend = st * acc;
-    pend_at = y + iu * a.nd + L.colc;
+    pend_at = y + iu * a.ld + L.colc;
     iu = inx;
   }
   if (L.wr && pend_at) __builtin_nontemporal_store(pend, pend_at);
@@ -1055,9 +1078,10 @@ __global__ __launch_bounds__(256) void k_hv_cols_quad(Hv2Args a, const double* _
   const int64_t colc = act ? col : nd - 2;
   const uint32_t laneoff = (uint32_t)colc * 8u;
   const bool wr = act && q == 0;
-  const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(x, (uint32_t)(a.nu * nd * 8));
+  const int64_t ld = a.ld;
+  const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(x, (uint32_t)(a.nu * ld * 8));
   const double st = (e.sx ? *e.sx : 1.0) * a.hop[0];
-  const uint32_t nd8 = (uint32_t)nd * 8u;
+  const uint32_t nd8 = (uint32_t)ld * 8u;  // bytes per row of x
   const int64_t rend = row0 + rch < a.nu ? row0 + rch : a.nu;
   int64_t iu = row0 + wave;
   QuadList nxt;
@@ -1080,7 +1104,7 @@ __global__ __launch_bounds__(256) void k_hv_cols_quad(Hv2Args a, const double* _
     s0 += __shfl_xor(s0, 32, 64);
     s1 += __shfl_xor(s1, 32, 64);
     pend = f64x2{st * s0, st * s1};
-    pend_at = (f64x2*)(y + iu * nd + colc);
+    pend_at = (f64x2*)(y + iu * ld + colc);
     iu = inx;
   }
   if (wr && pend_at) __builtin_nontemporal_store(pend, pend_at);
@@ -1091,10 +1115,11 @@ static int env_int(const char* name, int dflt) {
   return v && *v ? atoi(v) : dflt;
 }
 
-static Hv2Args make_hv2_args(const ed_op* op) {
+static Hv2Args make_hv2_args(const ed_op* op, int64_t ld) {
   Hv2Args a;
   a.nu = op->num_up;
   a.nd = op->num_dn;
+  a.ld = ld;
   a.st_up = op->st_up;
   a.st_dn = op->st_dn;
   a.d_up = op->d_up;
@@ -1201,7 +1226,7 @@ constexpr int64_t kMinRowBlocks = 512;
 // tables exist (CMPY_AMD_HV_SPLIT=0 turns them off), general packed tables
 // otherwise.  CMPY_AMD_HV_MODE: 0 single fused pass, 1 two passes, unset =
 // two passes from D = 65536 up.
-static TwoPassPlan plan_two_pass(const ed_op* op) {
+static TwoPassPlan plan_two_pass(const ed_op* op, int64_t ld) {
   TwoPassPlan p{};
   p.ok = false;
   const int mode = op->hv_mode >= 0 ? op->hv_mode : env_int("CMPY_AMD_HV_MODE", -1);
@@ -1212,7 +1237,7 @@ static TwoPassPlan plan_two_pass(const ed_op* op) {
   bool uni = op->num_bonds > 0;
   for (double h : op->hop)
     if (h != op->hop[0]) uni = false;
-  const bool split = uni && op->dn.off_pos && op->up.tgt_sorted && op->size * 8 <= (1ll << 32) &&
+  const bool split = uni && op->dn.off_pos && op->up.tgt_sorted && op->num_up * ld * 8 <= (1ll << 32) &&
                      op->up.tgt_stride <= 32 && env_int("CMPY_AMD_HV_SPLIT", 1) != 0;
   int dev = 0, max_lds = 65536;
   (void)hipGetDevice(&dev);
@@ -1270,7 +1295,7 @@ static TwoPassPlan plan_two_pass(const ed_op* op) {
   // pad row included); CMPY_AMD_HV_PAIR=0 turns it off, CMPY_AMD_HV_PTW picks
   // 32- (default) or 64-column tiles for the pair lists, CMPY_AMD_HV_PRCH
   // rows per block, CMPY_AMD_HV_FIRST=0 runs it after the row pass
-  if (split && op->up.pair16 && (op->num_up + 1) * op->num_dn * 8 < (1ll << 32) &&
+  if (split && op->up.pair16 && (op->num_up + 1) * ld * 8 < (1ll << 32) &&
       env_int("CMPY_AMD_HV_PAIR", 1) != 0) {
     p.tw = env_int("CMPY_AMD_HV_PTW", 32) == 64 ? 64 : 32;
     p.rch = std::max(4, std::min(256, env_int("CMPY_AMD_HV_PRCH", 8))) / 4 * 4;
@@ -1280,7 +1305,7 @@ static TwoPassPlan plan_two_pass(const ed_op* op) {
       // lists with 16-byte loads where num_dn is even (CMPY_AMD_HV_QUAD=0
       // turns them off)
       p.col_first = true;
-      if (op->up.quad16 && op->num_dn % 2 == 0 && env_int("CMPY_AMD_HV_QUAD", 1) != 0) {
+      if (op->up.quad16 && op->num_dn % 2 == 0 && ld % 2 == 0 && env_int("CMPY_AMD_HV_QUAD", 1) != 0) {
         p.tw = 32;
         p.ntiles = (op->num_dn + 31) / 32;
         p.ck = k_hv_cols_quad;
@@ -1310,9 +1335,9 @@ static TwoPassPlan plan_two_pass(const ed_op* op) {
   return p;
 }
 
-static int launch_two_pass(const ed_op* op, const TwoPassPlan& p, const double* x, double* y,
+static int launch_two_pass(const ed_op* op, const TwoPassPlan& p, const double* x, double* y, int64_t ld,
                            const HvEpilogue& epi, hipStream_t s, int* num_partials) {
-  const Hv2Args a = make_hv2_args(op);
+  const Hv2Args a = make_hv2_args(op, ld);
   if (p.col_first) {
     hipLaunchKernelGGL(p.ck, dim3((unsigned)p.grid_c), dim3(256), 0, s, a, x, y, epi, p.ntiles, p.nrc, p.rch,
                        nullptr);
@@ -1337,17 +1362,26 @@ static int pick_rb(const ed_op* op) {
   return 1;
 }
 
-static void grid_shape(const ed_op* op, int rb, int64_t* nrb, int64_t* nct) {
+static void grid_shape(const ed_op* op, int rb, int64_t ld, int64_t* nrb, int64_t* nct) {
   *nrb = (op->num_up + rb - 1) / rb;
-  *nct = (op->num_dn + 255) / 256;
+  *nct = (ld + 255) / 256;  // column tiles cover the row padding too (written as zeros)
 }
 
-int matvec_num_partials(const ed_op* op) {
-  const TwoPassPlan p = plan_two_pass(op);
+int matvec_num_partials(const ed_op* op, int64_t ld) {
+  const TwoPassPlan p = plan_two_pass(op, ld);
   if (p.ok) return (int)(p.col_first ? p.grid_r : p.grid_r + p.grid_c);
   int64_t nrb, nct;
-  grid_shape(op, pick_rb(op), &nrb, &nct);
+  grid_shape(op, pick_rb(op), ld, &nrb, &nct);
   return (int)(nrb * nct);
+}
+
+int64_t vector_ld(const ed_op* op) {
+  const int64_t nd = op->num_dn;
+  const int64_t ld = (nd + 15) / 16 * 16;
+  if (ld == nd || env_int("CMPY_AMD_HV_PITCH", 1) == 0) return nd;
+  // pad only where the two-pass sweep runs on the padded rows (its column
+  // slabs then cover whole 128-byte lines)
+  return plan_two_pass(op, ld).ok ? ld : nd;
 }
 
 template <bool LZ>
@@ -1363,14 +1397,15 @@ static void launch_rb(int rb, const HvArgs& a, const double* x, double* y, const
   }
 }
 
-int launch_matvec(const ed_op* op, const double* x, double* y, const HvEpilogue& epi, hipStream_t s,
+int launch_matvec(const ed_op* op, const double* x, double* y, int64_t ld, const HvEpilogue& epi, hipStream_t s,
                   int* num_partials) {
-  const TwoPassPlan p = plan_two_pass(op);
-  if (p.ok) return launch_two_pass(op, p, x, y, epi, s, num_partials);
-  const HvArgs a = make_hv_args(op);
+  const TwoPassPlan p = plan_two_pass(op, ld);
+  if (p.ok) return launch_two_pass(op, p, x, y, ld, epi, s, num_partials);
+  HvArgs a = make_hv_args(op);
+  a.ld = ld;
   const int rb = pick_rb(op);
   int64_t nrb, nct;
-  grid_shape(op, rb, &nrb, &nct);
+  grid_shape(op, rb, ld, &nrb, &nct);
   ED_REQUIRE(nrb * nct < (1ll << 31), "grid too large");
   const bool lz = epi.xprev || epi.sx || epi.cprev || epi.partial;
   if (lz)
@@ -1422,6 +1457,30 @@ extern "C" int ed_op_matvec(ed_op* op, const double* x_dev, double* y_dev, int64
              "x and y overlap");
   HvEpilogue none{nullptr, nullptr, nullptr, nullptr};
   for (int64_t v = 0; v < nvec; ++v)
-    ED_TRY(launch_matvec(op, x_dev + v * D, y_dev + v * D, none, as_stream(stream), nullptr));
+    ED_TRY(launch_matvec(op, x_dev + v * D, y_dev + v * D, op->num_dn, none, as_stream(stream), nullptr));
+  return ED_OK;
+}
+
+extern "C" int ed_op_matvec_ld(ed_op* op, const double* x_dev, double* y_dev, int64_t ld, int64_t nvec,
+                               void* stream) {
+  clear_error();
+  ED_REQUIRE(op && x_dev && y_dev, "NULL argument");
+  ED_REQUIRE(nvec >= 0, "nvec < 0");
+  ED_REQUIRE(ld >= op->num_dn && ld <= op->num_dn + 4096, "ld=%lld outside [num_dn, num_dn + 4096]",
+             (long long)ld);
+  const int64_t n = op->num_up * ld;  // doubles per vector
+  const char* xb = (const char*)x_dev;
+  const char* yb = (const char*)y_dev;
+  ED_REQUIRE(xb + sizeof(double) * n * nvec <= yb || yb + sizeof(double) * n * nvec <= xb, "x and y overlap");
+  HvEpilogue none{nullptr, nullptr, nullptr, nullptr};
+  for (int64_t v = 0; v < nvec; ++v)
+    ED_TRY(launch_matvec(op, x_dev + v * n, y_dev + v * n, ld, none, as_stream(stream), nullptr));
+  return ED_OK;
+}
+
+extern "C" int ed_op_vector_ld(const ed_op* op, int64_t* ld) {
+  clear_error();
+  ED_REQUIRE(op && ld, "NULL argument");
+  *ld = vector_ld(op);
   return ED_OK;
 }

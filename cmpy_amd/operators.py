@@ -286,6 +286,31 @@ class SectorHamiltonOperator(LinearOperator):
         _lib.call("ed_op_matvec", self._h, _lib.dptr(x), _lib.dptr(out), nvec, _lib.stream_ptr())
         return out
 
+    @property
+    def vector_ld(self) -> int:
+        """Row stride (doubles) of the padded vector layout the library's own
+        Lanczos uses for this sector: num_dn rounded up to 128-byte rows where
+        the two-pass sweep runs, num_dn otherwise."""
+        ld = C.c_int64()
+        _lib.call("ed_op_vector_ld", self._h, C.byref(ld))
+        return ld.value
+
+    def matvec_device_ld(self, x: torch.Tensor, ld: int, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """y = H x for CUDA float64 tensors holding num_up rows of ``ld`` doubles
+        each ((num_up, ld) or (k, num_up, ld)); entries past num_dn in a row are
+        not read in x and are written as zeros in y."""
+        if x.dtype != torch.float64 or not x.is_cuda:
+            raise TypeError("matvec_device_ld expects a CUDA float64 tensor")
+        x = x.contiguous()
+        n = self.num_up * int(ld)
+        nvec = x.numel() // n if n else 0
+        if n == 0 or x.numel() != nvec * n:
+            raise ValueError(f"expected rows of {ld} doubles ({n} per vector), got shape {tuple(x.shape)}")
+        if out is None:
+            out = torch.empty_like(x)
+        _lib.call("ed_op_matvec_ld", self._h, _lib.dptr(x), _lib.dptr(out), int(ld), nvec, _lib.stream_ptr())
+        return out
+
     def diagonal_device(self) -> torch.Tensor:
         d = torch.empty(self.size, dtype=torch.float64, device="cuda")
         _lib.call("ed_op_diagonal", self._h, _lib.dptr(d), _lib.stream_ptr())

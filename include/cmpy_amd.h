@@ -123,6 +123,20 @@ int ed_op_shape(const ed_op* op, int64_t* num_up, int64_t* num_dn);
  * overlap.                                                                  */
 int ed_op_matvec(ed_op* op, const double* x_dev, double* y_dev, int64_t nvec, void* stream);
 
+/* The same H.v on vectors stored row by row with a row stride of ld doubles
+ * (num_dn <= ld <= num_dn + 4096; [nvec][num_up][ld]): entry (iu, id) of a
+ * vector sits at iu * ld + id.  x's entries past num_dn in a row are never
+ * read; y's are written as zeros.  No reference counterpart: the reference's
+ * vectors are contiguous (cmpy/operators.py:626-630, ld = num_dn here).
+ * Padding rows to whole 128-byte lines keeps the column pass's slabs of x
+ * line-aligned (see ed_op_vector_ld).                                       */
+int ed_op_matvec_ld(ed_op* op, const double* x_dev, double* y_dev, int64_t ld, int64_t nvec, void* stream);
+
+/* Row stride the library's own Lanczos vectors use for this operator:
+ * num_dn rounded up to 16 doubles where the two-pass sweep runs, else num_dn
+ * (environment CMPY_AMD_HV_PITCH=0: always num_dn).                         */
+int ed_op_vector_ld(const ed_op* op, int64_t* ld);
+
 /* y = H x for H given as triples sorted by target row (CSR by target):
  * y[t] = sum_{k=ptr[t]}^{ptr[t+1]-1} val[k] * x[src[k]], in stored order.
  * The COO HamiltonOperator path (cmpy/operators.py:614-630).              */

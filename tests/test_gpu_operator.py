@@ -232,6 +232,16 @@ def test_large_sector_matvec_both_kernels(gpu, geom, monkeypatch):
         op = SectorHamiltonOperator(ns, nu, nd, bonds, hops, eps, u, w)
         y = op.matvec(x)
         assert np.abs(y - y_ref).max() <= 1e-13 * scale, case
+        # rows padded to ld: padding of x never read (NaN), of y written as 0
+        nrow, ncol = op.num_up, op.num_dn
+        for ld in (ncol + (-ncol) % 16 or ncol + 16, ncol + 3):
+            xp = np.full((nrow, ld), np.nan)
+            xp[:, :ncol] = x.reshape(nrow, ncol)
+            yp = torch.full((nrow, ld), float("nan"), dtype=torch.float64, device="cuda")
+            op.matvec_device_ld(torch.from_numpy(xp).cuda(), ld, out=yp)
+            yp = yp.cpu().numpy()
+            assert np.all(yp[:, ncol:] == 0.0), (case, ld)
+            assert np.abs(yp[:, :ncol].ravel() - y).max() <= 1e-13 * scale, (case, ld)
         hops_u = np.full(len(bonds), 0.7)
         op_u = SectorHamiltonOperator(ns, nu, nd, bonds, hops_u, eps, u, w)
         ref_u = O.SectorOperator(ns, nu, nd, bonds, hops_u, eps, u, w)
@@ -260,6 +270,23 @@ def test_hv_4x4_matches_cpu_twin(gpu):
     hb = op.matvec_device(b)
     lhs, rhs = float(torch.dot(x, hb)), float(torch.dot(y, b))
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_padded_rows_default_stride(gpu):
+    """The library's own vector stride: 128-byte rows where the two-pass sweep
+    runs (num_dn not a multiple of 16), contiguous otherwise."""
+    from cmpy_amd import SectorHamiltonOperator
+
+    ns, bonds = 12, O.chain_bonds(12)
+    big = SectorHamiltonOperator(ns, 6, 6, bonds, [1.0] * 11, np.zeros(ns), np.full(ns, 4.0), ns)
+    assert big.num_dn == 924 and big.vector_ld == 928
+    small = SectorHamiltonOperator(6, 3, 3, O.chain_bonds(6), [1.0] * 5, np.zeros(6), np.full(6, 4.0), 6)
+    assert small.vector_ld == small.num_dn
+    from cmpy_amd import _lib as L
+
+    x = torch.zeros((big.num_up, 928), dtype=torch.float64, device="cuda")
+    with pytest.raises(L.EDInvalidArgument, match="ld="):
+        L.call("ed_op_matvec_ld", big.handle, L.dptr(x), L.dptr(x.clone()), 900, 1, L.stream_ptr())
 
 
 def test_invalid_arguments_raise(gpu):

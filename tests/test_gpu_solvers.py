@@ -454,3 +454,26 @@ def test_greens_cf_records_and_warns_reorth_fallback(gpu, monkeypatch):
     assert (r2.reorth_p, r2.reorth_h) == ("none", "none")
     r3 = ed.greens_cf(model, z, 3, 3, nsteps=50, reorth="partial")
     assert (r3.reorth_p, r3.reorth_h) == ("partial", "partial")
+
+
+@pytest.mark.parametrize("reorth", [False, "partial"])
+def test_groundstate_padded_rows_equal_contiguous(gpu, monkeypatch, reorth):
+    """The solver's vectors in 128-byte rows (the default where the two-pass
+    sweep runs) give the ground state of the contiguous layout: 12-site chain,
+    (6,6), rows of 924 states padded to 928."""
+    from cmpy_amd import HubbardModel, chain_neighbors, greens_cf, groundstate
+
+    model = HubbardModel(12, chain_neighbors(12), inter=4.0, hop=1.0, mu=2.0)
+    z = np.linspace(-6, 6, 64) + 1.0j
+    out = {}
+    for pitch in ("1", "0"):
+        monkeypatch.setenv("CMPY_AMD_HV_PITCH", pitch)
+        model.clear_operator_cache()
+        gs, st = groundstate(model, 6, 6, tol=1e-10, reorth=reorth, return_stats=True)
+        cf = greens_cf(model, z, 6, 6, nsteps=60, gs=gs, reorth="full")
+        out[pitch] = (gs, st, cf)
+    (g1, s1, c1), (g0, s0, c0) = out["1"], out["0"]
+    assert s1.converged and s0.converged and s1.residual <= 1e-10
+    assert abs(g1.energy - g0.energy) <= 1e-12 * abs(g0.energy)
+    assert abs(abs(np.dot(g1.state, g0.state)) - 1.0) <= 1e-12
+    assert np.abs(c1.gf - c0.gf).max() <= 1e-10
