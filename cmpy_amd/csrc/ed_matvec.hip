@@ -1177,6 +1177,22 @@ static int maxh_for(int m) {
   return 0;
 }
 
+// The half-width row kernels address their dynamic LDS from offset 0
+// (lds_f64* row = 0), which is right only while they declare no static LDS:
+// true if the compiled kernel has none (checked once per kernel).
+static bool lds_starts_at_zero(RowKernel k) {
+  static std::mutex mu;
+  static std::vector<std::pair<const void*, bool>> seen;
+  std::lock_guard<std::mutex> lk(mu);
+  for (auto& v : seen)
+    if (v.first == (const void*)k) return v.second;
+  hipFuncAttributes attr{};
+  const bool ok = hipFuncGetAttributes(&attr, (const void*)k) == hipSuccess && attr.sharedSizeBytes == 0;
+  if (!ok) (void)hipGetLastError();
+  seen.emplace_back((const void*)k, ok);
+  return ok;
+}
+
 // Grants a row kernel `bytes` of dynamic LDS (needed above 64 KiB); false if refused.
 static bool ensure_row_lds(RowKernel k, size_t bytes) {
   static std::mutex mu;
@@ -1255,7 +1271,8 @@ static TwoPassPlan plan_two_pass(const ed_op* op, int64_t ld) {
   // one-row blocks with the half-width tables where at most two rows fit
   const bool half = split && R <= 2 && op->dn.h16 &&
                     sizeof(double) * (size_t)(op->num_dn + 32 + kRowThreads / 64) <= budget &&
-                    env_int("CMPY_AMD_HV_HALF", 1) != 0;
+                    env_int("CMPY_AMD_HV_HALF", 1) != 0 && lds_starts_at_zero(k_hv_rows_half<false>) &&
+                    lds_starts_at_zero(k_hv_rows_half<true>) && lds_starts_at_zero(k_hv_rows_held<true>);
   if (half) {
     p.rows_per_block = R = 1;
     p.lds_bytes = sizeof(double) * (size_t)(op->num_dn + 32 + kRowThreads / 64);

@@ -3,12 +3,14 @@
 * 12-site open chain, (6,6), U in {2, 4, 8}: the 200-step continued fraction
   on 2048 real frequencies at Im z = 0.05, fully reorthogonalised, against the
   oracle's reorthogonalised Lanczos + continued fraction started from the same
-  c+psi0 / c psi0 (the GPU's ground state).  A 200-step fraction at Im z = 0.05
-  is not converged, and its value is ill-conditioned in the start vector:
-  the oracle itself moves by delta when the start vector is perturbed by
-  1e-15 (relative, random).  The tolerance is 100 delta (a Lanczos run's
-  rounding acts like a start-vector perturbation of order 1e-13), and at
-  Im z = 1, where the fraction is well conditioned, 1e-10.
+  c+psi0 / c psi0 (the GPU's ground state).  A 200-step fraction at
+  Im z = 0.05 is not converged, and its value is ill-conditioned in the start
+  vector: the oracle itself moves by delta when its start vector is perturbed
+  by 1e-15 (relative, random).  The tolerance is max(1e-8, 100 delta) (a
+  Lanczos run's rounding acts like a start-vector perturbation of up to
+  ~1e-13), and 1e-10 at Im z = 1, where the fraction is well conditioned.
+  Measured: delta = 7.1e-6 / 1.2e-8 / 6.8e-9 and |G_gpu - G_oracle| =
+  1.6e-5 / 1.2e-9 / 1.4e-9 at U = 2 / 4 / 8.
 * Batched impurity solves (512 models, n_b in 5..9, beta = 50): the largest
   models (D = 63,504 and 15,876) against the oracle's eigsh E0 (1e-10
   relative) and its reorthogonalised-Lanczos G(i w_n) (1e-8).
