@@ -1315,7 +1315,7 @@ static TwoPassPlan plan_two_pass(const ed_op* op, int64_t ld) {
   if (split && op->up.pair16 && (op->num_up + 1) * ld * 8 < (1ll << 32) &&
       env_int("CMPY_AMD_HV_PAIR", 1) != 0) {
     p.tw = env_int("CMPY_AMD_HV_PTW", 32) == 64 ? 64 : 32;
-    p.rch = std::max(4, std::min(256, env_int("CMPY_AMD_HV_PRCH", 8))) / 4 * 4;
+    p.rch = std::max(4, std::min(256, env_int("CMPY_AMD_HV_PRCH", 16))) / 4 * 4;
     p.ntiles = (op->num_dn + p.tw - 1) / p.tw;
     if (env_int("CMPY_AMD_HV_FIRST", 1) != 0) {
       // column pass first, writing y; the row pass adds to it.  The quad
