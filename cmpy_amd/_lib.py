@@ -125,6 +125,7 @@ SIGNATURES = {
     "ed_pole_sum": (C.c_int, [vp, vp, i64, vp, i64, vp, i32, vp]),
     "ed_lehmann_pairs": (C.c_int, [vp, vp, i64, vp, i64, dbl, dbl, vp, i64, vp, vp]),
     "ed_lehmann_pairs_ij": (C.c_int, [vp, vp, vp, i64, vp, i64, dbl, dbl, vp, i64, vp, vp]),
+    "ed_lehmann_pairs_noalloc": (C.c_int, [vp, vp, i64, vp, i64, dbl, dbl, vp, i64, vp, vp]),
     "ed_gemm_nt": (C.c_int, [vp, vp, vp, i64, i64, i64, vp]),
     "ed_occupation": (C.c_int, [vp, i64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp]),
     "ed_occupation_states": (C.c_int, [vp, i64, vp, i64, vp, i64, C.c_int, C.c_int, vp, vp, vp]),

@@ -618,21 +618,35 @@ int ed_pole_sum(const double* w_dev, const double* p_dev, int64_t m, const doubl
   return ED_OK;
 }
 
+// Scratch of the pair sum, in doubles: the per-group partials (2 per group
+// and frequency) and the two screening bounds (int64).  Every entry is
+// written by the kernels of the call that uses it before it is read.
+static int64_t pair_scratch_doubles(int64_t ng, int64_t nz) { return 2 * ng * nz + 2; }
+
+// `scratch`: device memory of pair_scratch_doubles(ng, nz) doubles, or null
+// for stream-ordered scratch allocated (and freed) here.
 static int lehmann_pairs(const double* ov, const double* ov2, const double* evals_dev, int64_t n,
                          const double* evals_p1_dev, int64_t m, double beta, double emin, const double* z_dev,
-                         int64_t nz, double* g_dev, hipStream_t s) {
+                         int64_t nz, double* g_dev, hipStream_t s, double* scratch = nullptr,
+                         int64_t scratch_doubles = 0) {
   ED_REQUIRE(n >= 0 && m >= 0 && nz >= 0, "negative size");
   if (n == 0 || m == 0 || nz == 0) return ED_OK;
   ED_REQUIRE(ov && evals_dev && evals_p1_dev && z_dev && g_dev, "NULL buffer");
   ED_REQUIRE(!std::isnan(beta) && std::isfinite(emin), "beta/emin invalid");
   const int64_t ng = (n * m + kPairsPerGroup - 1) / kPairsPerGroup;
   ED_REQUIRE(ng <= 65535, "sector pair too large for the pair-sum grid (%lld groups)", (long long)ng);
+  if (scratch && pair_scratch_doubles(ng, nz) > scratch_doubles)
+    return set_error(ED_ERR_LIMIT,
+                     "pair sum of %lld x %lld pairs on %lld frequencies needs %lld scratch doubles; the "
+                     "fixed scratch holds %lld (2 x groups x frequencies + 2, 8192 pairs per group)",
+                     (long long)m, (long long)n, (long long)nz, (long long)pair_scratch_doubles(ng, nz),
+                     (long long)scratch_doubles);
   // CMPY_AMD_PAIR_SCREEN=0 sums every pair
   const char* sv = getenv("CMPY_AMD_PAIR_SCREEN");
   const int screen = !(sv && *sv && atoi(sv) == 0);
   // stream-ordered scratch: no device-wide synchronisation per sector pair
-  double* part = nullptr;
-  ED_HIP(hipMallocAsync((void**)&part, sizeof(double) * 2 * ng * nz + 2 * sizeof(int64_t), s));
+  double* part = scratch;
+  if (!part) ED_HIP(hipMallocAsync((void**)&part, sizeof(double) * pair_scratch_doubles(ng, nz), s));
   int64_t* keep = reinterpret_cast<int64_t*>(part + 2 * ng * nz);
   int rc = ED_OK;
   hipLaunchKernelGGL(k_pairs_keep, dim3(1), dim3(256), 0, s, evals_dev, n, evals_p1_dev, m, beta, emin, screen, keep);
@@ -640,8 +654,35 @@ static int lehmann_pairs(const double* ov, const double* ov2, const double* eval
                      n, evals_p1_dev, m, beta, emin, keep, z_dev, nz, part);
   hipLaunchKernelGGL(k_pairs_reduce, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, part, ng, nz, g_dev);
   if (hipGetLastError() != hipSuccess) rc = set_error(ED_ERR_HIP, "pair-sum launch failed");
-  if (hipFreeAsync(part, s) != hipSuccess && rc == ED_OK) rc = set_error(ED_ERR_HIP, "hipFreeAsync failed");
+  if (!scratch && hipFreeAsync(part, s) != hipSuccess && rc == ED_OK) rc = set_error(ED_ERR_HIP, "hipFreeAsync failed");
   return rc;
+}
+
+// Fixed scratch of ed_lehmann_pairs_noalloc: 4 MiB per device, resident for
+// the life of the process.  Host code never passes the symbol itself to a
+// kernel (on the host it is only a stand-in): the device address comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+constexpr int64_t kPairScratchDoubles = int64_t(1) << 19;
+static __device__ __align__(256) double scratch[kPairScratchDoubles];
+
+static std::mutex g_scratch_mu;
+static std::vector<double*> g_scratch_ptr;  // per device; null until resolved
+
+static int pair_scratch_ptr(double** out) {
+  int dev = 0;
+  ED_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    double* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(ED_ERR_HIP, "hipGetSymbolAddress of the pair-sum scratch failed on device %d: %s", dev,
+                       hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return ED_OK;
 }
 
 int ed_lehmann_pairs(const double* overlap_dev, const double* evals_dev, int64_t n, const double* evals_p1_dev,
@@ -650,6 +691,19 @@ int ed_lehmann_pairs(const double* overlap_dev, const double* evals_dev, int64_t
   clear_error();
   return lehmann_pairs(overlap_dev, nullptr, evals_dev, n, evals_p1_dev, m, beta, emin, z_dev, nz, g_dev,
                        as_stream(stream));
+}
+
+int ed_lehmann_pairs_noalloc(const double* overlap_dev, const double* evals_dev, int64_t n,
+                             const double* evals_p1_dev, int64_t m, double beta, double emin, const double* z_dev,
+                             int64_t nz, double* g_dev, void* stream) {
+  clear_error();
+  if (n <= 0 || m <= 0 || nz <= 0)
+    return lehmann_pairs(overlap_dev, nullptr, evals_dev, n, evals_p1_dev, m, beta, emin, z_dev, nz, g_dev,
+                         as_stream(stream));  // validates and returns before any scratch is needed
+  double* p = nullptr;
+  ED_TRY(pair_scratch_ptr(&p));
+  return lehmann_pairs(overlap_dev, nullptr, evals_dev, n, evals_p1_dev, m, beta, emin, z_dev, nz, g_dev,
+                       as_stream(stream), p, kPairScratchDoubles);
 }
 
 int ed_lehmann_pairs_ij(const double* overlap_i_dev, const double* overlap_j_dev, const double* evals_dev, int64_t n,

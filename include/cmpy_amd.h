@@ -267,6 +267,20 @@ int ed_lehmann_pairs(const double* overlap_dev, const double* evals_dev, int64_t
                      const double* evals_p1_dev, int64_t m, double beta, double emin,
                      const double* z_dev, int64_t nz, double* g_dev, void* stream);
 
+/* ed_lehmann_pairs without any allocation.  No counterpart in the reference:
+ * this entry point is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Its scratch (per-group partials and the screening bounds) is a fixed
+ * 4 MiB device array per GPU (524,288 doubles, held until the process ends),
+ * so it needs 2 * ceil(n * m / 8192) * nz + 2 <= 524,288 and returns
+ * ED_ERR_LIMIT beyond that.  Same arguments, kernels and results as
+ * ed_lehmann_pairs.  Every call on one GPU uses the same array: calls that
+ * overlap on one GPU (different streams or threads) race.  Each call writes
+ * all the scratch it reads, so nothing has to be cleared between calls.   */
+int ed_lehmann_pairs_noalloc(const double* overlap_dev, const double* evals_dev, int64_t n,
+                             const double* evals_p1_dev, int64_t m, double beta, double emin,
+                             const double* z_dev, int64_t nz, double* g_dev, void* stream);
+
 /* Off-diagonal pair sum: as ed_lehmann_pairs with |ov[m,n]|^2 replaced by
  * ov_i[m,n] ov_j[m,n], where ov_x = V_p1^T (c+_x evecs), i.e.
  *   g[k] += sum_{m,n} <n|c_i|m><m|c+_j|n> (e^{-beta(E_n-emin)} + e^{-beta(E_m-emin)})
