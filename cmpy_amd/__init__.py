@@ -28,4 +28,6 @@ from .exactdiag import (
     occupation_up, occupation_dn, occupation, double_occupation, accumulate_gf,
 )
 
+from .streams import map_streams
+
 __version__ = "0.1.0"

@@ -63,7 +63,7 @@ class LanczosCfg(C.Structure):
     _fields_ = [
         ("max_iter", C.c_int32), ("min_iter", C.c_int32), ("check_every", C.c_int32),
         ("reorth", C.c_int32), ("want_vector", C.c_int32), ("fixed_iter", C.c_int32),
-        ("tol", C.c_double), ("seed", C.c_uint64),
+        ("tol", C.c_double), ("seed", C.c_uint64), ("time_matvec", C.c_int32), ("pad", C.c_int32),
     ]
 
 

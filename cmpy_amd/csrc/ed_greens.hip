@@ -539,8 +539,10 @@ int ed_cf_eval(const double* alpha, const double* beta, int32_t m, double norm2,
   std::vector<double> h(2 * (size_t)m);
   for (int j = 0; j < m; ++j) h[j] = alpha[j];
   for (int j = 0; j + 1 < m; ++j) h[m + j] = beta[j] * beta[j];
+  // stream-ordered buffer: no device-wide synchronisation, so evaluations on
+  // other streams keep running
   double* d = nullptr;
-  ED_HIP(hipMalloc(&d, sizeof(double) * 2 * m));
+  ED_HIP(hipMallocAsync((void**)&d, sizeof(double) * 2 * m, s));
   int rc = ED_OK;
   if (hipMemcpyAsync(d, h.data(), sizeof(double) * 2 * m, hipMemcpyHostToDevice, s) != hipSuccess) {
     rc = set_error(ED_ERR_HIP, "copying CF coefficients failed");
@@ -549,8 +551,9 @@ int ed_cf_eval(const double* alpha, const double* beta, int32_t m, double norm2,
                        sign, z_dev, nz, g_dev, accumulate);
     if (hipGetLastError() != hipSuccess) rc = set_error(ED_ERR_HIP, "k_cf_eval launch failed");
   }
+  (void)hipFreeAsync(d, s);
+  // h (pageable host memory) must outlive the copy
   if (hipStreamSynchronize(s) != hipSuccess && rc == ED_OK) rc = set_error(ED_ERR_HIP, "stream sync failed");
-  (void)hipFree(d);
   return rc;
 }
 

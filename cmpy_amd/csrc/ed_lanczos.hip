@@ -35,6 +35,22 @@ struct LzCoef {
 constexpr int kRedThreads = 1024;
 constexpr int kVecThreads = 256;
 
+// beta_{j+1} = sqrt(S) from S = |w|^2, zero (and the breakdown flag set)
+// where it falls below 1e-12 of |alpha_j| + beta_j or is not finite; then
+// scale_{j+1} = 1/beta_{j+1} and cprev_{j+1} = beta_{j+1} scale_j.
+__device__ __forceinline__ void finish_beta(LzCoef c, int j, double S) {
+  double b = sqrt(fmax(S, 0.0));
+  const double ref = fabs(c.alpha[j]) + (j > 0 ? c.beta[j] : 0.0);
+  if (c.misc[0] != 0.0 || !(b > 1e-12 * ref) || !isfinite(b)) {
+    if (!isfinite(b)) c.misc[3] = 1.0;
+    b = 0.0;
+    c.misc[0] = 1.0;
+  }
+  c.beta[j + 1] = b;
+  c.scale[j + 1] = b > 0.0 ? 1.0 / b : 0.0;
+  c.cprev[j + 1] = b * c.scale[j];
+}
+
 __global__ __launch_bounds__(kRedThreads) void k_reduce(const double* __restrict__ partial, int n,
                                                         int kind, LzCoef c, int j, double* out) {
   __shared__ double sh[kRedThreads / 64];
@@ -60,19 +76,9 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(const double* __restrict
     case RED_ALPHA:
       c.alpha[j] = c.scale[j] * S;
       break;
-    case RED_BETA: {
-      double b = sqrt(fmax(S, 0.0));
-      const double ref = fabs(c.alpha[j]) + (j > 0 ? c.beta[j] : 0.0);
-      if (c.misc[0] != 0.0 || !(b > 1e-12 * ref) || !isfinite(b)) {
-        if (!isfinite(b)) c.misc[3] = 1.0;
-        b = 0.0;
-        c.misc[0] = 1.0;
-      }
-      c.beta[j + 1] = b;
-      c.scale[j + 1] = b > 0.0 ? 1.0 / b : 0.0;
-      c.cprev[j + 1] = b * c.scale[j];
+    case RED_BETA:
+      finish_beta(c, j, S);
       break;
-    }
     case RED_SUM_TO:
       *out = S;
       break;
@@ -91,6 +97,27 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   if (threadIdx.x == 0)
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += sh[w];
   return r;
+}
+
+// alpha_j = s_j S with S the sum of the np partials of <w, u_j> that the
+// H.v sweep left, summed by every block in the same order (so every block
+// holds the same bits); block 0 stores alpha_j.  Returns alpha_j s_j, the
+// factor of u_j in the update.  sh holds blockDim.x / 64 doubles.
+__device__ __forceinline__ double fused_alpha(const double* __restrict__ hv_part, int np, double* alpha,
+                                              const double* scale, double* sh) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) v += hv_part[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double S = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) S += sh[w];
+  __syncthreads();  // sh is reused by the block's own sum
+  const double sj = *scale;
+  const double al = sj * S;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *alpha = al;
+  return al * sj;
 }
 
 __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
@@ -127,13 +154,13 @@ __global__ __launch_bounds__(kVecThreads) void k_start(double* __restrict__ u, c
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// dst[v][iu * nd + id] = s * src[v * src_stride + iu * ld + id] for vectors
-// v = blockIdx.y (rows of ld doubles back to the contiguous layout); s =
-// *scale (or 1/*scale with invert), or 1 when scale is null.
+// dst[v][iu * nd + id] = s_v * src[v * src_stride + iu * ld + id] for vectors
+// v = blockIdx.y (rows of ld doubles back to the contiguous layout); s_v =
+// scale[v] (or 1/scale[v] with invert), or 1 when scale is null.
 __global__ __launch_bounds__(kVecThreads) void k_compact(double* __restrict__ dst, const double* __restrict__ src,
                                                          int64_t src_stride, int64_t nu, int64_t nd, int64_t ld,
                                                          const double* scale, int invert) {
-  double s = scale ? *scale : 1.0;
+  double s = scale ? scale[blockIdx.y] : 1.0;
   if (scale && invert) s = s > 0.0 ? 1.0 / s : 0.0;
   const int64_t D = nu * nd;
   const double* sv = src + (int64_t)blockIdx.y * src_stride;
@@ -145,12 +172,16 @@ __global__ __launch_bounds__(kVecThreads) void k_compact(double* __restrict__ ds
   }
 }
 
-// w -= alpha_j s_j u;  psi += yj s_j u (optional);  partial |w|^2 (optional)
+// w -= alpha_j s_j u;  psi += yj s_j u (optional);  partial |w|^2 (optional).
+// With hv_part, alpha_j is first formed from the H.v partials (fused_alpha)
+// and stored; without, it is read from *alpha.  partial and hv_part must not
+// overlap (blocks write the one while others still read the other).
 __global__ __launch_bounds__(kVecThreads) void k_update(double* __restrict__ w, const double* __restrict__ u,
-                                                        int64_t D, const double* alpha, const double* scale,
-                                                        double* __restrict__ psi, double yj, double* partial) {
+                                                        int64_t D, double* alpha, const double* scale,
+                                                        double* __restrict__ psi, double yj, double* partial,
+                                                        const double* __restrict__ hv_part, int np) {
   __shared__ double sh[kVecThreads / 64];
-  const double a = (*alpha) * (*scale);
+  const double a = hv_part ? fused_alpha(hv_part, np, alpha, scale, sh) : (*alpha) * (*scale);
   const double ys = yj * (*scale);
   double acc = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -170,10 +201,11 @@ __global__ __launch_bounds__(kVecThreads) void k_update(double* __restrict__ w, 
 // k_update on pairs of doubles (16-byte loads and stores; all pointers
 // 16-byte aligned, D2 = D / 2 with D even)
 __global__ __launch_bounds__(kVecThreads) void k_update2(double2* __restrict__ w, const double2* __restrict__ u,
-                                                         int64_t D2, const double* alpha, const double* scale,
-                                                         double2* __restrict__ psi, double yj, double* partial) {
+                                                         int64_t D2, double* alpha, const double* scale,
+                                                         double2* __restrict__ psi, double yj, double* partial,
+                                                         const double* __restrict__ hv_part, int np) {
   __shared__ double sh[kVecThreads / 64];
-  const double a = (*alpha) * (*scale);
+  const double a = hv_part ? fused_alpha(hv_part, np, alpha, scale, sh) : (*alpha) * (*scale);
   const double ys = yj * (*scale);
   double acc = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -243,36 +275,71 @@ __global__ __launch_bounds__(kVecThreads) void k_resid(const double* __restrict_
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// Blocks (c, i0): partials of <V_i, w> over chunk c for i = i0, i0 +
-// gridDim.y, ... < nv (a bounded grid, so a gated-off launch is cheap).
-// gate (optional): skip the whole launch unless *gate != 0 (partial
-// reorthogonalisation decides on the device whether a step needs it).
+// Block (c0, i0) sums <V_i, w> over the chunks c = c0, c0 + gridDim.x, ...
+// < nchunks for i = i0, i0 + gridDim.y, ... < nv, into partial[i *
+// gridDim.x + c0] (a bounded grid, so a gated-off launch costs one small
+// dispatch).  gate (optional): skip the whole launch unless *gate != 0
+// (partial reorthogonalisation decides on the device whether a step needs it).
 constexpr int kMultidotRows = 32;
+constexpr int kMultidotChunks = 32;  // blocks along the chunks
 __global__ __launch_bounds__(kVecThreads) void k_multidot(const double* __restrict__ V, int64_t D,
                                                           const double* __restrict__ w, int64_t chunk,
                                                           double* partial, int nchunks, int nv, const int* gate) {
   if (gate && *gate == 0) return;
   __shared__ double sh[kVecThreads / 64];
-  const int c = blockIdx.x;
-  const int64_t k0 = (int64_t)c * chunk, k1 = std::min<int64_t>(D, k0 + chunk);
   for (int i = blockIdx.y; i < nv; i += gridDim.y) {
     const double* vi = V + (int64_t)i * D;
     double acc = 0.0;
-    for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) acc += vi[k] * w[k];
+    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+      const int64_t k0 = (int64_t)c * chunk, k1 = std::min<int64_t>(D, k0 + chunk);
+      for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) acc += vi[k] * w[k];
+    }
     const double s = block_sum(acc, sh);
-    if (threadIdx.x == 0) partial[(int64_t)i * nchunks + c] = s;
+    if (threadIdx.x == 0) partial[(int64_t)i * gridDim.x + blockIdx.x] = s;
     __syncthreads();  // sh is reused by the next vector's sum
   }
 }
 
-__global__ __launch_bounds__(kVecThreads) void k_reduce_rows(const double* __restrict__ partial, int nchunks,
-                                                             double* out, const int* gate) {
+// One classical Gram-Schmidt pass against the stored basis v_i = scale_i V_i:
+// coef_i = scale_i^2 sum_c dots[i][c] (the ng partials of <V_i, w> per vector
+// that k_multidot left, summed by every block in the same order), then
+// w -= sum_i coef_i V_i, and the partial |w|^2 of this block into
+// wpart[blockIdx.x].  gate as in k_multidot.
+__global__ __launch_bounds__(kVecThreads) void k_gs_update(const double* __restrict__ V, int nv, int64_t D,
+                                                           const double* __restrict__ dots, int ng,
+                                                           const double* __restrict__ scale, double* __restrict__ w,
+                                                           double* __restrict__ wpart, const int* gate) {
   if (gate && *gate == 0) return;
+  extern __shared__ double cs[];
   __shared__ double sh[kVecThreads / 64];
-  double acc = 0.0;
-  for (int c = threadIdx.x; c < nchunks; c += blockDim.x) acc += partial[(int64_t)blockIdx.x * nchunks + c];
-  const double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) out[blockIdx.x] = s;
+  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+    double acc = 0.0;
+    for (int c = 0; c < ng; ++c) acc += dots[(int64_t)i * ng + c];
+    cs[i] = acc * scale[i] * scale[i];
+  }
+  __syncthreads();
+  double n2 = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < D; k += stride) {
+    double acc = 0.0;
+    for (int i = 0; i < nv; ++i) acc += cs[i] * V[(int64_t)i * D + k];
+    const double wk = w[k] - acc;
+    w[k] = wk;
+    n2 += wk * wk;
+  }
+  const double t = block_sum(n2, sh);
+  if (threadIdx.x == 0) wpart[blockIdx.x] = t;
+}
+
+// beta_{j+1} from n partials of |w|^2 (finish_beta), unless *gate == 0.
+__global__ __launch_bounds__(kRedThreads) void k_beta_gated(const double* __restrict__ partial, int n, LzCoef c,
+                                                            int j, const int* gate) {
+  if (*gate == 0) return;
+  __shared__ double sh[kRedThreads / 64];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += kRedThreads) v += partial[i];
+  const double S = block_sum(v, sh);
+  if (threadIdx.x == 0) finish_beta(c, j, S);
 }
 
 // w -= sum_i coef_i V_i   (sign = -1)   or   w = sum_i coef_i V_i (sign = 0)
@@ -291,18 +358,22 @@ __global__ __launch_bounds__(kVecThreads) void k_axpy_multi(const double* __rest
   }
 }
 
+constexpr int kAxpyBlocks = 512;  // grid of the (gated) basis combination
 static int vec_blocks(int64_t D) { return (int)std::max<int64_t>(1, std::min<int64_t>((D + 255) / 256, 2048)); }
 
 // k_update, as 16-byte accesses where D is even and every pointer is 16-byte
-// aligned; vb partials either way.
-static void launch_update(int vb, hipStream_t s, double* w, const double* u, int64_t D, const double* alpha,
-                          const double* scale, double* psi, double yj, double* partial) {
+// aligned; vb partials either way.  hv_part (np partials, optional): form
+// and store alpha_j first (see k_update).
+static void launch_update(int vb, hipStream_t s, double* w, const double* u, int64_t D, double* alpha,
+                          const double* scale, double* psi, double yj, double* partial,
+                          const double* hv_part = nullptr, int np = 0) {
   const uintptr_t al = (uintptr_t)w | (uintptr_t)u | (uintptr_t)psi;
   if (D % 2 == 0 && (al & 15) == 0)
     hipLaunchKernelGGL(k_update2, dim3(vb), dim3(kVecThreads), 0, s, (double2*)w, (const double2*)u, D / 2, alpha,
-                       scale, (double2*)psi, yj, partial);
+                       scale, (double2*)psi, yj, partial, hv_part, np);
   else
-    hipLaunchKernelGGL(k_update, dim3(vb), dim3(kVecThreads), 0, s, w, u, D, alpha, scale, psi, yj, partial);
+    hipLaunchKernelGGL(k_update, dim3(vb), dim3(kVecThreads), 0, s, w, u, D, alpha, scale, psi, yj, partial,
+                       hv_part, np);
 }
 
 int launch_reduce(const double* partial, int n, double* out, int kind, hipStream_t stream) {
@@ -417,13 +488,17 @@ void tri_lowest_vector(const double* a, const double* b, int m, double lam, std:
 // ---------------------------------------------------------------------------
 // Workspace layout
 // ---------------------------------------------------------------------------
+// Partials: region A (npart doubles: the H.v sweep's and the Gram-Schmidt
+// dot products'), M + 2 coefficients, region B (npart2 doubles: the vector
+// updates' |w|^2), then with reorthogonalisation the orthogonality estimate
+// (3 (M + 2) doubles and two flags).
 struct LzLayout {
-  int64_t coef_off, part_off, vec_off, total;
+  int64_t coef_off, part_off, part2_off, pro_off, vec_off, total;
   int64_t D;           // doubles per stored vector: nu rows of ld (zeros past nd)
   int64_t nu, nd, ld;  // rows, states per row, row stride
   int M;
   int nvec;
-  int npart;
+  int npart, npart2;
 };
 
 static int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
@@ -440,10 +515,14 @@ static LzLayout lz_layout(const ed_op* op, int M, int reorth, int want_vector, i
   int64_t chunks = std::min<int64_t>((L.D + 4095) / 4096, 256);
   int64_t P3 = reorth ? (int64_t)(M + 1) * chunks : 0;
   L.npart = (int)std::max<int64_t>({(int64_t)P1, (int64_t)P2, P3, (int64_t)(M + 2)});
+  L.npart2 = P2;
   L.nvec = reorth ? (M + 2) : (3 + (want_vector ? 1 : 0) + basis_vectors);
   L.coef_off = 0;
   L.part_off = align256(L.coef_off + sizeof(double) * (4 * (int64_t)M + 4 + 16));
-  L.vec_off = align256(L.part_off + sizeof(double) * (int64_t)(L.npart + M + 2));
+  L.part2_off = align256(L.part_off + sizeof(double) * (int64_t)(L.npart + M + 2));
+  L.pro_off = align256(L.part2_off + sizeof(double) * (int64_t)L.npart2);
+  const int64_t pro_bytes = reorth ? (int64_t)sizeof(double) * (3 * ((int64_t)M + 2) + 1) : 0;
+  L.vec_off = align256(L.pro_off + pro_bytes);
   L.total = L.vec_off + align256(sizeof(double) * L.D) * L.nvec;
   return L;
 }
@@ -471,7 +550,8 @@ static void launch_start(const LzLayout& L, hipStream_t s, double* u, const doub
 }
 
 // nvec vectors stored in the padded layout (src_stride doubles apart) into
-// the contiguous layout, scaled by *scale (1/*scale with invert; 1 if null).
+// the contiguous layout, vector v scaled by scale[v] (1/scale[v] with invert;
+// 1 if null).
 static void launch_compact(const LzLayout& L, hipStream_t s, double* dst, const double* src, int64_t src_stride,
                            int nvec, const double* scale, int invert) {
   hipLaunchKernelGGL(k_compact, dim3(vec_blocks(L.nu * L.nd), (unsigned)nvec), dim3(kVecThreads), 0, s, dst, src,
@@ -511,8 +591,13 @@ struct EventTimer {
   }
 };
 
-static int run_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, double* u_prev, double* u_cur,
-                    double* w, int j, hipStream_t s, EventTimer* tm, double* psi, double yj, bool reduce) {
+// One step of the three-vector recurrence: H.v with the epilogue (partials
+// of <w, u_j> into part), the update with alpha_j formed in it (|w|^2
+// partials into part2), beta_{j+1}.  Without reduce (the eigenvector's
+// second pass) alpha_j and beta_{j+1} are already known.
+static int run_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, double* part2, double* u_prev,
+                    double* u_cur, double* w, int j, hipStream_t s, EventTimer* tm, double* psi, double yj,
+                    bool reduce) {
   HvEpilogue e;
   e.xprev = j > 0 ? u_prev : nullptr;
   e.sx = c.scale + j;
@@ -528,10 +613,10 @@ static int run_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, double
     hipEvent_t b = tm->next();
     if (b) ED_HIP(hipEventRecord(b, s));
   }
-  if (reduce) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, np, (int)RED_ALPHA, c, j, nullptr);
   const int vb = vec_blocks(L.D);
-  launch_update(vb, s, w, u_cur, L.D, c.alpha + j, c.scale + j, psi, yj, reduce ? part : nullptr);
-  if (reduce) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_BETA, c, j, nullptr);
+  launch_update(vb, s, w, u_cur, L.D, c.alpha + j, c.scale + j, psi, yj, reduce ? part2 : nullptr,
+                reduce ? part : nullptr, np);
+  if (reduce) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part2, vb, (int)RED_BETA, c, j, nullptr);
   ED_HIP(hipGetLastError());
   return ED_OK;
 }
@@ -541,7 +626,9 @@ struct ReorthCtx {
   double* V;
   double* w;
   double* coefs;
+  double* part2;  // region B
   int64_t chunks, chunk;
+  int gx;         // k_multidot blocks along the chunks (dot partials per vector)
 };
 
 static ReorthCtx reorth_ctx(char* base, const LzLayout& L, double* part) {
@@ -549,17 +636,19 @@ static ReorthCtx reorth_ctx(char* base, const LzLayout& L, double* part) {
   r.V = vec_ptr(base, L, 0);
   r.w = vec_ptr(base, L, L.M + 1);
   r.coefs = part + L.npart;
+  r.part2 = reinterpret_cast<double*>(base + L.part2_off);
   r.chunks = std::min<int64_t>((L.D + 4095) / 4096, 256);
   r.chunk = (L.D + r.chunks - 1) / r.chunks;
+  r.gx = (int)std::min<int64_t>(r.chunks, kMultidotChunks);
   return r;
 }
 
 static int reorth_start(const LzLayout& L, LzCoef c, double* part, const ReorthCtx& r, const double* v0,
                         uint64_t seed, hipStream_t s) {
   const int vb = vec_blocks(L.D);
-  launch_start(L, s, r.w, v0, seed, part);
+  // V_0 = the start vector, unnormalised (v_0 = scale_0 V_0)
+  launch_start(L, s, r.V, v0, seed, part);
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_NORM0, c, 0, nullptr);
-  hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, r.V, r.w, L.D, c.scale, 0, nullptr);
   ED_HIP(hipGetLastError());
   return ED_OK;
 }
@@ -580,6 +669,9 @@ struct ProState {
   int M;
 };
 
+// The estimate's rows and flags in the workspace (LzLayout::pro_off), reset.
+static int pro_init(ProState& pro, char* base, const LzLayout& L, hipStream_t s);
+
 __global__ __launch_bounds__(256) void k_pro_init(ProState p) {
   for (int i = threadIdx.x; i < 3 * (p.M + 2); i += blockDim.x) p.om[i] = 0.0;
   if (threadIdx.x == 0) {
@@ -590,10 +682,13 @@ __global__ __launch_bounds__(256) void k_pro_init(ProState p) {
 }
 
 // bt = |w| before orthogonalisation, summed here from the np partials of
-// |w|^2 that the vector update left.
-__global__ __launch_bounds__(256) void k_pro_omega(ProState p, const double* __restrict__ alpha,
-                                                   const double* __restrict__ beta, int j,
-                                                   const double* __restrict__ wpart, int np) {
+// |w|^2 that the vector update left.  On a step that needs no Gram-Schmidt
+// this |w| is final: beta_{j+1} is set here (finish_beta), and the gated
+// kernels after it do nothing.  One block of 256 threads.
+__global__ __launch_bounds__(256) void k_pro_omega(ProState p, LzCoef c, int j, const double* __restrict__ wpart,
+                                                   int np) {
+  const double* __restrict__ alpha = c.alpha;
+  const double* __restrict__ beta = c.beta;
   __shared__ double sh[256 / 64];
   __shared__ int decide;
   __shared__ double bt_sh;
@@ -634,80 +729,71 @@ __global__ __launch_bounds__(256) void k_pro_omega(ProState p, const double* __r
     decide = re;
     nxt[j] = eps;
     nxt[j + 1] = 1.0;
+    if (!re) finish_beta(c, j, w2);
   }
   __syncthreads();
   if (decide)
     for (int k = threadIdx.x; k < j; k += blockDim.x) nxt[k] = eps;
 }
 
-// Owner of a ProState's device buffer: waits for the stream, then frees.
-struct ProBuffer {
-  double* p;
-  hipStream_t s;
-  ~ProBuffer() {
-    if (p) {
-      (void)hipStreamSynchronize(s);
-      (void)hipFree(p);
-    }
-  }
-};
-
-static int pro_alloc(ProState& pro, ProBuffer& buf, hipStream_t s) {
-  ED_HIP(hipMalloc(&buf.p, sizeof(double) * (3 * (size_t)(pro.M + 2) + 1)));
-  pro.om = buf.p;
-  pro.flags = reinterpret_cast<int*>(buf.p + 3 * (size_t)(pro.M + 2));
+static int pro_init(ProState& pro, char* base, const LzLayout& L, hipStream_t s) {
+  double* p = reinterpret_cast<double*>(base + L.pro_off);
+  pro.om = p;
+  pro.flags = reinterpret_cast<int*>(p + 3 * (size_t)(pro.M + 2));
   hipLaunchKernelGGL(k_pro_init, dim3(1), dim3(256), 0, s, pro);
   ED_HIP(hipGetLastError());
   return ED_OK;
 }
 
-// Step j: w = H v_j - beta_j v_{j-1}; alpha_j = <w, v_j>; w -= alpha_j v_j;
-// classical Gram-Schmidt against v_0..v_j: two passes on every step, or,
-// with a ProState, one pass where the orthogonality estimate asks for it
-// (the basis is then semi-orthogonal, |<v_i, v_k>| <= sqrt(eps), and w's
-// components along it are O(sqrt(eps) |w|), so one pass leaves O(eps));
-// beta_{j+1} = |w|; v_{j+1} = w / beta_{j+1}.
+// Step j, with the basis kept unnormalised (v_i = scale_i V_i, as the
+// three-vector recurrence keeps its vectors): W = V_{j+1} = s_j H V_j -
+// beta_j s_{j-1} V_{j-1}; alpha_j = s_j <W, V_j> (formed in the update);
+// W -= alpha_j s_j V_j; classical Gram-Schmidt against v_0..v_j: two passes on
+// every step, or, with a ProState, one pass where the orthogonality estimate
+// asks for it (the basis is then semi-orthogonal, |<v_i, v_k>| <= sqrt(eps),
+// and W's components along it are O(sqrt(eps) |W|), so one pass leaves
+// O(eps)); beta_{j+1} = |W|, scale_{j+1} = 1/beta_{j+1}.  Launches per step:
+// the two H.v sweeps, the update, then with a ProState the estimate (which
+// sets beta_{j+1} when no pass is due) and three gated kernels (dots,
+// combination, beta); without, two passes of (dots, combination) and beta.
 static int reorth_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, const ReorthCtx& r, int j,
                        hipStream_t s, EventTimer* tm, const ProState* pro = nullptr) {
   const int64_t D = L.D;
   const int vb = vec_blocks(D);
   double* vj = r.V + (int64_t)j * D;
+  double* w = r.V + (int64_t)(j + 1) * D;
   HvEpilogue e;
   e.xprev = j > 0 ? r.V + (int64_t)(j - 1) * D : nullptr;
-  e.sx = c.misc + 1;
-  e.cprev = c.beta + j;
+  e.sx = c.scale + j;
+  e.cprev = c.cprev + j;
   e.partial = part;
   if (tm && tm->on) {
     hipEvent_t a = tm->next();
     if (a) ED_HIP(hipEventRecord(a, s));
   }
   int np = 0;
-  ED_TRY(launch_matvec(op, vj, r.w, L.ld, e, s, &np));
+  ED_TRY(launch_matvec(op, vj, w, L.ld, e, s, &np));
   if (tm && tm->on) {
     hipEvent_t b = tm->next();
     if (b) ED_HIP(hipEventRecord(b, s));
   }
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, np, (int)RED_SUM_TO, c, j, c.alpha + j);
-  launch_update(vb, s, r.w, vj, D, c.alpha + j, c.misc + 1, nullptr, 0.0, pro ? part : nullptr);
+  launch_update(vb, s, w, vj, D, c.alpha + j, c.scale + j, nullptr, 0.0, r.part2, part, np);
   const int* gate = nullptr;
   if (pro) {
-    hipLaunchKernelGGL(k_pro_omega, dim3(1), dim3(256), 0, s, *pro, c.alpha, c.beta, j, part, vb);
+    hipLaunchKernelGGL(k_pro_omega, dim3(1), dim3(256), 0, s, *pro, c, j, r.part2, vb);
     gate = pro->flags;
   }
+  const int gb = std::min(vb, kAxpyBlocks);
   for (int pass = 0; pass < (pro ? 1 : 2); ++pass) {
-    hipLaunchKernelGGL(k_multidot, dim3((unsigned)r.chunks, (unsigned)std::min(j + 1, kMultidotRows)),
-                       dim3(kVecThreads), 0, s, r.V, D, r.w, r.chunk, part, (int)r.chunks, j + 1, gate);
-    hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)(j + 1)), dim3(kVecThreads), 0, s, part, (int)r.chunks, r.coefs,
-                       gate);
-    hipLaunchKernelGGL(k_axpy_multi, dim3(vb), dim3(kVecThreads), sizeof(double) * (j + 1), s, r.V, j + 1, D,
-                       r.coefs, r.w, 0, gate);
+    hipLaunchKernelGGL(k_multidot, dim3((unsigned)r.gx, (unsigned)std::min(j + 1, kMultidotRows)),
+                       dim3(kVecThreads), 0, s, r.V, D, w, r.chunk, part, (int)r.chunks, j + 1, gate);
+    hipLaunchKernelGGL(k_gs_update, dim3((unsigned)gb), dim3(kVecThreads), sizeof(double) * (j + 1), s, r.V, j + 1,
+                       D, part, r.gx, c.scale, w, r.part2, gate);
   }
-  // |w|^2 after the passes; on a step without them the update's partials
-  // (already in part) are those of the same w
-  hipLaunchKernelGGL(k_norm2, dim3(vb), dim3(kVecThreads), 0, s, r.w, D, part, gate);
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part, vb, (int)RED_BETA, c, j, nullptr);
-  hipLaunchKernelGGL(k_scale, dim3(vb), dim3(kVecThreads), 0, s, r.V + (int64_t)(j + 1) * D, r.w, D, c.scale + j + 1, 0,
-                     nullptr);
+  if (pro)
+    hipLaunchKernelGGL(k_beta_gated, dim3(1), dim3(kRedThreads), 0, s, r.part2, gb, c, j, gate);
+  else
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, r.part2, gb, (int)RED_BETA, c, j, nullptr);
   ED_HIP(hipGetLastError());
   return ED_OK;
 }
@@ -758,6 +844,8 @@ extern "C" int ed_lanczos_groundstate(ed_op* op, const ed_lanczos_cfg* cfg, cons
 
   hipLaunchKernelGGL(k_init_misc, dim3(1), dim3(64), 0, s, c.misc);
   EventTimer tm;
+  tm.on = cfg->time_matvec != 0;
+  double* part2 = reinterpret_cast<double*>(base + L.part2_off);
   std::vector<double> ha(M), hb(M + 1), y;
   int m = 0;
   bool converged = false, breakdown = false;
@@ -797,7 +885,7 @@ extern "C" int ed_lanczos_groundstate(ed_op* op, const ed_lanczos_cfg* cfg, cons
     int ip = 2, ic = 0, iw = 1;  // u_prev, u_cur, w
     int steps = 0;
     for (int j = 0; j < M; ++j) {
-      ED_TRY(run_step(op, L, c, part, u[ip], u[ic], u[iw], j, s, &tm, nullptr, 0.0, true));
+      ED_TRY(run_step(op, L, c, part, part2, u[ip], u[ic], u[iw], j, s, &tm, nullptr, 0.0, true));
       steps = j + 1;
       const int old_p = ip;
       ip = ic;
@@ -823,7 +911,7 @@ extern "C" int ed_lanczos_groundstate(ed_op* op, const ed_lanczos_cfg* cfg, cons
       iw = 1;
       for (int j = 0; j < m; ++j) {
         if (j + 1 < m) {
-          ED_TRY(run_step(op, L, c, part, u[ip], u[ic], u[iw], j, s, nullptr, psi, y[j], false));
+          ED_TRY(run_step(op, L, c, part, part2, u[ip], u[ic], u[iw], j, s, nullptr, psi, y[j], false));
         } else {
           // last vector: accumulate only (w is scratch)
           launch_update(vb, s, u[iw], u[ic], D, c.misc + 2, c.scale + j, psi, y[j], nullptr);
@@ -842,10 +930,7 @@ extern "C" int ed_lanczos_groundstate(ed_op* op, const ed_lanczos_cfg* cfg, cons
     double* coefs = rc.coefs;
     ED_TRY(reorth_start(L, c, part, rc, v0_dev, cfg->seed, s));
     ProState pro{nullptr, nullptr, M};
-    ProBuffer pro_buf{nullptr, s};
-    if (cfg->reorth == 2) {
-      ED_TRY(pro_alloc(pro, pro_buf, s));
-    }
+    if (cfg->reorth == 2) ED_TRY(pro_init(pro, base, L, s));
     int steps = 0;
     for (int j = 0; j < M; ++j) {
       ED_TRY(reorth_step(op, L, c, part, rc, j, s, &tm, cfg->reorth == 2 ? &pro : nullptr));
@@ -862,9 +947,13 @@ extern "C" int ed_lanczos_groundstate(ed_op* op, const ed_lanczos_cfg* cfg, cons
     }
     res->iterations = steps;
     if (cfg->want_vector) {
-      ED_HIP(hipMemcpyAsync(coefs, y.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+      // psi = sum_j y_j v_j = sum_j (y_j / beta_j) V_j  (beta_0 = |v0|)
+      std::vector<double> yc(m);
+      for (int k = 0; k < m; ++k) yc[k] = y[k] / hb[k];
+      ED_HIP(hipMemcpyAsync(coefs, yc.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
       hipLaunchKernelGGL(k_axpy_multi, dim3(vb), dim3(kVecThreads), sizeof(double) * m, s, V, m, D, coefs,
                          vec_ptr(base, L, M + 1), 1, nullptr);
+      ED_HIP(hipStreamSynchronize(s));  // yc goes out of scope
       ED_HIP(hipGetLastError());
     }
   }
@@ -935,11 +1024,10 @@ extern "C" int ed_lanczos_coeffs(ed_op* op, const double* v0_dev, int32_t nsteps
     const ReorthCtx rc = reorth_ctx(base, L, part);
     ED_TRY(reorth_start(L, c, part, rc, v0_dev, 0, s));
     ProState pro{nullptr, nullptr, M};
-    ProBuffer pro_buf{nullptr, s};
-    if (reorth == 2) ED_TRY(pro_alloc(pro, pro_buf, s));
+    if (reorth == 2) ED_TRY(pro_init(pro, base, L, s));
     for (int j = 0; j < M; ++j) ED_TRY(reorth_step(op, L, c, part, rc, j, s, nullptr, reorth == 2 ? &pro : nullptr));
-    // basis vector j is at rc.V + j D
-    if (basis_dev) launch_compact(L, s, basis_dev, rc.V, D, M, nullptr, 0);
+    // basis vector j is rc.V + j D, scaled by scale_j
+    if (basis_dev) launch_compact(L, s, basis_dev, rc.V, D, M, c.scale, 0);
   } else {
     double* u[3] = {vec_ptr(base, L, 0), vec_ptr(base, L, 1), vec_ptr(base, L, 2)};
     launch_start(L, s, u[0], v0_dev, 0, part);
@@ -948,7 +1036,8 @@ extern "C" int ed_lanczos_coeffs(ed_op* op, const double* v0_dev, int32_t nsteps
     int ip = 2, ic = 0, iw = 1;
     for (int j = 0; j < M; ++j) {
       if (basis_dev) launch_compact(L, s, basis_dev + (int64_t)j * L.nu * L.nd, u[ic], 0, 1, c.scale + j, 0);
-      ED_TRY(run_step(op, L, c, part, u[ip], u[ic], u[iw], j, s, nullptr, nullptr, 0.0, true));
+      ED_TRY(run_step(op, L, c, part, reinterpret_cast<double*>(base + L.part2_off), u[ip], u[ic], u[iw], j, s,
+                      nullptr, nullptr, 0.0, true));
       const int old_p = ip;
       ip = ic;
       ic = iw;

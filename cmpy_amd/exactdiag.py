@@ -77,7 +77,8 @@ def _operator(model_or_op, n_up=None, n_dn=None) -> SectorHamiltonOperator:
 def groundstate(model_or_op, n_up: int = None, n_dn: int = None, *, tol: float = 1e-10,
                 max_iter: int = 1000, min_iter: int = 0, check_every: int = 10,
                 reorth: Optional[bool] = None, want_vector: bool = True, fixed_iter: bool = False,
-                seed: int = 12345, v0=None, device: bool = False, return_stats: bool = False):
+                seed: int = 12345, v0=None, device: bool = False, return_stats: bool = False,
+                time_matvec: bool = False):
     """Lowest eigenpair of one sector by device Lanczos.
 
     Converged when the Ritz residual |H psi - E psi| <= tol (absolute, psi
@@ -85,7 +86,10 @@ def groundstate(model_or_op, n_up: int = None, n_dn: int = None, *, tol: float =
     to full reorthogonalisation up to 2e5 states, partial above while the
     Krylov basis fits in 16 GiB, and the three-vector recurrence beyond.
     Returns ``EigenState(energy, state, n_up, n_dn)``; ``state`` is a numpy array,
-    or a CUDA tensor when ``device=True``.
+    or a CUDA tensor when ``device=True``.  With ``return_stats`` also the
+    ``LanczosStats``, whose ``matvec_ms`` (the summed H.v time, from HIP events
+    that cost a few microseconds of GPU time per step) is measured only with
+    ``time_matvec=True`` and is 0 otherwise.
     """
     op = _operator(model_or_op, n_up, n_dn)
     D = op.size
@@ -99,7 +103,8 @@ def groundstate(model_or_op, n_up: int = None, n_dn: int = None, *, tol: float =
     else:
         mode = _reorth_mode(reorth, D, mi)
     cfg = _lib.LanczosCfg(mi, int(min_iter), int(check_every), mode, int(bool(want_vector)),
-                          int(bool(fixed_iter)), float(tol), int(seed) & 0xFFFFFFFFFFFFFFFF)
+                          int(bool(fixed_iter)), float(tol), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                          int(bool(time_matvec)))
     nbytes = C.c_int64()
     _lib.call("ed_lanczos_workspace", op.handle, C.byref(cfg), C.byref(nbytes))
     work = torch.empty(nbytes.value, dtype=torch.uint8, device="cuda")

@@ -185,6 +185,10 @@ typedef struct ed_lanczos_cfg {
   int32_t fixed_iter;   /* 1: run exactly max_iter steps (no early stop)      */
   double tol;           /* stop when the Ritz residual |H psi - E psi| <= tol  */
   uint64_t seed;        /* start vector seed (used when v0_dev is NULL)       */
+  int32_t time_matvec;  /* 1: time every H.v sweep with HIP events into
+                           matvec_ms (each pair of events costs a few us of
+                           GPU time per step); 0: matvec_ms = 0               */
+  int32_t pad;
 } ed_lanczos_cfg;
 
 typedef struct ed_lanczos_result {
@@ -195,7 +199,7 @@ typedef struct ed_lanczos_result {
   int32_t converged;
   int32_t breakdown;     /* invariant subspace reached                         */
   int32_t pad;
-  double matvec_ms;      /* summed H.v kernel time (HIP events)                */
+  double matvec_ms;      /* summed H.v kernel time (HIP events; time_matvec)   */
   double total_ms;       /* wall time of the call                              */
 } ed_lanczos_result;
 

@@ -477,3 +477,42 @@ def test_groundstate_padded_rows_equal_contiguous(gpu, monkeypatch, reorth):
     assert abs(g1.energy - g0.energy) <= 1e-12 * abs(g0.energy)
     assert abs(abs(np.dot(g1.state, g0.state)) - 1.0) <= 1e-12
     assert np.abs(c1.gf - c0.gf).max() <= 1e-10
+
+
+def test_map_streams_matches_sequential(gpu):
+    """Three independent chain solves (ground state + continued fraction) on
+    three streams give the same bits as one after another: the same kernels
+    in the same order on each stream, with fixed-order reductions.  The
+    10-site (5,5) sector (D = 63,504) takes the partial-reorthogonalisation
+    fraction, so the gated kernels and the in-workspace estimate run."""
+    from cmpy_amd import HubbardModel, chain_neighbors, greens_cf, groundstate, map_streams
+
+    z = np.linspace(-6, 6, 96) + 0.1j
+    models = [HubbardModel(10, chain_neighbors(10), inter=u, hop=1.0, mu=u / 2) for u in (1.0, 3.0, 6.0)]
+
+    def solve(m):
+        gs = groundstate(m, 5, 5, tol=1e-10, device=True)
+        r = greens_cf(m, z, 5, 5, nsteps=120, gs=gs)
+        return gs.energy, r.gf, r.reorth_p
+
+    seq = [solve(m) for m in models]
+    conc = map_streams(solve, models)
+    again = map_streams(solve, models, num_streams=2)
+    for (e1, g1, mode), (e2, g2, _), (e3, g3, _) in zip(seq, conc, again):
+        assert mode == "partial"
+        assert e1 == e2 == e3
+        assert np.array_equal(g1, g2) and np.array_equal(g1, g3)
+
+
+def test_map_streams_raises_worker_error(gpu):
+    from cmpy_amd import map_streams
+
+    def f(x):
+        if x == 2:
+            raise ValueError("item 2")
+        return torch.full((4,), float(x), device="cuda")
+
+    assert [float(t[0]) for t in map_streams(f, [0, 1, 3])] == [0.0, 1.0, 3.0]
+    with pytest.raises(ValueError, match="item 2"):
+        map_streams(f, [0, 1, 2, 3])
+    assert map_streams(f, []) == []
