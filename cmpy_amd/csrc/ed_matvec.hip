@@ -1238,16 +1238,24 @@ static ColKernel col_kernel_split(int tw, int maxl) {
 
 constexpr int64_t kMinRowBlocks = 512;
 
+// Below this many states the single fused pass (k_hv) is the faster sweep:
+// its rows gather their up-hop sources straight from x, which at this size
+// stays in the caches, and it is one launch instead of two.  Measured
+// back-to-back on MI355X: 12-site chain (6,6), D = 853,776: 19.0 us fused
+// against 26.5 us in two passes; 14-site chain (7,7), D = 11.8 M: 208 against
+// 177 us; 4x4 (5,5), D = 19.1 M: 1063 against 307 us (break-even near 3 M).
+constexpr int64_t kTwoPassMinStates = int64_t(1) << 21;
+
 // Two-pass plan.  Kernels: sign-split tables when hopping is uniform and the
 // tables exist (CMPY_AMD_HV_SPLIT=0 turns them off), general packed tables
 // otherwise.  CMPY_AMD_HV_MODE: 0 single fused pass, 1 two passes, unset =
-// two passes from D = 65536 up.
+// two passes from kTwoPassMinStates up.
 static TwoPassPlan plan_two_pass(const ed_op* op, int64_t ld) {
   TwoPassPlan p{};
   p.ok = false;
   const int mode = op->hv_mode >= 0 ? op->hv_mode : env_int("CMPY_AMD_HV_MODE", -1);
   if (!op->packed || mode == 0) return p;
-  if (mode < 0 && op->size < 65536) return p;
+  if (mode < 0 && op->size < kTwoPassMinStates) return p;
   const int maxh_up = maxh_for(op->max_hops_up);
   if (!maxh_up || !maxh_for(op->max_hops_dn)) return p;
   bool uni = op->num_bonds > 0;

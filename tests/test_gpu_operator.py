@@ -272,16 +272,22 @@ def test_hv_4x4_matches_cpu_twin(gpu):
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
 
 
-def test_padded_rows_default_stride(gpu):
+def test_padded_rows_default_stride(gpu, monkeypatch):
     """The library's own vector stride: 128-byte rows where the two-pass sweep
-    runs (num_dn not a multiple of 16), contiguous otherwise."""
+    runs (num_dn not a multiple of 16), contiguous otherwise.  The 12-site
+    chain's (6,6) sector (D = 853,776) takes the fused single pass by default
+    (below 2^21 states) and the two passes with CMPY_AMD_HV_MODE=1."""
     from cmpy_amd import SectorHamiltonOperator
 
     ns, bonds = 12, O.chain_bonds(12)
-    big = SectorHamiltonOperator(ns, 6, 6, bonds, [1.0] * 11, np.zeros(ns), np.full(ns, 4.0), ns)
-    assert big.num_dn == 924 and big.vector_ld == 928
+    args = (ns, 6, 6, bonds, [1.0] * 11, np.zeros(ns), np.full(ns, 4.0), ns)
+    fused = SectorHamiltonOperator(*args)
+    assert fused.num_dn == 924 and fused.vector_ld == 924
     small = SectorHamiltonOperator(6, 3, 3, O.chain_bonds(6), [1.0] * 5, np.zeros(6), np.full(6, 4.0), 6)
     assert small.vector_ld == small.num_dn
+    monkeypatch.setenv("CMPY_AMD_HV_MODE", "1")
+    big = SectorHamiltonOperator(*args)
+    assert big.num_dn == 924 and big.vector_ld == 928
     from cmpy_amd import _lib as L
 
     x = torch.zeros((big.num_up, 928), dtype=torch.float64, device="cuda")

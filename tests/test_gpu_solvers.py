@@ -460,15 +460,18 @@ def test_greens_cf_records_and_warns_reorth_fallback(gpu, monkeypatch):
 def test_groundstate_padded_rows_equal_contiguous(gpu, monkeypatch, reorth):
     """The solver's vectors in 128-byte rows (the default where the two-pass
     sweep runs) give the ground state of the contiguous layout: 12-site chain,
-    (6,6), rows of 924 states padded to 928."""
+    (6,6), rows of 924 states padded to 928, with the two-pass sweep forced
+    (CMPY_AMD_HV_MODE=1; this size takes the fused pass by default)."""
     from cmpy_amd import HubbardModel, chain_neighbors, greens_cf, groundstate
 
+    monkeypatch.setenv("CMPY_AMD_HV_MODE", "1")
     model = HubbardModel(12, chain_neighbors(12), inter=4.0, hop=1.0, mu=2.0)
     z = np.linspace(-6, 6, 64) + 1.0j
     out = {}
     for pitch in ("1", "0"):
         monkeypatch.setenv("CMPY_AMD_HV_PITCH", pitch)
         model.clear_operator_cache()
+        assert model.hamilton_operator(n_up=6, n_dn=6).vector_ld == (928 if pitch == "1" else 924)
         gs, st = groundstate(model, 6, 6, tol=1e-10, reorth=reorth, return_stats=True)
         cf = greens_cf(model, z, 6, 6, nsteps=60, gs=gs, reorth="full")
         out[pitch] = (gs, st, cf)
