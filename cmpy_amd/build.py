@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -35,18 +36,45 @@ CFLAGS = [
 ]
 LDFLAGS = ["-shared", f"-L{ROCM}/lib", "-lrocsolver", "-lrocblas"]
 
+# Largest private (scratch) size per lane a kernel may compile to.  The
+# persistent batch kernels (1024-thread workgroups, one per CU slot) run at
+# 240-304 bytes/lane; two builds that spilled to ~1.37 KB/lane (164-166
+# spilled VGPRs) faulted on MI355X with a memory-aperture violation, while the
+# same code run on the CPU under AddressSanitizer (tests/cpu_kernels) stays in
+# bounds.  A build past this limit is refused rather than shipped.
+SCRATCH_LIMIT = 512
+_REMARK = re.compile(r"remark:\s+(Function Name: (\S+)|ScratchSize \[bytes/lane\]: (\d+))")
+
+
+def kernel_scratch(remarks: str) -> dict:
+    """Per-kernel ScratchSize (bytes/lane) from -Rpass-analysis=kernel-resource-usage output."""
+    out, name = {}, None
+    for m in _REMARK.finditer(remarks):
+        if m.group(2):
+            name = m.group(2)
+        elif name is not None:
+            out[name] = int(m.group(3))
+            name = None
+    return out
+
 
 def _compile(src: Path, obj: Path, verbose: bool) -> None:
-    cmd = [HIPCC, *CFLAGS, "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *CFLAGS, "-Rpass-analysis=kernel-resource-usage", "-c", str(src), "-o", str(obj)]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
+    out = r.stdout + r.stderr
     err = "\n".join(
-        line for line in (r.stdout + r.stderr).splitlines()
-        if "packed-fp32-ops" not in line
+        line for line in out.splitlines()
+        if "packed-fp32-ops" not in line and "kernel-resource-usage" not in line
     )
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{err}")
+    over = {k: v for k, v in kernel_scratch(out).items() if v > SCRATCH_LIMIT}
+    if over:
+        obj.unlink(missing_ok=True)
+        raise RuntimeError(f"{src.name}: kernels over the {SCRATCH_LIMIT} bytes/lane scratch limit "
+                           f"(register spills): {over}")
     if verbose and err.strip():
         print(err, flush=True)
 
