@@ -75,6 +75,13 @@ class LanczosResult(C.Structure):
     ]
 
 
+class CfParts(C.Structure):
+    _fields_ = [
+        ("alpha", C.POINTER(C.c_double) * 2), ("beta", C.POINTER(C.c_double) * 2),
+        ("norm2", C.c_double * 2), ("steps", C.c_int32 * 2),
+    ]
+
+
 class BatchDesc(C.Structure):
     _fields_ = [
         ("num_models", C.c_int64), ("num_bath", C.POINTER(C.c_int32)),
@@ -123,6 +130,10 @@ SIGNATURES = {
     "ed_cf_eval": (C.c_int, [P(dbl), P(dbl), i32, dbl, dbl, i32, vp, i64, vp, i32, vp]),
     "ed_tevo_eval": (C.c_int, [P(dbl), P(dbl), i32, dbl, dbl, i32, vp, i64, vp, i32, vp]),
     "ed_pole_sum": (C.c_int, [vp, vp, i64, vp, i64, vp, i32, vp]),
+    "ed_greens_cf_workspace": (C.c_int, [vp, vp, i32, i32, P(i64)]),
+    "ed_greens_cf": (C.c_int, [vp, vp] + [i32] * 6 + [vp, dbl, i32, i32, vp, i64, vp, P(CfParts), vp, i64, vp]),
+    "ed_greens_lehmann_t0_workspace": (C.c_int, [vp, vp, P(i64)]),
+    "ed_greens_lehmann_t0": (C.c_int, [vp, vp] + [i32] * 6 + [vp, dbl, vp, i64, vp, vp, i64, vp]),
     "ed_lehmann_pairs": (C.c_int, [vp, vp, i64, vp, i64, dbl, dbl, vp, i64, vp, vp]),
     "ed_lehmann_pairs_ij": (C.c_int, [vp, vp, vp, i64, vp, i64, dbl, dbl, vp, i64, vp, vp]),
     "ed_lehmann_pairs_noalloc": (C.c_int, [vp, vp, i64, vp, i64, dbl, dbl, vp, i64, vp, vp]),

@@ -21,10 +21,15 @@
 // the kernel's end.  All vectors and spin tables of a group live in its slot
 // of a global workspace (L1/L2-resident for these sizes); Lanczos scalars are
 // reduced in a fixed order, so results are deterministic for a given
-// configuration.
+// configuration.  The ground-state Lanczos vectors of every model are kept
+// in a store of their own ((max_iter + 1) D doubles per model), so psi is
+// one combination of them instead of a second Lanczos pass; a batch whose
+// store would not fit in 60% of the free device memory is solved in chunks
+// of models, one launch each.
 #include "ed_internal.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstring>
 #include <numeric>
@@ -79,6 +84,8 @@ struct Config {
   double* store_psi;
   double* store_q;
   double* store_coef;      // [num_models][4 * kcap + 8]
+  double* gs_store;        // ground-state Krylov vectors, (max_iter + 1) D per model
+  const int64_t* off_gs;   // [num_models]
 };
 
 // ---------------------------------------------------------------------------
@@ -667,30 +674,26 @@ __device__ void solve_model(const Group<BLOCK>& g, const Config& cf, int mi, con
   double* const beta = sl.beta();
   double* const psi = sl.vec(3);
 
-  // ---- ground state, pass 1: coefficients and convergence ----
+  // ---- ground state: every Lanczos vector kept (u_j = V_j, v_j = s_j u_j) ----
   const uint64_t seed = cf.seed ^ (0xA0761D6478BD642Full * (uint64_t)(mi + 1));
+  double* const V = cf.gs_store + cf.off_gs[mi];
   double n2 = 0.0;
-  {
-    double* u0 = sl.vec(1);
-    for (int64_t k = g.rank; k < D; k += g.size) {
-      const double r = start_value(seed, k);
-      u0[k] = r;
-      n2 += r * r;
-    }
+  for (int64_t k = g.rank; k < D; k += g.size) {
+    const double r = start_value(seed, k);
+    V[k] = r;
+    n2 += r * r;
   }
   n2 = g.sum(n2);
   g.sync();
   const double s0 = 1.0 / sqrt(n2);
-  double* u = sl.vec(1);
-  double* up = sl.vec(0);
-  double* w = sl.vec(2);
   double sj = s0, sp = 0.0, bj = 0.0;
   int m = 0;
   double theta = 0.0, slast = 0.0, ritz = 1e300;
   const int cap = (int)((int64_t)cf.max_iter < D ? (int64_t)cf.max_iter : D);
   for (int j = 0; j < cap; ++j) {
     double aj, nn;
-    lanczos_step<BLOCK>(g, p, U, Dn, u, up, w, sj, sp, bj, false, 0.0, &aj, &nn);
+    const double* u = V + (int64_t)j * D;
+    lanczos_step<BLOCK>(g, p, U, Dn, u, j > 0 ? u - D : u, V + (int64_t)(j + 1) * D, sj, sp, bj, false, 0.0, &aj, &nn);
     m = j + 1;
     const double bn = sqrt(nn);
     if (g.rank == 0) {
@@ -705,42 +708,19 @@ __device__ void solve_model(const Group<BLOCK>& g, const Config& cf, int mi, con
       ritz = breakdown ? 0.0 : bn * slast;
       if (ritz <= cf.tol || breakdown) break;
     }
-    double* t = up;
-    up = u;
-    u = w;
-    w = t;
     sp = sj;
     sj = 1.0 / bn;
     bj = bn;
   }
-  // ---- pass 2: psi = sum_j s_j v_j, regenerated with the stored coefficients ----
-  u = sl.vec(1);
-  up = sl.vec(0);
-  w = sl.vec(2);
-  for (int64_t k = g.rank; k < D; k += g.size) {
-    u[k] = start_value(seed, k);
-    psi[k] = 0.0;
-  }
-  g.sync();
-  sj = s0;
-  sp = 0.0;
-  bj = 0.0;
+  // psi = sum_j sv_j s_j V_j, in step order
   {
-    const double* sv = sl.sv();
-    for (int j = 0; j < m; ++j) {
-      const double cj = sv[j] * sj;
-      for (int64_t k = g.rank; k < D; k += g.size) psi[k] += cj * u[k];
-      if (j == m - 1) break;
-      double aj, nn;
-      lanczos_step<BLOCK>(g, p, U, Dn, u, up, w, sj, sp, bj, true, alpha[j], &aj, &nn);
-      const double bn = beta[j];
-      double* t = up;
-      up = u;
-      u = w;
-      w = t;
-      sp = sj;
-      sj = 1.0 / bn;
-      bj = bn;
+    double* const c = sl.tc();
+    for (int j = g.rank; j < m; j += g.size) c[j] = sl.sv()[j] * (j == 0 ? s0 : 1.0 / beta[j - 1]);
+    g.sync();
+    for (int64_t k = g.rank; k < D; k += g.size) {
+      double acc = 0.0;
+      for (int j = 0; j < m; ++j) acc += c[j] * V[(int64_t)j * D + k];
+      psi[k] = acc;
     }
   }
   g.sync();
@@ -1433,6 +1413,12 @@ struct ed_batch {
   bool keep = false;
   int32_t kcap = 0;
   int64_t *d_off_psi = nullptr, *d_off_q = nullptr;
+  // ground-state vector store: per-model offsets (within the model's chunk),
+  // chunks of models in solve order (first model, count)
+  double* d_gs_store = nullptr;
+  int64_t* d_off_gs = nullptr;
+  std::vector<int64_t> gs_need;                 // doubles per model
+  std::vector<std::pair<int64_t, int64_t>> chunks;
   double *d_store_psi = nullptr, *d_store_q = nullptr, *d_store_coef = nullptr;
   double *d_gre = nullptr, *d_gim = nullptr, *d_grad_eps = nullptr, *d_grad_v = nullptr;
   int32_t* d_cg = nullptr;
@@ -1448,6 +1434,8 @@ struct ed_batch {
   bool tuned = false;
   double tune_ms = 0.0;
   Config cfg{};
+  // per chunk: wave and block queue bounds in d_queues
+  std::vector<std::array<int32_t, 4>> chunk_q;
   // captured graph of (reset counters, solve) for repeated solves
   hipGraphExec_t graph = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -1464,7 +1452,7 @@ static void free_batch(ed_batch* b) {
                   (void*)b->d_resid, (void*)b->d_wn, (void*)b->d_iters, (void*)b->d_ws, (void*)b->d_off_psi,
                   (void*)b->d_off_q, (void*)b->d_store_psi, (void*)b->d_store_q, (void*)b->d_store_coef,
                   (void*)b->d_gre, (void*)b->d_gim, (void*)b->d_grad_eps, (void*)b->d_grad_v, (void*)b->d_cg,
-                  (void*)b->d_bcounter, (void*)b->d_bws})
+                  (void*)b->d_bcounter, (void*)b->d_bws, (void*)b->d_gs_store, (void*)b->d_off_gs})
     if (p) (void)hipFree(p);
   delete b;
 }
@@ -1488,6 +1476,21 @@ static int configure(ed_batch* b, int64_t wave_max_dim, int grid_blocks, int wav
   std::vector<int32_t> qw, qb;
   int64_t vw = 1, vb = 1;
   int32_t lw = 1, lb = 1;
+  // queues of every chunk, [wave | block] one chunk after another
+  std::vector<int32_t> q;
+  b->chunk_q.clear();
+  for (const auto& ch : b->chunks) {
+    std::array<int32_t, 4> cq{};
+    cq[0] = (int32_t)q.size();
+    for (int64_t k = ch.first; k < ch.first + ch.second; ++k)
+      if (b->dim[b->order[k]] <= wave_max_dim) q.push_back(b->order[k]);
+    cq[1] = (int32_t)q.size() - cq[0];
+    cq[2] = (int32_t)q.size();
+    for (int64_t k = ch.first; k < ch.first + ch.second; ++k)
+      if (b->dim[b->order[k]] > wave_max_dim) q.push_back(b->order[k]);
+    cq[3] = (int32_t)q.size() - cq[2];
+    b->chunk_q.push_back(cq);
+  }
   for (int32_t mi : b->order) {
     const ModelParams& p = b->params[mi];
     const int ns = p.nb + 1;
@@ -1520,17 +1523,15 @@ static int configure(ed_batch* b, int64_t wave_max_dim, int grid_blocks, int wav
     ED_TRY(balloc(&b->d_ws, bytes));
     b->ws_bytes = bytes;
   }
-  std::vector<int32_t> q(qw);
-  q.insert(q.end(), qb.begin(), qb.end());
   ED_HIP(hipMemcpyAsync(b->d_queues, q.data(), sizeof(int32_t) * q.size(), hipMemcpyHostToDevice, s));
   ED_HIP(hipStreamSynchronize(s));
   Config& c = b->cfg;
   c.num_models = b->num_models;
   c.params = b->d_params;
-  c.queue_wave = b->d_queues;
-  c.queue_block = b->d_queues + qw.size();
-  c.n_wave = (int32_t)qw.size();
-  c.n_block = (int32_t)qb.size();
+  c.queue_wave = b->d_queues + b->chunk_q[0][0];
+  c.n_wave = b->chunk_q[0][1];
+  c.queue_block = b->d_queues + b->chunk_q[0][2];
+  c.n_block = b->chunk_q[0][3];
   c.counters = b->d_counters;
   c.wave_blocks = wave_blocks;
   c.ws_wave = b->d_ws;
@@ -1560,6 +1561,8 @@ static int configure(ed_batch* b, int64_t wave_max_dim, int grid_blocks, int wav
   c.store_psi = b->d_store_psi;
   c.store_q = b->d_store_q;
   c.store_coef = b->d_store_coef;
+  c.gs_store = b->d_gs_store;
+  c.off_gs = b->d_off_gs;
   b->wave_max_dim = wave_max_dim;
   b->grid = grid_blocks;
   b->wave_blocks = wave_blocks;
@@ -1570,9 +1573,17 @@ static int configure(ed_batch* b, int64_t wave_max_dim, int grid_blocks, int wav
   return ED_OK;
 }
 
+// One persistent launch per chunk of models (one in all but very large batches).
 static int launch(ed_batch* b, hipStream_t s) {
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, b->d_counters);
-  hipLaunchKernelGGL(k_batch_solve, dim3((unsigned)b->grid), dim3(kBlock), 0, s, b->cfg);
+  for (const auto& cq : b->chunk_q) {
+    Config c = b->cfg;
+    c.queue_wave = b->d_queues + cq[0];
+    c.n_wave = cq[1];
+    c.queue_block = b->d_queues + cq[2];
+    c.n_block = cq[3];
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, b->d_counters);
+    hipLaunchKernelGGL(k_batch_solve, dim3((unsigned)b->grid), dim3(kBlock), 0, s, c);
+  }
   ED_HIP(hipGetLastError());
   return ED_OK;
 }
@@ -1755,6 +1766,45 @@ extern "C" int ed_batch_create(ed_batch** out, const ed_batch_desc* d, void* str
     }
     if (hipMemcpyAsync(b->d_off_psi, off_psi.data(), sizeof(int64_t) * M, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(b->d_off_q, off_q.data(), sizeof(int64_t) * 2 * M, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      free_batch(b);
+      return set_error(ED_ERR_HIP, "upload of store offsets failed");
+    }
+  }
+  // ground-state vector store: (min(max_iter, D) + 1) D doubles per model,
+  // chunks of models (in solve order) within 60% of the free device memory
+  {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)16 << 30;
+    const int64_t budget = (int64_t)(0.6 * (double)free_b / sizeof(double));
+    const int64_t M = d->num_models;
+    b->gs_need.resize(M);
+    for (int64_t i = 0; i < M; ++i) b->gs_need[i] = (std::min<int64_t>(b->max_iter, b->dim[i]) + 1) * b->dim[i];
+    std::vector<int64_t> off(M, 0);
+    int64_t first = 0, used = 0, largest = 0;
+    for (int64_t k = 0; k < M; ++k) {
+      const int64_t need = b->gs_need[b->order[k]];
+      if (need > budget) {
+        free_batch(b);
+        return set_error(ED_ERR_NOMEM, "model %lld: ground-state store of %lld bytes exceeds the free device memory",
+                         (long long)b->order[k], (long long)(need * 8));
+      }
+      if (used + need > budget && k > first) {
+        b->chunks.emplace_back(first, k - first);
+        largest = std::max(largest, used);
+        first = k;
+        used = 0;
+      }
+      off[b->order[k]] = used;
+      used += need;
+    }
+    b->chunks.emplace_back(first, M - first);
+    largest = std::max(largest, used);
+    if ((rc = balloc(&b->d_gs_store, largest)) != ED_OK || (rc = balloc(&b->d_off_gs, M)) != ED_OK) {
+      free_batch(b);
+      return rc;
+    }
+    if (hipMemcpyAsync(b->d_off_gs, off.data(), sizeof(int64_t) * M, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
       free_batch(b);
       return set_error(ED_ERR_HIP, "upload of store offsets failed");

@@ -586,12 +586,19 @@ def _z_device(z: np.ndarray) -> torch.Tensor:
     return torch.from_numpy(np.ascontiguousarray(np.stack([z.real, z.imag], axis=1).ravel())).cuda()
 
 
-def _ladder_device(psi: torch.Tensor, ns, n_up, n_dn, pos, sigma, create, sign_mode):
+def _ladder_sector(ns, n_up, n_dn, sigma, create):
+    """Sector one particle above (create) or below in spin sigma, or None."""
     d = 1 if create else -1
     nu = n_up + (d if sigma == UP else 0)
     nd = n_dn + (d if sigma == DN else 0)
-    if not (0 <= nu <= ns and 0 <= nd <= ns):
+    return (nu, nd) if (0 <= nu <= ns and 0 <= nd <= ns) else None
+
+
+def _ladder_device(psi: torch.Tensor, ns, n_up, n_dn, pos, sigma, create, sign_mode):
+    sec = _ladder_sector(ns, n_up, n_dn, sigma, create)
+    if sec is None:
         return None, None
+    nu, nd = sec
     lib = _lib.lib()
     size = int(lib.ed_binom(ns, nu)) * int(lib.ed_binom(ns, nd))
     y = torch.empty(size, dtype=torch.float64, device="cuda")
@@ -626,20 +633,18 @@ def greens_lehmann_t0(model, z, n_up, n_dn, pos=0, sigma=UP, gs: EigenState = No
     sign_mode = _sign_mode(model, sign_mode)
     e0, psi = _gs(model, n_up, n_dn, gs, tol)
     zt = _z_device(z)
-    g = torch.zeros(zt.numel(), dtype=torch.float64, device="cuda")
+    ops = []
     for create in (True, False):
-        phi, sec = _ladder_device(psi, ns, n_up, n_dn, pos, sigma, create, sign_mode)
-        if phi is None:
-            continue
-        op = model.hamilton_operator(n_up=sec[0], n_dn=sec[1])
-        w, V = _dense_eigh_device(op)
-        M = V.shape[0]
-        ov = torch.empty(M, dtype=torch.float64, device="cuda")
-        _lib.call("ed_gemm_nt", _lib.dptr(V), _lib.dptr(phi), _lib.dptr(ov), M, 1, op.size, _lib.stream_ptr())
-        weights = ov * ov
-        poles = (w - e0) if create else (e0 - w)
-        _lib.call("ed_pole_sum", _lib.dptr(weights), _lib.dptr(poles.contiguous()), M, _lib.dptr(zt),
-                  zt.numel() // 2, _lib.dptr(g), 1, _lib.stream_ptr())
+        sec = _ladder_sector(ns, n_up, n_dn, sigma, create)
+        ops.append(model.hamilton_operator(n_up=sec[0], n_dn=sec[1]) if sec is not None else None)
+    hp, hm = (op.handle if op is not None else None for op in ops)
+    nbytes = C.c_int64()
+    _lib.call("ed_greens_lehmann_t0_workspace", hp, hm, C.byref(nbytes))
+    work = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device="cuda")
+    g = torch.empty(zt.numel(), dtype=torch.float64, device="cuda")
+    _lib.call("ed_greens_lehmann_t0", hp, hm, ns, n_up, n_dn, pos, sigma, SIGN_MODES[sign_mode], _lib.dptr(psi),
+              float(e0), _lib.dptr(zt), zt.numel() // 2, _lib.dptr(g), _lib.dptr(work), nbytes.value,
+              _lib.stream_ptr())
     gh = g.view(-1, 2).cpu().numpy()
     return (gh[:, 0] + 1j * gh[:, 1]).reshape(np.shape(z))
 
@@ -735,22 +740,36 @@ def greens_cf(model, z, n_up, n_dn, pos=0, sigma=UP, nsteps=200, gs: EigenState 
     sign_mode = _sign_mode(model, sign_mode)
     e0, psi = _gs(model, n_up, n_dn, gs, tol)
     zt = _z_device(z)
+    nz = zt.numel() // 2
     g = torch.zeros(zt.numel(), dtype=torch.float64, device="cuda")
     coeffs, modes = {}, {}
-    for create in (True, False):
-        phi, sec = _ladder_device(psi, ns, n_up, n_dn, pos, sigma, create, sign_mode)
-        if phi is None:
+    # one ed_greens_cf call per part (the other operator NULL), so that each
+    # part gets the reorthogonalisation mode its own sector size calls for
+    for k, create in enumerate((True, False)):
+        sec = _ladder_sector(ns, n_up, n_dn, sigma, create)
+        if sec is None:
             coeffs[create] = (np.zeros(0), np.zeros(0), 0.0)
             continue
         op = model.hamilton_operator(n_up=sec[0], n_dn=sec[1])
-        used = []
-        a, b, n2 = _cf_coeffs(op, phi, nsteps, reorth, used)
-        coeffs[create] = (a, b, n2)
-        modes[create] = used[0]
-        if len(a):
-            _lib.call("ed_cf_eval", a.ctypes.data_as(C.POINTER(C.c_double)),
-                      (b if len(b) else np.zeros(1)).ctypes.data_as(C.POINTER(C.c_double)), len(a), n2, e0,
-                      1 if create else -1, _lib.dptr(zt), zt.numel() // 2, _lib.dptr(g), 1, _lib.stream_ptr())
+        steps = int(max(1, min(nsteps, op.size)))
+        mode = _reorth_mode(reorth, op.size, steps, warn=True)
+        modes[create] = _MODE_NAMES[mode]
+        hp, hm = (op.handle, None) if create else (None, op.handle)
+        nbytes = C.c_int64()
+        _lib.call("ed_greens_cf_workspace", hp, hm, steps, mode, C.byref(nbytes))
+        work = torch.empty(nbytes.value, dtype=torch.uint8, device="cuda")
+        a = np.zeros(steps)
+        b = np.zeros(steps)
+        parts = _lib.CfParts()
+        parts.alpha[k] = a.ctypes.data_as(C.POINTER(C.c_double))
+        parts.beta[k] = b.ctypes.data_as(C.POINTER(C.c_double))
+        gk = torch.empty(zt.numel(), dtype=torch.float64, device="cuda")
+        _lib.call("ed_greens_cf", hp, hm, ns, n_up, n_dn, pos, sigma, SIGN_MODES[sign_mode], _lib.dptr(psi),
+                  float(e0), steps, mode, _lib.dptr(zt), nz, _lib.dptr(gk), C.byref(parts), _lib.dptr(work),
+                  nbytes.value, _lib.stream_ptr())
+        m = int(parts.steps[k])
+        coeffs[create] = (a[:m].copy(), b[: max(m - 1, 0)].copy(), float(parts.norm2[k]))
+        g += gk
     gh = g.view(-1, 2).cpu().numpy()
     gf = (gh[:, 0] + 1j * gh[:, 1]).reshape(np.shape(z))
     ap, bp, np_ = coeffs[True]

@@ -254,6 +254,43 @@ int ed_tevo_eval(const double* alpha, const double* beta, int32_t m, double norm
                  int32_t sign, const double* t_dev, int64_t nt, double* g_dev, int32_t accumulate,
                  void* stream);
 
+/* T = 0 G(z) of c = c_{pos,sigma} in one call, from the ground state psi0
+ * (device, contiguous) of sector (n_up, n_dn) of num_sites sites, energy e0:
+ *   G(z) = <psi0| c (z + e0 - H)^-1 c+ |psi0> + <psi0| c+ (z - e0 + H)^-1 c |psi0>,
+ * the T = 0 limit of the sum accumulated by accumulate_gf / _accumulate_sum
+ * (cmpy/exactdiag.py:110-134).  op_p / op_m: operators of the sectors one
+ * particle above / below in spin sigma, NULL where that sector does not
+ * exist (its part is then 0).  z_dev, g_dev: [nz] complex interleaved; g is
+ * overwritten.  sign_mode: ED_SIGN_* of the ladder operators.               */
+
+/* Continued fractions: particle and hole Lanczos runs of nsteps each (capped
+ * at the sector size; reorth as ed_lanczos_coeffs) started at c+ psi0 and
+ * c psi0 -- the recurrence of iter_lanczos_coeffs (cmpy/exactdiag.py:324-348)
+ * -- evaluated as ed_cf_eval does.  parts (optional) receives per part
+ * (0 particle, 1 hole) the step count, |phi|^2 and, where the caller gives
+ * host arrays of nsteps doubles, alpha_0.. and beta_1.. .                   */
+typedef struct ed_cf_parts {
+  double* alpha[2];   /* in: host [nsteps] or NULL                         */
+  double* beta[2];    /* in: host [nsteps] or NULL (steps - 1 written)      */
+  double norm2[2];    /* out: |c+ psi0|^2, |c psi0|^2                        */
+  int32_t steps[2];   /* out: Lanczos steps (0: empty part)                 */
+} ed_cf_parts;
+
+int ed_greens_cf_workspace(const ed_op* op_p, const ed_op* op_m, int32_t nsteps, int32_t reorth, int64_t* bytes);
+int ed_greens_cf(ed_op* op_p, ed_op* op_m, int32_t num_sites, int32_t n_up, int32_t n_dn, int32_t pos,
+                 int32_t sigma, int32_t sign_mode, const double* psi_dev, double e0, int32_t nsteps, int32_t reorth,
+                 const double* z_dev, int64_t nz, double* g_dev, ed_cf_parts* parts, void* work_dev,
+                 int64_t work_bytes, void* stream);
+
+/* Dense Lehmann sums: both neighbour sectors diagonalised on the device
+ * (ed_op_dense + ed_dense_eigh, at most 65,536 states), overlaps <m|c+|psi0>
+ * and <n|c|psi0>, then ed_pole_sum -- greens_lehmann_t0 of the Python layer,
+ * the ground-state term of gf_lehmann (cmpy/exactdiag.py:215-245).          */
+int ed_greens_lehmann_t0_workspace(const ed_op* op_p, const ed_op* op_m, int64_t* bytes);
+int ed_greens_lehmann_t0(ed_op* op_p, ed_op* op_m, int32_t num_sites, int32_t n_up, int32_t n_dn, int32_t pos,
+                         int32_t sigma, int32_t sign_mode, const double* psi_dev, double e0, const double* z_dev,
+                         int64_t nz, double* g_dev, void* work_dev, int64_t work_bytes, void* stream);
+
 /* Pole sum g[k] (+)= sum_m w_m / (z_k - p_m): the T = 0 Lehmann sum of
  * _accumulate_sum (cmpy/exactdiag.py:110-129) restricted to the ground state. */
 int ed_pole_sum(const double* w_dev, const double* p_dev, int64_t m, const double* z_dev,

@@ -111,6 +111,7 @@ int main(int argc, char** argv) {
     return 1;
   }
   std::printf("switches %llu\n", (unsigned long long)emu::E().switches);
+  std::printf("chunks %zu\n", b->chunks.size());
   for (int64_t i = 0; i < M; ++i) {
     std::printf("e0 %lld %.17g\n", (long long)i, e0[i]);
     std::printf("it %lld %d %d %d\n", (long long)i, iters[3 * i], iters[3 * i + 1], iters[3 * i + 2]);

@@ -406,6 +406,13 @@ inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int) {
   *v = a == hipDeviceAttributeMultiprocessorCount ? emu::device_cus() : 65536;
   return hipSuccess;
 }
+// Free device memory as the emulated device reports it: EMU_FREE_BYTES, or 8 GiB.
+inline hipError_t hipMemGetInfo(size_t* free_b, size_t* total_b) {
+  const char* v = std::getenv("EMU_FREE_BYTES");
+  *free_b = v && *v ? (size_t)std::strtoull(v, nullptr, 10) : ((size_t)8 << 30);
+  *total_b = (size_t)16 << 30;
+  return hipSuccess;
+}
 template <class T>
 inline hipError_t hipMalloc(T** p, size_t n) {
   // device memory starts as NaN bit patterns (0xFF bytes), so reads of

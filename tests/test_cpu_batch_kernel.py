@@ -65,8 +65,11 @@ def _write(path, models, wn, wave_max_dim, grid, wave_blocks, keep=0, max_iter=3
                     + " ".join(f"{x:.17g}" for x in m["eb"]) + " " + " ".join(f"{x:.17g}" for x in m["v"]) + "\n")
 
 
-def _run(exe, path, timeout=600):
-    r = subprocess.run([str(exe), str(path)], capture_output=True, text=True, timeout=timeout)
+def _run(exe, path, timeout=600, env=None):
+    import os
+
+    r = subprocess.run([str(exe), str(path)], capture_output=True, text=True, timeout=timeout,
+                       env={**os.environ, **(env or {})})
     return r
 
 
@@ -170,3 +173,22 @@ def test_planted_slot_overrun_is_reported(tmp_path):
     r = _run(exe, path)
     assert r.returncode != 0
     assert "AddressSanitizer" in r.stderr and "use-after-poison" in r.stderr, r.stderr[-3000:]
+
+
+def test_chunked_store_matches_single_launch(harness, tmp_path):
+    """A ground-state vector store larger than the device offers (here 30 MB
+    of "free" memory: a budget of 18 MB against 31 MB for the six models)
+    splits the batch into chunks of models, one launch each: clean, and every
+    result identical to the single launch."""
+    models = _models((7, 6, 5), per=2)
+    wn = (2 * np.arange(8) + 1) * np.pi / 50.0
+    path = tmp_path / "chunks.txt"
+    _write(path, models, wn, 1225, 2, 1)
+    one = _run(harness, path)
+    _clean(one)
+    many = _run(harness, path, env={"EMU_FREE_BYTES": str(30_000_000)})
+    _clean(many)
+    r1 = _parse(one.stdout, len(models), len(wn))
+    r2 = _parse(many.stdout, len(models), len(wn))
+    assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[2], r2[2])
+    assert "chunks 1" in one.stdout and "chunks 1" not in many.stdout
