@@ -25,7 +25,7 @@ from .exactdiag import (
     EigenState, groundstate, compute_groundstate, solve_sector, GreensFunctionMeasurement,
     gf_lehmann, greens_lehmann_t0, greens_cf, iter_lanczos_coeffs, lanczos_coeffs,
     lanczos_matrix, lanczos_ground_state, gf_greater, gf_lesser, gf_tevo, greens_lehmann, greens_cf_ij,
-    occupation_up, occupation_dn, occupation, double_occupation, accumulate_gf,
+    occupation_up, occupation_dn, occupation, double_occupation, accumulate_gf, fourier_t2z,
 )
 
 from .streams import map_streams

@@ -130,6 +130,7 @@ SIGNATURES = {
     "ed_cf_eval": (C.c_int, [P(dbl), P(dbl), i32, dbl, dbl, i32, vp, i64, vp, i32, vp]),
     "ed_tevo_eval": (C.c_int, [P(dbl), P(dbl), i32, dbl, dbl, i32, vp, i64, vp, i32, vp]),
     "ed_pole_sum": (C.c_int, [vp, vp, i64, vp, i64, vp, i32, vp]),
+    "ed_fourier_t2z": (C.c_int, [vp, vp, i64, vp, i64, vp, vp]),
     "ed_greens_cf_workspace": (C.c_int, [vp, vp, i32, i32, P(i64)]),
     "ed_greens_cf": (C.c_int, [vp, vp] + [i32] * 6 + [vp, dbl, i32, i32, vp, i64, vp, P(CfParts), vp, i64, vp]),
     "ed_greens_lehmann_t0_workspace": (C.c_int, [vp, vp, P(i64)]),

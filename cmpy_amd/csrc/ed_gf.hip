@@ -10,6 +10,9 @@
 //   G(z) = <psi0| c (z + E0 - H)^-1 c+ |psi0> + <psi0| c+ (z - E0 + H)^-1 c |psi0>,
 // the T = 0 limit of the Lehmann sum that cmpy accumulates in
 // _accumulate_sum (cmpy/exactdiag.py:110-134) for c = c_{pos,sigma}.
+//
+// Also here: ed_fourier_t2z, the real-time to complex-frequency transform of
+// fourier_t2z (cmpy/exactdiag.py:311-316).
 #include "ed_internal.h"
 
 #include <algorithm>
@@ -52,9 +55,92 @@ __global__ void k_lehmann_terms(const double* __restrict__ ov, const double* __r
   }
 }
 
+struct c64 {
+  double re, im;
+};
+__device__ __forceinline__ c64 cmul(c64 a, c64 b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ c64 cdivc(c64 a, c64 b) {
+  // Smith's algorithm
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double r = b.im / b.re, d = b.re + b.im * r;
+    return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  }
+  const double r = b.re / b.im, d = b.re * r + b.im;
+  return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+__device__ __forceinline__ c64 cexp_(c64 x) {
+  double sn, cs;
+  sincos(x.im, &sn, &cs);
+  const double e = exp(x.re);
+  return {e * cs, e * sn};
+}
+
+// G(z_j) = int_0^T e^{i z_j t} G(t) dt with G linear between the samples
+// t_0 < ... < t_{nt-1}: per interval of length h, with a = i z and x = a h,
+//   e^{a t_k} h [g_k E1(x) + (g_{k+1} - g_k) E2(x)],
+//   E1 = (e^x - 1)/x = sum_n x^n/(n+1)!,  E2 = (x e^x - e^x + 1)/x^2 = sum_n (n+1) x^n/(n+2)!,
+// the series (12 terms) for |x| < 0.1.  One thread per frequency, the
+// intervals summed in order.
+__global__ __launch_bounds__(256) void k_fourier_t2z(const double* __restrict__ t, const double* __restrict__ g,
+                                                     int64_t nt, const double* __restrict__ z, int64_t nz,
+                                                     double* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nz) return;
+  const c64 a = {-z[2 * j + 1], z[2 * j]};  // i z
+  c64 acc = {0.0, 0.0};
+  for (int64_t k = 0; k + 1 < nt; ++k) {
+    const double h = t[k + 1] - t[k];
+    const c64 x = {a.re * h, a.im * h};
+    c64 e1, e2;
+    if (x.re * x.re + x.im * x.im < 0.01) {
+      e1 = {0.0, 0.0};
+      e2 = {0.0, 0.0};
+      double f1 = 1.0, f2 = 1.0;  // 1/(n+1)!, 1/(n+2)! built downward from n = 11
+      for (int n = 1; n <= 13; ++n) f2 /= (double)n;  // 1/13!
+      f1 = f2 * 13.0;                                  // 1/12!
+      for (int n = 11; n >= 0; --n) {
+        e1 = cmul(e1, x);
+        e1.re += f1;
+        e2 = cmul(e2, x);
+        e2.re += (double)(n + 1) * f2;
+        f2 = f1;              // 1/(n+1)! becomes the next 1/(n+2)!
+        f1 *= (double)(n + 1);  // 1/n!
+      }
+    } else {
+      const c64 ex = cexp_(x);
+      const c64 em1 = {ex.re - 1.0, ex.im};
+      e1 = cdivc(em1, x);
+      const c64 xe = cmul(x, ex);
+      e2 = cdivc({xe.re - em1.re, xe.im - em1.im}, cmul(x, x));
+    }
+    const c64 ph = cexp_({a.re * t[k], a.im * t[k]});
+    const c64 gk = {g[2 * k], g[2 * k + 1]};
+    const c64 dg = {g[2 * k + 2] - gk.re, g[2 * k + 3] - gk.im};
+    const c64 p1 = cmul(gk, e1), p2 = cmul(dg, e2);
+    const c64 term = cmul(ph, {h * (p1.re + p2.re), h * (p1.im + p2.im)});
+    acc.re += term.re;
+    acc.im += term.im;
+  }
+  out[2 * j] = acc.re;
+  out[2 * j + 1] = acc.im;
+}
+
 }  // namespace ed
 
 using namespace ed;
+
+extern "C" int ed_fourier_t2z(const double* t_dev, const double* g_dev, int64_t nt, const double* z_dev, int64_t nz,
+                              double* out_dev, void* stream) {
+  clear_error();
+  ED_REQUIRE(nt >= 0 && nz >= 0, "negative size");
+  ED_REQUIRE(nz == 0 || (z_dev && out_dev), "NULL buffer");
+  ED_REQUIRE(nt == 0 || (t_dev && g_dev), "NULL buffer");
+  if (nz == 0) return ED_OK;
+  hipLaunchKernelGGL(k_fourier_t2z, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, as_stream(stream), t_dev,
+                     g_dev, nt, z_dev, nz, out_dev);
+  ED_HIP(hipGetLastError());
+  return ED_OK;
+}
 
 extern "C" int ed_greens_cf_workspace(const ed_op* op_p, const ed_op* op_m, int32_t nsteps, int32_t reorth,
                                       int64_t* bytes) {

@@ -42,7 +42,7 @@ __all__ = [
     "GreensFunctionMeasurement", "gf_lehmann", "greens_lehmann_t0", "greens_cf", "CFResult",
     "iter_lanczos_coeffs", "lanczos_coeffs", "lanczos_matrix", "lanczos_ground_state",
     "gf_greater", "gf_lesser", "gf_tevo", "greens_lehmann", "greens_cf_ij", "LehmannIJ",
-    "occupation_up", "occupation_dn", "occupation", "double_occupation", "accumulate_gf",
+    "occupation_up", "occupation_dn", "occupation", "double_occupation", "accumulate_gf", "fourier_t2z",
 ]
 
 
@@ -901,6 +901,34 @@ def gf_tevo(model, start, stop, num=1000, pos=0, sigma=UP, nsteps=None, reorth: 
     times, gf_g = gf_greater(model, gs, start, stop, num, pos, sigma, nsteps, reorth, sign_mode)
     times, gf_l = gf_lesser(model, gs, start, stop, num, pos, sigma, nsteps, reorth, sign_mode)
     return times, gf_g - gf_l
+
+
+def fourier_t2z(times, gf_t, omegas, delta=1e-2, eta=None):
+    """Real-time G to complex frequencies (cmpy/exactdiag.py:311-316):
+    z = omegas + i eta, eta = -log(delta) / times[-1] unless given, and
+
+        G(z) = int_{t_0}^{t_end} e^{i z t} G(t) dt
+
+    with G linear between the samples and each interval integrated exactly on
+    the device (``ed_fourier_t2z``) -- the piecewise-linear rule that
+    gftool.fourier.tt2z applies by default.  Returns (z, G(z))."""
+    t = np.ascontiguousarray(times, dtype=np.float64).ravel()
+    g = np.ascontiguousarray(gf_t, dtype=np.complex128).ravel()
+    if t.size != g.size:
+        raise ValueError("times and gf_t must have the same length")
+    if t.size and np.any(np.diff(t) <= 0):
+        raise ValueError("times must increase")
+    if eta is None:
+        eta = -np.log(delta) / t[-1]
+    z = np.asarray(omegas, dtype=np.float64) + 1j * eta
+    zt = _z_device(z)
+    tt = torch.from_numpy(t).cuda()
+    gt_ = torch.from_numpy(np.ascontiguousarray(np.stack([g.real, g.imag], axis=1).ravel())).cuda()
+    out = torch.empty_like(zt)
+    _lib.call("ed_fourier_t2z", _lib.dptr(tt), _lib.dptr(gt_), t.size, _lib.dptr(zt), zt.numel() // 2,
+              _lib.dptr(out), _lib.stream_ptr())
+    oh = out.view(-1, 2).cpu().numpy()
+    return z, (oh[:, 0] + 1j * oh[:, 1]).reshape(np.shape(z))
 
 
 # =============================================================================

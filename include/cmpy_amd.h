@@ -291,6 +291,15 @@ int ed_greens_lehmann_t0(ed_op* op_p, ed_op* op_m, int32_t num_sites, int32_t n_
                          int32_t sigma, int32_t sign_mode, const double* psi_dev, double e0, const double* z_dev,
                          int64_t nz, double* g_dev, void* work_dev, int64_t work_bytes, void* stream);
 
+/* Real time to complex frequency, the transform of fourier_t2z
+ * (cmpy/exactdiag.py:311-316, which calls gftool.fourier.tt2z):
+ *   out[j] = int_{t_0}^{t_{nt-1}} e^{i z_j t} G(t) dt,  G linear between the
+ * samples (the piecewise-linear rule, each interval integrated exactly).
+ * t_dev: [nt] increasing times; g_dev: [nt] and z_dev, out_dev: [nz] complex
+ * interleaved.                                                              */
+int ed_fourier_t2z(const double* t_dev, const double* g_dev, int64_t nt, const double* z_dev, int64_t nz,
+                   double* out_dev, void* stream);
+
 /* Pole sum g[k] (+)= sum_m w_m / (z_k - p_m): the T = 0 Lehmann sum of
  * _accumulate_sum (cmpy/exactdiag.py:110-129) restricted to the ground state. */
 int ed_pole_sum(const double* w_dev, const double* p_dev, int64_t m, const double* z_dev,

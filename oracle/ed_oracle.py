@@ -941,6 +941,42 @@ def tevo_two_site_ref(times, u, v):
     return -2j * (a1 * np.cos(e1 * t) + a2 * np.cos(e2 * t))
 
 
+def fourier_t2z(times, gf_t, omegas, delta=1e-2, eta=None):
+    """fourier_t2z (cmpy/exactdiag.py:311-316): z = omegas + i eta with
+    eta = -log(delta) / times[-1] by default, and G(z) = int_0^T e^{izt} G(t) dt
+    by gftool.fourier.tt2z (gftool >= 0.11.0, absent here; its default is the
+    piecewise-linear rule tt2z_lin).  Restated from that rule's definition:
+    G(t) linear between the samples, each interval integrated exactly,
+      int_{t_k}^{t_k+h} e^{a t} (g_k + (g_{k+1} - g_k) s/h) dt
+        = e^{a t_k} h [g_k E1(x) + (g_{k+1} - g_k) E2(x)],  a = i z, x = a h,
+      E1(x) = (e^x - 1)/x,  E2(x) = (x e^x - e^x + 1)/x^2,
+    with the power series of E1, E2 for |x| < 0.1 (no cancellation)."""
+    t = np.asarray(times, dtype=np.float64)
+    g = np.asarray(gf_t, dtype=np.complex128)
+    if eta is None:
+        eta = -np.log(delta) / t[-1]
+    z = np.asarray(omegas, dtype=np.float64) + 1j * eta
+    out = np.zeros(z.shape, dtype=np.complex128)
+    for k in range(len(t) - 1):
+        h = t[k + 1] - t[k]
+        x = 1j * z * h
+        small = np.abs(x) < 0.1
+        xs = np.where(small, 1.0, x)
+        e1 = np.where(small, _series(x, 1), np.expm1(xs) / xs)
+        e2 = np.where(small, _series(x, 2), (xs * np.exp(xs) - np.expm1(xs)) / xs ** 2)
+        out += np.exp(1j * z * t[k]) * h * (g[k] * e1 + (g[k + 1] - g[k]) * e2)
+    return z, out
+
+
+def _series(x, kind, terms=12):
+    """E1(x) = sum_n x^n / (n+1)!,  E2(x) = sum_n (n+1) x^n / (n+2)!."""
+    acc = np.zeros_like(x, dtype=np.complex128)
+    for n in range(terms - 1, -1, -1):
+        c = 1.0 / math.factorial(n + 1) if kind == 1 else (n + 1) / math.factorial(n + 2)
+        acc = acc * x + c
+    return acc
+
+
 # =============================================================================
 # Lattices (replacing lattpy's neighbour lists used by the reference tests)
 # =============================================================================
@@ -1115,6 +1151,9 @@ PARITY = {
     "Hubbard and Anderson sectors; tevo_two_site_ref at half filling",
     "tevo_expm_multiply": "tevo_part (dense eigh) and the two-site closed form",
     "tevo_two_site_ref": "impurity_params_ref poles (pinned via lehmann_t0); t = 0 value -i",
+    "fourier_t2z / _series": "exact on piecewise-linear G(t) (closed-form integral); closed-form transform of "
+                              "sums of exponentials to O(h^2); two-site tevo_two_site_ref transform; "
+                              "series/direct branches agree",
     "lehmann_ij/lehmann_ij_many": "U = 0 resolvent_ij on rings and open chains at interior sites (a wrong string sign "
     "or a dropped factor fails it); i = j equals gf_lehmann; G_ij = G_ji",
     "resolvent_ij": "direct inversion of z - h",

@@ -728,3 +728,69 @@ def test_ladder_block_equals_columns(mode, sigma):
             blk = fn(X, ns, nu, nd, pos, sigma, mode)
             cols = np.column_stack([fn(X[:, k], ns, nu, nd, pos, sigma, mode) for k in range(6)])
             assert_array_equal(blk, cols)
+
+
+# --- fourier_t2z (cmpy/exactdiag.py:311-316) ---------------------------------
+
+
+def _t2z_exact_linear(t0, t1, c0, c1, z):
+    """int_{t0}^{t1} e^{izt} (c0 + c1 t) dt in closed form."""
+    a = 1j * z
+    F = lambda t: np.exp(a * t) * (c0 / a + c1 * (t / a - 1.0 / a**2))
+    return F(t1) - F(t0)
+
+
+def test_fourier_t2z_exact_on_piecewise_linear():
+    rng = np.random.default_rng(4)
+    t = np.sort(np.concatenate([[0.0, 7.0], rng.uniform(0, 7, 60)]))
+    w = np.linspace(-4, 4, 33)
+    # one line through every sample: exact
+    c0, c1 = 0.4 - 0.9j, -0.3 + 0.2j
+    z, gw = O.fourier_t2z(t, c0 + c1 * t, w, eta=0.3)
+    assert np.abs(gw - _t2z_exact_linear(t[0], t[-1], c0, c1, z)).max() <= 1e-12
+    # a kink at a sample: exact piece by piece
+    tm = t[30]
+    g = np.where(t <= tm, c0 + c1 * t, (c0 + c1 * tm) - 2.0 * (t - tm))
+    z, gw = O.fourier_t2z(t, g, w, eta=0.3)
+    ref = _t2z_exact_linear(t[0], tm, c0, c1, z) + _t2z_exact_linear(tm, t[-1], c0 + c1 * tm + 2.0 * tm, -2.0, z)
+    assert np.abs(gw - ref).max() <= 1e-12
+
+
+def test_fourier_t2z_second_order_on_exponentials():
+    e, T = 1.7, 6.0
+    w = np.linspace(-3, 3, 25)
+    errs = []
+    for n in (301, 601):
+        t = np.linspace(0, T, n)
+        z, gw = O.fourier_t2z(t, np.exp(-1j * e * t), w, eta=0.5)
+        exact = (np.exp(1j * (z - e) * T) - 1) / (1j * (z - e))
+        errs.append(np.abs(gw - exact).max())
+    assert 3.8 < errs[0] / errs[1] < 4.2  # O(h^2)
+    assert errs[1] < 1e-4
+
+
+def test_fourier_t2z_two_site_closed_form():
+    """Transform of g> - g< of the two-site SIAM, -2i sum_k a_k cos(e_k t):
+    G(z) = -sum_k a_k [(e^{i(z-e_k)T} - 1)/(z - e_k) + (e^{i(z+e_k)T} - 1)/(z + e_k)],
+    which tends to the Lange G(z) = sum_k a_k [1/(z - e_k) + 1/(z + e_k)] as T grows."""
+    u, v = 2.0, 0.5
+    a1, a2, e1, e2 = O.impurity_params_ref(u, v)
+    T = 40.0
+    t = np.linspace(0, T, 8001)
+    w = np.linspace(-3, 3, 41)
+    z, gw = O.fourier_t2z(t, O.tevo_two_site_ref(t, u, v), w, delta=1e-4)
+    assert abs(z[0].imag - (-np.log(1e-4) / T)) <= 1e-15
+    ref = np.zeros_like(z)
+    for a, e in ((a1, e1), (a2, e2)):
+        ref -= a * ((np.exp(1j * (z - e) * T) - 1) / (z - e) + (np.exp(1j * (z + e) * T) - 1) / (z + e))
+    assert np.abs(gw - ref).max() <= 1e-5
+    lange = sum(a * (1 / (z - e) + 1 / (z + e)) for a, e in ((a1, e1), (a2, e2)))
+    assert np.abs(gw - lange).max() <= 1e-3  # e^{-eta T} = delta = 1e-4 relative tail
+
+
+def test_fourier_t2z_series_and_direct_agree():
+    x = 0.1 * np.exp(1j * np.linspace(0, 2 * np.pi, 17)) * 0.999
+    e1 = np.expm1(x) / x
+    e2 = (x * np.exp(x) - np.expm1(x)) / x**2
+    assert np.abs(O._series(x, 1) - e1).max() <= 1e-14
+    assert np.abs(O._series(x, 2) - e2).max() <= 1e-13
