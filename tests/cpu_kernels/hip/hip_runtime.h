@@ -351,6 +351,14 @@ inline V emu_mfma_f64_16x16x4(double a, double b, V c, int cbsz, int abid, int b
 }
 #define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, x, y, z) emu_mfma_f64_16x16x4(a, b, c, x, y, z)
 
+// Integer min/max overloads of the device headers.
+inline int max(int a, int b) { return a > b ? a : b; }
+inline int min(int a, int b) { return a < b ? a : b; }
+inline long long max(long long a, long long b) { return a > b ? a : b; }
+inline long long min(long long a, long long b) { return a < b ? a : b; }
+inline unsigned max(unsigned a, unsigned b) { return a > b ? a : b; }
+inline unsigned min(unsigned a, unsigned b) { return a < b ? a : b; }
+
 template <class T>
 inline T atomicAdd(T* p, T v) {
   const T old = *p;
@@ -398,6 +406,10 @@ inline int& device_cus() {
 
 inline const char* hipGetErrorString(hipError_t e) { return e == hipSuccess ? "no error" : "emulated error"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipGetDeviceCount(int* n) {
+  *n = 1;
+  return hipSuccess;
+}
 inline hipError_t hipGetDevice(int* d) {
   *d = 0;
   return hipSuccess;

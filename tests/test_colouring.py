@@ -2,7 +2,8 @@
 
 cmpy_amd/csrc/ed_colouring.h orders each 32-state half-wave's hop sources so
 that every gather slot reads 32 distinct LDS bank pairs (index mod 32).  A
-small g++ driver runs the header on the CPU; the checks are the colouring's
+small g++ driver runs the header on the CPU, under AddressSanitizer and UBSan;
+the checks are the colouring's
 defining properties: every slot is a permutation of the 32 bank pairs, every
 lane keeps exactly its own sources, pads are zero-pad indices D..D+31, and
 the slot count equals the largest lane or bank-pair degree (the Konig bound
@@ -51,7 +52,10 @@ def driver(tmp_path_factory):
     src = d / "drv.cpp"
     src.write_text(DRIVER)
     exe = d / "drv"
-    subprocess.run(["g++", "-O2", "-std=c++17", f"-I{ROOT / 'cmpy_amd' / 'csrc'}", str(src), "-o", str(exe)],
+    # AddressSanitizer and UBSan on the host table builder: every test below
+    # runs the colouring under both and fails on any report
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+                    "-fno-omit-frame-pointer", f"-I{ROOT / 'cmpy_amd' / 'csrc'}", str(src), "-o", str(exe)],
                    check=True)
     return exe
 
