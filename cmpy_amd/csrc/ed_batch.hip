@@ -16,9 +16,13 @@
 // Work layout.  Models are split by sector size into two queues, both sorted
 // by decreasing cost.  Workgroups [0, wave_blocks) run "wave mode": each wave
 // takes one small model at a time; the other workgroups run "block mode": the
-// whole workgroup takes one large model.  Every group pops models from its
+// whole workgroup takes one large model.  Every group pops tasks from its
 // queue with one atomic add until the queue is empty, so every wave reaches
-// the kernel's end.  All vectors and spin tables of a group live in its slot
+// the kernel's end.  A solve is two persistent launches over the same queues:
+// the ground-state tasks (one per model), then the continued-fraction tasks
+// (particle and hole of each model, two tasks), so a model's two fractions
+// run on two groups at once; a last small kernel sums G from each model's
+// coefficient record.  All vectors and spin tables of a group live in its slot
 // of a global workspace (L1/L2-resident for these sizes); Lanczos scalars are
 // reduced in a fixed order, so results are deterministic for a given
 // configuration.  The ground-state Lanczos vectors of every model are kept
@@ -39,7 +43,19 @@ namespace batch {
 
 constexpr int kMaxBath = ED_BATCH_MAX_BATH;
 constexpr int kMaxSites = kMaxBath + 1;
-constexpr int kBlock = 1024;  // threads per workgroup
+constexpr int kBlock = 1024;      // threads per workgroup of the backward kernel
+// Solve kernels, per phase (ground-state tasks, continued-fraction tasks):
+// threads per workgroup (the VGPR budget is 512 / waves per SIMD) and whether
+// block-mode sweeps take the wave-private row path.  Measured on the
+// 512-model batch (bath sizes 5..9): the ground-state phase is fastest with
+// 16 waves and workgroup rows (the wave-private path needs more than the 128
+// VGPRs of 16 waves: 27-30 ms against 32-40 ms on 8 waves), the fraction phase
+// with 12 waves and wave-private rows (54-57 ms against 70 ms).
+constexpr int kBlockGround = 1024;
+constexpr bool kFastGround = false;
+constexpr int kBlockFrac = 768;
+constexpr bool kFastFrac = true;
+constexpr int kBlockSolveMax = kBlockGround > kBlockFrac ? kBlockGround : kBlockFrac;
 
 struct ModelParams {
   int32_t nb, n_up, n_dn, pad;
@@ -54,7 +70,7 @@ struct Config {
   const int32_t* queue_wave;  // model indices, decreasing cost
   const int32_t* queue_block;
   int32_t n_wave, n_block;
-  int32_t* counters;          // [2]: next position in each queue
+  int32_t* counters;          // [4]: next position in each queue, ground-state then fraction tasks
   int32_t wave_blocks;        // workgroups [0, wave_blocks) run wave mode
   // slot workspace
   char* ws_wave;
@@ -64,7 +80,7 @@ struct Config {
   int64_t vec_cap_wave, vec_cap_block;     // vector capacity (doubles)
   // algorithm
   int32_t max_iter, cf_steps, check_every, sign_width;
-  int32_t staged;    // block mode: row-staged H.v sweep (LDS) where it applies
+  int32_t staged;    // block mode: 1 wave-private row staging (else workgroup rows), 2 workgroup rows, 0 none
   double tol, cf_tol;
   uint64_t seed;
   const double* wn;  // [nw] Matsubara frequencies (imaginary parts)
@@ -75,9 +91,10 @@ struct Config {
   double* g;      // [num_models][nw][2]
   int32_t* iters; // [num_models][3]: ground-state, particle and hole Lanczos steps
   double* resid;  // [num_models][2]: Ritz residual, explicit residual
-  // kept for the adjoint backward (keep != 0): per model psi [D], the
-  // continued-fraction Krylov vectors state-major Qt[D_part][kcap] (particle,
-  // hole) and coefficients [a+ | b+ | a- | b- (kcap each) | e0, n2+, n2-, m+, m-]
+  // per model coefficient record [a+ | b+ | a- | b- (kcap each) | e0, n2+,
+  // n2-, m+, m-] (kcap = cf_steps); kept for the adjoint backward (keep != 0):
+  // per model psi [D] and the continued-fraction Krylov vectors state-major
+  // Qt[D_part][kcap] (particle, hole)
   int32_t keep, kcap;
   const int64_t* off_psi;  // [num_models]
   const int64_t* off_q;    // [num_models][2]
@@ -96,10 +113,12 @@ using lds_u32 = __attribute__((address_space(3))) uint32_t;
 using lds_i32 = __attribute__((address_space(3))) int32_t;
 
 constexpr int kLdsChunks = 64;  // 64-state chunks of the dn sector the LDS cache covers
-// LDS sizes: staged rows (two buffers of R rows of at most 64 * 16 / R
-// columns: 2048 doubles), cached dn and up hop slots (12 bytes each) and
-// up-state counts
+// LDS sizes: staged rows (solve kernel: one row of at most 256 columns per
+// wave, 4096 doubles; backward kernel: 2048 doubles, one row of at most 128
+// columns per wave or two buffers of R rows of at most 64 * 16 / R columns),
+// cached dn and up hop slots (12 bytes each) and up-state counts
 constexpr int kLdsRows = 2048;
+constexpr int kLdsRowsSolve = 4096;
 constexpr int kLdsDnSolve = 4096;   // solve kernel: 48 KiB
 constexpr int kLdsUpSolve = 4096;   // 48 KiB
 constexpr int kLdsUpRowsSolve = 4096;
@@ -136,6 +155,7 @@ struct Group {
   double* red;  // LDS: 2 * 16 doubles (block mode)
   BlockLds lds{};
   bool has_lds = false;
+  bool wave_rows = false;  // block mode: wave-private row staging where the dn sector fits
 
   __device__ void sync() const {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -434,11 +454,93 @@ __device__ double hv_rows_block(const Group<true>& g, const ModelParams& p, cons
   return dot;
 }
 
+// Block-mode sweep 1 of lanczos_step with wave-private rows, for dn sectors of
+// at most 64 C states: w = sj H u - bj sp uprev, returning this thread's part
+// of <w, u>.  Wave wv takes rows wv, wv + waves, ...; lane l holds columns
+// l + 64 c (c < C).  Per row: the row of u and the up hops (contiguous pieces
+// of other rows, HB hops at a time) are loaded together, the row is staged in
+// the wave's own LDS buffer for the dn gathers through the cached lists, and
+// no workgroup barrier is taken, so every wave streams its rows at its own
+// pace.
+template <int C>
+__device__ double hv_rows_wave(const Group<true>& g, const ModelParams& p, const SpinList& U, const SpinList& Dn,
+                               const double* __restrict__ u, const double* __restrict__ uprev,
+                               double* __restrict__ w, double sj, double sp, double bj) {
+  constexpr int HB = C >= 4 ? 2 : 3;  // up hops whose loads (C columns each) are in flight together
+  const int nd = Dn.n, nu = U.n;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = (int)(threadIdx.x & 63);
+  const int nwv = (int)(blockDim.x >> 6);
+  lds_f64* const row = g.lds.rows + wave * (64 * C);
+  const double bsp = bj * sp;
+  const int us = g.lds.umeta[1];  // up slots per state
+  const int nch = (nd + 63) >> 6;
+  double dot = 0.0;
+  for (int iu = wave; iu < nu; iu += nwv) {
+    const int ro = iu * nd;
+    const int cu = rfl(g.lds.ucnt[iu]);
+    double x[C], acc[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int id = lane + 64 * c;
+      x[c] = id < nd ? u[ro + id] : 0.0;
+      acc[c] = 0.0;
+    }
+    for (int e0 = 0; e0 < cu; e0 += HB) {
+      double gv[HB][C];
+      double hv[HB];
+#pragma unroll
+      for (int e = 0; e < HB; ++e) {
+        const bool on = e0 + e < cu;  // uniform over the wave
+        const int t = on ? rfl((int)g.lds.utgt[iu * us + e0 + e]) : 0;
+        hv[e] = on ? rfl_d(g.lds.uval[iu * us + e0 + e]) : 0.0;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const int id = lane + 64 * c;
+          gv[e][c] = (on && id < nd) ? u[t * nd + id] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < HB; ++e)
+#pragma unroll
+        for (int c = 0; c < C; ++c) acc[c] = fma(hv[e], gv[e][c], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int id = lane + 64 * c;
+      if (id < nd) row[id] = x[c];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const double dgu = U.dg[iu];
+    const uint32_t stu = (uint32_t)rfl((int)U.st[iu]);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int id = lane + 64 * c;
+      const int idc = id < nd ? id : nd - 1;
+      const double xp = bsp != 0.0 ? uprev[ro + idc] : 0.0;
+      double a = acc[c];
+      const int cm = c < nch ? rfl(g.lds.meta[2 + c]) : 0;  // largest dn-hop count of the column block
+      for (int e = 0; e < cm; ++e) a = fma(g.lds.val[e * nd + idc], row[g.lds.tgt[e * nd + idc]], a);
+      a = fma(Dn.dg[idc] + dgu + ((stu & Dn.st[idc] & 1u) ? p.u : 0.0), x[c], a);
+      const double wk = sj * a - bsp * xp;
+      if (id < nd) {
+        w[ro + id] = wk;
+        dot += wk * x[c];
+      }
+    }
+    // every lane's gathers from the row precede the next row's staging
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  return dot;
+}
+
 // One Lanczos step on sector (U, Dn), lazily scaled: v_j = sj * u, v_{j-1} =
 // sp * uprev.  w = sj H u - bj * sp * uprev; then (if alpha < 0 on entry
 // coefficient unknown) a = <w, v_j>; w -= a v_j; returns |w|^2 through *nrm2.
 // With known coefficients (regeneration passes) a_in is used as given.
-template <bool BLOCK>
+template <bool BLOCK, bool FAST = false>
 __device__ void lanczos_step(const Group<BLOCK>& g, const ModelParams& p, const SpinList& U, const SpinList& Dn,
                              const double* __restrict__ u, const double* __restrict__ uprev, double* __restrict__ w,
                              double sj, double sp, double bj, bool known, double a_in, double* a_out, double* nrm2) {
@@ -453,7 +555,15 @@ __device__ void lanczos_step(const Group<BLOCK>& g, const ModelParams& p, const 
     if (g.has_lds && g.lds.meta[0] == nd && nd > 0 && nch <= nwv && 2ll * (nwv / nch) * nd <= g.lds.rows_cap &&
         D < (1ll << 31)) {
       if (up_cached(g, U)) {
-        dot = hv_rows_block(g, p, U, Dn, u, uprev, w, sj, sp, bj);
+        const int rc = g.lds.rows_cap;
+        if (FAST && g.wave_rows && nd <= 64 && nwv * 64 <= rc)
+          dot = hv_rows_wave<1>(g, p, U, Dn, u, uprev, w, sj, sp, bj);
+        else if (FAST && g.wave_rows && nd <= 128 && nwv * 128 <= rc)
+          dot = hv_rows_wave<2>(g, p, U, Dn, u, uprev, w, sj, sp, bj);
+        else if (FAST && g.wave_rows && nd <= 256 && nwv * 256 <= rc)
+          dot = hv_rows_wave<4>(g, p, U, Dn, u, uprev, w, sj, sp, bj);
+        else
+          dot = hv_rows_block(g, p, U, Dn, u, uprev, w, sj, sp, bj);
         staged = true;
       }
     }
@@ -485,10 +595,28 @@ __device__ void lanczos_step(const Group<BLOCK>& g, const ModelParams& p, const 
   if (!known) a = sj * g.sum(dot);
   *a_out = a;
   g.sync();
-  // sweep 2: w -= a sj u, |w|^2
+  // sweep 2: w -= a sj u, |w|^2 (four states per member in flight; the
+  // same order of operations as one state at a time)
   double n2 = 0.0;
-  for (int64_t k = g.rank; k < D; k += g.size) {
-    const double wk = w[k] - a * sj * u[k];
+  const double as = a * sj;
+  const int64_t gs = g.size;
+  int64_t k = g.rank;
+  for (; FAST && k + 3 * gs < D; k += 4 * gs) {
+    double wv[4], uv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      wv[r] = w[k + r * gs];
+      uv[r] = u[k + r * gs];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double wk = wv[r] - as * uv[r];
+      w[k + r * gs] = wk;
+      n2 += wk * wk;
+    }
+  }
+  for (; k < D; k += gs) {
+    const double wk = w[k] - as * u[k];
     w[k] = wk;
     n2 += wk * wk;
   }
@@ -598,7 +726,7 @@ __device__ cplx cf_value(const double* a, const double* b, int m, double norm2, 
 // at most cf_tol * max(|G|, n2) over each of the last two 8-step checks (the
 // fraction converges last at the frequency closest to the real axis).
 // Returns the step count.
-template <bool BLOCK>
+template <bool BLOCK, bool FAST>
 __device__ int cf_coeffs(const Group<BLOCK>& g, const ModelParams& p, const SpinList& U, const SpinList& Dn,
                          const SlotRef& sl, double n2, int cap, double e0, double sgn, double wmin, double cf_tol,
                          double* a, double* b, double* qt, int kcap) {
@@ -614,7 +742,7 @@ __device__ int cf_coeffs(const Group<BLOCK>& g, const ModelParams& p, const Spin
     double aj, nn;
     if (qt)  // keep the normalised Krylov vector v_j = sj u, state-major
       for (int64_t k = g.rank; k < D; k += g.size) qt[k * kcap + j] = sj * u[k];
-    lanczos_step<BLOCK>(g, p, U, Dn, u, up, w, sj, sp, bj, false, 0.0, &aj, &nn);
+    lanczos_step<BLOCK, FAST>(g, p, U, Dn, u, up, w, sj, sp, bj, false, 0.0, &aj, &nn);
     a[j] = aj;
     m = j + 1;
     const double bn = sqrt(nn);
@@ -651,15 +779,16 @@ __device__ __forceinline__ double start_value(uint64_t seed, int64_t k) {
   return (double)(mix64(seed ^ (0xD1B54A32D192ED03ull * (uint64_t)(k + 1))) >> 11) * 0x1.0p-53 * 2.0 - 1.0;
 }
 
-template <bool BLOCK>
-__device__ void solve_model(const Group<BLOCK>& g, const Config& cf, int mi, const SlotRef& sl) {
+// Ground-state task of model mi: Lanczos over the (n_up, n_dn) sector with
+// every vector kept in the model's store, psi = sum_j sv_j v_j written over the
+// store's first vector, its explicit residual, E0 (also into the model's
+// coefficient record) and the step count.
+template <bool BLOCK, bool FAST>
+__device__ void solve_ground(const Group<BLOCK>& g, const Config& cf, int mi, const SlotRef& sl) {
   const ModelParams& p = cf.params[mi];
   const int ns = p.nb + 1;
-  const int width = cf.sign_width;
-  const int n_m = build_list<BLOCK>(g, p, ns, p.n_up - 1, width, sl, 0);
-  const int n_0 = build_list<BLOCK>(g, p, ns, p.n_up, width, sl, 1);
-  const int n_p = build_list<BLOCK>(g, p, ns, p.n_up + 1, width, sl, 2);
-  const int nd = build_list<BLOCK>(g, p, ns, p.n_dn, width, sl, 3);
+  const int n_0 = build_list<BLOCK>(g, p, ns, p.n_up, cf.sign_width, sl, 1);
+  const int nd = build_list<BLOCK>(g, p, ns, p.n_dn, cf.sign_width, sl, 3);
   g.sync();
   const SpinList U = sl.list(1, n_0);
   const SpinList Dn = sl.list(3, nd);
@@ -672,9 +801,8 @@ __device__ void solve_model(const Group<BLOCK>& g, const Config& cf, int mi, con
   const int64_t D = (int64_t)n_0 * nd;
   double* const alpha = sl.alpha();
   double* const beta = sl.beta();
-  double* const psi = sl.vec(3);
 
-  // ---- ground state: every Lanczos vector kept (u_j = V_j, v_j = s_j u_j) ----
+  // ---- Lanczos with every vector kept (u_j = V_j, v_j = s_j u_j) ----
   const uint64_t seed = cf.seed ^ (0xA0761D6478BD642Full * (uint64_t)(mi + 1));
   double* const V = cf.gs_store + cf.off_gs[mi];
   double n2 = 0.0;
@@ -693,7 +821,7 @@ __device__ void solve_model(const Group<BLOCK>& g, const Config& cf, int mi, con
   for (int j = 0; j < cap; ++j) {
     double aj, nn;
     const double* u = V + (int64_t)j * D;
-    lanczos_step<BLOCK>(g, p, U, Dn, u, j > 0 ? u - D : u, V + (int64_t)(j + 1) * D, sj, sp, bj, false, 0.0, &aj, &nn);
+    lanczos_step<BLOCK, FAST>(g, p, U, Dn, u, j > 0 ? u - D : u, V + (int64_t)(j + 1) * D, sj, sp, bj, false, 0.0, &aj, &nn);
     m = j + 1;
     const double bn = sqrt(nn);
     if (g.rank == 0) {
@@ -712,7 +840,9 @@ __device__ void solve_model(const Group<BLOCK>& g, const Config& cf, int mi, con
     sj = 1.0 / bn;
     bj = bn;
   }
-  // psi = sum_j sv_j s_j V_j, in step order
+  // psi = sum_j sv_j s_j V_j in step order, over V_0: each state's entries of
+  // every V_j are read by the one member that then overwrites V_0 there
+  double* const psi = V;
   {
     double* const c = sl.tc();
     for (int j = g.rank; j < m; j += g.size) c[j] = sl.sv()[j] * (j == 0 ? s0 : 1.0 / beta[j - 1]);
@@ -739,102 +869,97 @@ __device__ void solve_model(const Group<BLOCK>& g, const Config& cf, int mi, con
     double aj, nn;
     // w = H psi - theta psi (unit scale, no previous vector, known coefficient)
     double* wr = sl.vec(2);
-    lanczos_step<BLOCK>(g, p, U, Dn, psi, sl.vec(0), wr, 1.0, 0.0, 0.0, true, theta, &aj, &nn);
+    lanczos_step<BLOCK, FAST>(g, p, U, Dn, psi, sl.vec(0), wr, 1.0, 0.0, 0.0, true, theta, &aj, &nn);
     double r2 = 0.0;
     for (int64_t k = g.rank; k < D; k += g.size) r2 += wr[k] * wr[k];
     res2 = g.sum(r2);
     g.sync();
   }
-
-  // ---- particle (c+_{0,up} psi, sector n_up + 1) and hole (c_{0,up} psi, n_up - 1) ----
-  int mp = 0, mm = 0;
-  double np2 = 0.0, nm2 = 0.0;
-  for (int part = 0; part < 2; ++part) {
-    const int l = part == 0 ? 2 : 0;
-    const SpinList T = sl.list(l, part == 0 ? n_p : n_m);
-    if (T.n == 0) continue;
-    const int64_t Dt = (int64_t)T.n * nd;
-    double* phi = sl.vec(1);
-    double nn = 0.0;
-    // gather: target row it takes source row rank(st ^ 1) when its bit 0 fits
-    for (int64_t k = g.rank; k < Dt; k += g.size) {
-      const int it = (int)(k / nd), id = (int)(k - (int64_t)it * nd);
-      const uint32_t st = T.st[it];
-      const bool ok = part == 0 ? (st & 1u) != 0 : (st & 1u) == 0;
-      double v = 0.0;
-      if (ok) v = psi[(int64_t)comb_rank(st ^ 1u) * nd + id];
-      phi[k] = v;
-      nn += v * v;
-    }
-    nn = g.sum(nn);
-    g.sync();
-    if (!(nn > 0.0)) continue;
-    if constexpr (BLOCK) {
-      if (g.has_lds) cache_up(g, T);
-    }
-    if (part == 0) {
-      np2 = nn;
-      double* qt = cf.keep ? cf.store_q + cf.off_q[2 * mi] : nullptr;
-      mp = cf_coeffs<BLOCK>(g, p, T, Dn, sl, nn, cf.cf_steps, theta, 1.0, cf.wmin, cf.cf_tol, sl.cfc(0), sl.cfc(1), qt,
-                            cf.kcap);
-    } else {
-      nm2 = nn;
-      double* qt = cf.keep ? cf.store_q + cf.off_q[2 * mi + 1] : nullptr;
-      mm = cf_coeffs<BLOCK>(g, p, T, Dn, sl, nn, cf.cf_steps, theta, -1.0, cf.wmin, cf.cf_tol, sl.cfc(2), sl.cfc(3),
-                            qt, cf.kcap);
-    }
-  }
-  // coefficient arrays were written by every member with the same values
-  g.sync();
-  for (int n = g.rank; n < cf.nw; n += g.size) {
-    cplx gsum = {0.0, 0.0};
-    if (mp > 0) {
-      const cplx c = cf_value(sl.cfc(0), sl.cfc(1), mp, np2, theta, 1.0, cf.wn[n]);
-      gsum.re += c.re;
-      gsum.im += c.im;
-    }
-    if (mm > 0) {
-      // hole part: -norm2 / (-z + e0 - a0 - ...) = norm2 / (z - e0 + a0 + ...)
-      const cplx c = cf_value(sl.cfc(2), sl.cfc(3), mm, nm2, theta, -1.0, cf.wn[n]);
-      gsum.re += c.re;
-      gsum.im += c.im;
-    }
-    cf.g[((int64_t)mi * cf.nw + n) * 2] = gsum.re;
-    cf.g[((int64_t)mi * cf.nw + n) * 2 + 1] = gsum.im;
-  }
   if (cf.keep) {
     double* ps = cf.store_psi + cf.off_psi[mi];
     for (int64_t k = g.rank; k < D; k += g.size) ps[k] = psi[k];
-    double* co = cf.store_coef + (int64_t)mi * (4 * cf.kcap + 8);
-    for (int j = g.rank; j < cf.kcap; j += g.size) {
-      co[j] = j < mp ? sl.cfc(0)[j] : 0.0;
-      co[cf.kcap + j] = j < mp - 1 ? sl.cfc(1)[j] : 0.0;
-      co[2 * cf.kcap + j] = j < mm ? sl.cfc(2)[j] : 0.0;
-      co[3 * cf.kcap + j] = j < mm - 1 ? sl.cfc(3)[j] : 0.0;
-    }
-    if (g.rank == 0) {
-      double* sc = co + 4 * cf.kcap;
-      sc[0] = theta;
-      sc[1] = np2;
-      sc[2] = nm2;
-      sc[3] = mp;
-      sc[4] = mm;
-    }
   }
   if (g.rank == 0) {
+    cf.store_coef[(int64_t)mi * (4 * cf.kcap + 8) + 4 * cf.kcap] = theta;
     cf.e0[mi] = theta;
     cf.iters[3 * mi] = m;
-    cf.iters[3 * mi + 1] = mp;
-    cf.iters[3 * mi + 2] = mm;
     cf.resid[2 * mi] = ritz;
     cf.resid[2 * mi + 1] = sqrt(res2);
   }
   g.sync();
 }
 
-__global__ __launch_bounds__(kBlock) void k_batch_solve(Config cf) {
+// Continued-fraction task of model mi, after its ground-state task (an
+// earlier launch): part 0 the particle fraction from c+_{0,up} psi (sector
+// n_up + 1), part 1 the hole fraction from c_{0,up} psi (n_up - 1).  The
+// coefficients (zero past the step count), |phi|^2 and the step count go to
+// the part's half of the model's coefficient record
+// [a+ | b+ | a- | b- (kcap each) | e0, n2+, n2-, m+, m-].
+template <bool BLOCK, bool FAST>
+__device__ void solve_fraction(const Group<BLOCK>& g, const Config& cf, int mi, int part, const SlotRef& sl) {
+  const ModelParams& p = cf.params[mi];
+  const int ns = p.nb + 1;
+  const int l = part == 0 ? 2 : 0;
+  const int nt = build_list<BLOCK>(g, p, ns, p.n_up + (part == 0 ? 1 : -1), cf.sign_width, sl, l);
+  const int nd = build_list<BLOCK>(g, p, ns, p.n_dn, cf.sign_width, sl, 3);
+  g.sync();
+  const SpinList T = sl.list(l, nt);
+  const SpinList Dn = sl.list(3, nd);
+  const int kc = cf.kcap;
+  double* const co = cf.store_coef + (int64_t)mi * (4 * kc + 8);
+  double* const a = co + 2 * part * kc;
+  double* const b = a + kc;
+  const double theta = cf.e0[mi];
+  const double* const psi = cf.gs_store + cf.off_gs[mi];
+  const int64_t Dt = (int64_t)nt * nd;
+  double* const phi = sl.vec(1);
+  double nn = 0.0;
+  // gather: target row it takes source row rank(st ^ 1) when its bit 0 fits
+  for (int64_t k = g.rank; k < Dt; k += g.size) {
+    const int it = (int)(k / nd), id = (int)(k - (int64_t)it * nd);
+    const uint32_t st = T.st[it];
+    const bool ok = part == 0 ? (st & 1u) != 0 : (st & 1u) == 0;
+    double v = 0.0;
+    if (ok) v = psi[(int64_t)comb_rank(st ^ 1u) * nd + id];
+    phi[k] = v;
+    nn += v * v;
+  }
+  nn = g.sum(nn);
+  g.sync();
+  int m = 0;
+  if (nn > 0.0) {
+    if constexpr (BLOCK) {
+      if (g.has_lds) {
+        cache_dn(g, Dn);
+        cache_up(g, T);
+      }
+    }
+    double* qt = cf.keep ? cf.store_q + cf.off_q[2 * mi + part] : nullptr;
+    m = cf_coeffs<BLOCK, FAST>(g, p, T, Dn, sl, nn, cf.cf_steps, theta, part == 0 ? 1.0 : -1.0, cf.wmin, cf.cf_tol, a, b,
+                         qt, kc);
+  }
+  // coefficient arrays were written by every member with the same values
+  g.sync();
+  for (int j = g.rank; j < kc; j += g.size) {
+    if (j >= m) a[j] = 0.0;
+    if (j >= m - 1) b[j] = 0.0;
+  }
+  if (g.rank == 0) {
+    co[4 * kc + 1 + part] = nn;
+    co[4 * kc + 3 + part] = m;
+    cf.iters[3 * mi + 1 + part] = m;
+  }
+  g.sync();
+}
+
+// PHASE 0: ground-state tasks, one per model of the queues.  PHASE 1: the
+// fraction tasks, two per model (queue position q >> 1, part q & 1), in the
+// same order.  Each phase pops its own pair of counters.
+template <int PHASE>
+__global__ __launch_bounds__(PHASE ? kBlockFrac : kBlockGround) void k_batch_solve(Config cf) {
   __shared__ double red[32];
   __shared__ int popped;
+  int32_t* const counters = cf.counters + 2 * PHASE;
   if ((int)blockIdx.x < cf.wave_blocks) {
     // wave mode
     const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -842,15 +967,19 @@ __global__ __launch_bounds__(kBlock) void k_batch_solve(Config cf) {
     const int64_t slot = (int64_t)blockIdx.x * (blockDim.x >> 6) + wid;
     const SlotRef sl{cf.ws_wave + slot * cf.slot_bytes_wave, cf.list_cap_wave, cf.vec_cap_wave, cf.max_iter,
                      cf.cf_steps};
+    const int n = PHASE ? 2 * cf.n_wave : cf.n_wave;
     while (true) {
       int q = 0;
-      if (g.rank == 0) q = atomicAdd(&cf.counters[0], 1);
-      q = __builtin_amdgcn_readfirstlane(__shfl(q, 0, 64));  // uniform model index
-      if (q >= cf.n_wave) break;
-      solve_model<false>(g, cf, __builtin_amdgcn_readfirstlane(cf.queue_wave[q]), sl);
+      if (g.rank == 0) q = atomicAdd(&counters[0], 1);
+      q = __builtin_amdgcn_readfirstlane(__shfl(q, 0, 64));  // uniform task index
+      if (q >= n) break;
+      if (PHASE == 0)
+        solve_ground<false, kFastGround>(g, cf, __builtin_amdgcn_readfirstlane(cf.queue_wave[q]), sl);
+      else
+        solve_fraction<false, kFastFrac>(g, cf, __builtin_amdgcn_readfirstlane(cf.queue_wave[q >> 1]), q & 1, sl);
     }
   } else {
-    __shared__ double s_rows[kLdsRows];
+    __shared__ double s_rows[kLdsRowsSolve];
     __shared__ uint32_t s_tgt[kLdsDnSolve];
     __shared__ double s_val[kLdsDnSolve];
     __shared__ int32_t s_meta[2 + kLdsChunks];
@@ -859,23 +988,55 @@ __global__ __launch_bounds__(kBlock) void k_batch_solve(Config cf) {
     __shared__ int32_t s_ucnt[kLdsUpRowsSolve];
     __shared__ int32_t s_umeta[4];
     Group<true> g{(int)threadIdx.x, (int)blockDim.x, red};
-    g.lds = BlockLds{(lds_f64*)s_rows, kLdsRows,          (lds_u32*)s_tgt,  (lds_f64*)s_val,
+    g.lds = BlockLds{(lds_f64*)s_rows, kLdsRowsSolve,     (lds_u32*)s_tgt,  (lds_f64*)s_val,
                      kLdsDnSolve,      (lds_i32*)s_meta,  (lds_u32*)s_utgt, (lds_f64*)s_uval,
                      (lds_i32*)s_ucnt, kLdsUpSolve,       kLdsUpRowsSolve,  (lds_i32*)s_umeta};
     if (threadIdx.x == 0) s_umeta[0] = 0;
     g.has_lds = cf.staged != 0;
+    g.wave_rows = cf.staged == 1;
     const int64_t slot = (int64_t)blockIdx.x - cf.wave_blocks;
     const SlotRef sl{cf.ws_block + slot * cf.slot_bytes_block, cf.list_cap_block, cf.vec_cap_block, cf.max_iter,
                      cf.cf_steps};
+    const int n = PHASE ? 2 * cf.n_block : cf.n_block;
     while (true) {
-      if (threadIdx.x == 0) popped = atomicAdd(&cf.counters[1], 1);
+      if (threadIdx.x == 0) popped = atomicAdd(&counters[1], 1);
       __syncthreads();
       const int q = __builtin_amdgcn_readfirstlane(popped);
       __syncthreads();
-      if (q >= cf.n_block) break;
-      solve_model<true>(g, cf, __builtin_amdgcn_readfirstlane(cf.queue_block[q]), sl);
+      if (q >= n) break;
+      if (PHASE == 0)
+        solve_ground<true, kFastGround>(g, cf, __builtin_amdgcn_readfirstlane(cf.queue_block[q]), sl);
+      else
+        solve_fraction<true, kFastFrac>(g, cf, __builtin_amdgcn_readfirstlane(cf.queue_block[q >> 1]), q & 1, sl);
     }
   }
+}
+
+// G(i w_n) of every model from its coefficient record: the particle plus the
+// hole continued fraction, one thread per (model, frequency).
+__global__ void k_batch_g(Config cf) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cf.num_models * cf.nw) return;
+  const int64_t mi = t / cf.nw;
+  const int n = (int)(t - mi * cf.nw);
+  const int kc = cf.kcap;
+  const double* co = cf.store_coef + mi * (4 * kc + 8);
+  const double e0 = co[4 * kc];
+  const int mp = (int)co[4 * kc + 3], mm = (int)co[4 * kc + 4];
+  cplx gsum = {0.0, 0.0};
+  if (mp > 0) {
+    const cplx c = cf_value(co, co + kc, mp, co[4 * kc + 1], e0, 1.0, cf.wn[n]);
+    gsum.re += c.re;
+    gsum.im += c.im;
+  }
+  if (mm > 0) {
+    // hole part: -norm2 / (-z + e0 - a0 - ...) = norm2 / (z - e0 + a0 + ...)
+    const cplx c = cf_value(co + 2 * kc, co + 3 * kc, mm, co[4 * kc + 2], e0, -1.0, cf.wn[n]);
+    gsum.re += c.re;
+    gsum.im += c.im;
+  }
+  cf.g[2 * t] = gsum.re;
+  cf.g[2 * t + 1] = gsum.im;
 }
 
 // ===========================================================================
@@ -903,7 +1064,7 @@ struct BwdConfig {
   int32_t lcap;
   int64_t vcap;
   int32_t kcap, nw, sign_width, max_bath;
-  int32_t staged;     // row-staged H.v sweep (LDS) where it applies
+  int32_t staged;     // 1 wave-private row staging where it fits (else workgroup rows), 2 workgroup rows, 0 none
   const double* wn;
   const double* gre;  // [num_models][nw] dL/dRe G
   const double* gim;  // [num_models][nw] dL/dIm G
@@ -1362,6 +1523,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_backward(BwdConfig bc) {
                    (lds_i32*)s_ucnt, kLdsUpBwd,         kLdsUpRowsBwd,    (lds_i32*)s_umeta};
   if (threadIdx.x == 0) s_umeta[0] = 0;
   g.has_lds = bc.staged != 0;
+  g.wave_rows = bc.staged == 1;
   const BwdSlot bs{SlotRef{bc.ws + (int64_t)blockIdx.x * bc.slot_bytes, bc.lcap, bc.vcap, 0, bc.kcap}};
   while (true) {
     if (threadIdx.x == 0) popped = atomicAdd(bc.counter, 1);
@@ -1374,7 +1536,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_backward(BwdConfig bc) {
 }
 
 __global__ void k_reset_counters(int32_t* c) {
-  if (threadIdx.x < 2) c[threadIdx.x] = 0;
+  if (threadIdx.x < 4) c[threadIdx.x] = 0;
 }
 
 }  // namespace batch
@@ -1513,7 +1675,7 @@ static int configure(ed_batch* b, int64_t wave_max_dim, int grid_blocks, int wav
   if (!qb.empty() && wave_blocks >= grid_blocks) grid_blocks = wave_blocks + 1;
   const int64_t sw = SlotRef{nullptr, lw, vw, b->max_iter, b->cf_steps}.bytes();
   const int64_t sb = SlotRef{nullptr, lb, vb, b->max_iter, b->cf_steps}.bytes();
-  const int64_t n_wave_slots = (int64_t)wave_blocks * (kBlock / 64);
+  const int64_t n_wave_slots = (int64_t)wave_blocks * (kBlockSolveMax / 64);
   const int64_t n_block_slots = grid_blocks - wave_blocks;
   const int64_t bytes = n_wave_slots * sw + n_block_slots * sb + 256;
   if (bytes > b->ws_bytes) {
@@ -1573,7 +1735,8 @@ static int configure(ed_batch* b, int64_t wave_max_dim, int grid_blocks, int wav
   return ED_OK;
 }
 
-// One persistent launch per chunk of models (one in all but very large batches).
+// Per chunk of models (one in all but very large batches): the ground-state
+// launch, then the fraction launch; then G of every model.
 static int launch(ed_batch* b, hipStream_t s) {
   for (const auto& cq : b->chunk_q) {
     Config c = b->cfg;
@@ -1582,8 +1745,11 @@ static int launch(ed_batch* b, hipStream_t s) {
     c.queue_block = b->d_queues + cq[2];
     c.n_block = cq[3];
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, b->d_counters);
-    hipLaunchKernelGGL(k_batch_solve, dim3((unsigned)b->grid), dim3(kBlock), 0, s, c);
+    hipLaunchKernelGGL(k_batch_solve<0>, dim3((unsigned)b->grid), dim3(kBlockGround), 0, s, c);
+    hipLaunchKernelGGL(k_batch_solve<1>, dim3((unsigned)b->grid), dim3(kBlockFrac), 0, s, c);
   }
+  const int64_t nt = b->num_models * (int64_t)b->cfg.nw;
+  if (nt > 0) hipLaunchKernelGGL(k_batch_g, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, b->cfg);
   ED_HIP(hipGetLastError());
   return ED_OK;
 }
@@ -1718,14 +1884,16 @@ extern "C" int ed_batch_create(ed_batch** out, const ed_batch_desc* d, void* str
                        (long long)i, (long long)b->dim[i]);
     }
   }
+  b->kcap = d->cf_steps;
   b->order.resize(d->num_models);
   std::iota(b->order.begin(), b->order.end(), 0);
   std::stable_sort(b->order.begin(), b->order.end(), [&](int32_t a, int32_t c) { return b->dim[a] > b->dim[c]; });
   hipStream_t s = as_stream(stream);
   int rc = ED_OK;
   if ((rc = balloc(&b->d_params, d->num_models)) != ED_OK || (rc = balloc(&b->d_queues, d->num_models)) != ED_OK ||
-      (rc = balloc(&b->d_counters, 2)) != ED_OK || (rc = balloc(&b->d_e0, d->num_models)) != ED_OK ||
-      (rc = balloc(&b->d_iters, 3 * d->num_models)) != ED_OK || (rc = balloc(&b->d_resid, 2 * d->num_models)) != ED_OK) {
+      (rc = balloc(&b->d_counters, 4)) != ED_OK || (rc = balloc(&b->d_e0, d->num_models)) != ED_OK ||
+      (rc = balloc(&b->d_iters, 3 * d->num_models)) != ED_OK || (rc = balloc(&b->d_resid, 2 * d->num_models)) != ED_OK ||
+      (rc = balloc(&b->d_store_coef, d->num_models * (4 * (int64_t)d->cf_steps + 8))) != ED_OK) {
     free_batch(b);
     return rc;
   }
@@ -1742,7 +1910,6 @@ extern "C" int ed_batch_create(ed_batch** out, const ed_batch_desc* d, void* str
     }
     // per-model stores for the adjoint backward
     b->keep = true;
-    b->kcap = d->cf_steps;
     const int64_t M = d->num_models;
     std::vector<int64_t> off_psi(M), off_q(2 * M);
     int64_t np = 0, nq = 0;
@@ -1758,7 +1925,6 @@ extern "C" int ed_batch_create(ed_batch** out, const ed_batch_desc* d, void* str
     }
     if ((rc = balloc(&b->d_off_psi, M)) != ED_OK || (rc = balloc(&b->d_off_q, 2 * M)) != ED_OK ||
         (rc = balloc(&b->d_store_psi, np)) != ED_OK || (rc = balloc(&b->d_store_q, nq)) != ED_OK ||
-        (rc = balloc(&b->d_store_coef, M * (4 * (int64_t)b->kcap + 8))) != ED_OK ||
         (rc = balloc(&b->d_grad_eps, M * d->max_bath)) != ED_OK || (rc = balloc(&b->d_grad_v, M * d->max_bath)) != ED_OK ||
         (rc = balloc(&b->d_cg, M)) != ED_OK || (rc = balloc(&b->d_bcounter, 1)) != ED_OK) {
       free_batch(b);
