@@ -364,6 +364,14 @@ __device__ __forceinline__ double rfl_d(double v) {
 
 constexpr int kUpPre = 6;  // up hops whose loads start before the staging barrier
 
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+
 // Block-mode sweep 1 of lanczos_step with staged rows, for dn sectors of at
 // most 64 * 16 states: w = sj H u - bj sp uprev, returning this thread's part
 // of <w, u>.  Each step stages R = 16 / (64-column chunks per row) rows of u
@@ -372,6 +380,11 @@ constexpr int kUpPre = 6;  // up hops whose loads start before the staging barri
 // 64-column piece of another row, the first kUpPre loads issued before the
 // staging barrier) and dn hops gathered from the staged row through the
 // cached lists.
+// LEAN: the row's up-hop list is read once (one entry per lane) and broadcast
+// hop by hop, hop sources are addressed from wave-uniform row bases, and the
+// dn hops go four slots at a time (table reads, then gathers, then the sums
+// in slot order); the same arithmetic in the same order.
+template <bool LEAN>
 __device__ double hv_rows_block(const Group<true>& g, const ModelParams& p, const SpinList& U, const SpinList& Dn,
                                 const double* __restrict__ u, const double* __restrict__ uprev,
                                 double* __restrict__ w, double sj, double sp, double bj) {
@@ -405,14 +418,26 @@ __device__ double hv_rows_block(const Group<true>& g, const ModelParams& p, cons
     double xp = 0.0, dgu = 0.0;
     uint32_t stu = 0;
     const int ro = iu * nd;
+    const int lane = (int)(threadIdx.x & 63);
+    int lt = 0;
+    double lv = 0.0;
     if (on) {
       cu = rfl(g.lds.ucnt[iu]);
+      if constexpr (LEAN) {
+        lt = lane < cu ? (int)g.lds.utgt[iu * us + lane] : 0;
+        lv = lane < cu ? g.lds.uval[iu * us + lane] : 0.0;
+      }
 #pragma unroll
       for (int e = 0; e < kUpPre; ++e) {
         if (e < cu) {
-          const int t = rfl((int)g.lds.utgt[iu * us + e]);
-          vu[e] = rfl_d(g.lds.uval[iu * us + e]);
-          gu[e] = u[t * nd + idc];
+          if constexpr (LEAN) {
+            vu[e] = readlane_d(lv, e);
+            gu[e] = (u + (int64_t)__builtin_amdgcn_readlane(lt, e) * nd)[idc];
+          } else {
+            const int t = rfl((int)g.lds.utgt[iu * us + e]);
+            vu[e] = rfl_d(g.lds.uval[iu * us + e]);
+            gu[e] = u[t * nd + idc];
+          }
         }
       }
       if (bsp != 0.0) xp = uprev[ro + idc];
@@ -431,16 +456,37 @@ __device__ double hv_rows_block(const Group<true>& g, const ModelParams& p, cons
 #pragma unroll
         for (int e = 0; e < kUpPre; ++e) {
           if (e0 + e < cu) {
-            const int t = rfl((int)g.lds.utgt[iu * us + e0 + e]);
-            vu[e] = rfl_d(g.lds.uval[iu * us + e0 + e]);
-            gu[e] = u[t * nd + idc];
+            if constexpr (LEAN) {
+              vu[e] = readlane_d(lv, e0 + e);
+              gu[e] = (u + (int64_t)__builtin_amdgcn_readlane(lt, e0 + e) * nd)[idc];
+            } else {
+              const int t = rfl((int)g.lds.utgt[iu * us + e0 + e]);
+              vu[e] = rfl_d(g.lds.uval[iu * us + e0 + e]);
+              gu[e] = u[t * nd + idc];
+            }
           }
         }
 #pragma unroll
         for (int e = 0; e < kUpPre; ++e)
           if (e0 + e < cu) acc = fma(vu[e], gu[e], acc);
       }
-      for (int e = 0; e < cm; ++e) acc = fma(g.lds.val[e * nd + idc], row[g.lds.tgt[e * nd + idc]], acc);
+      int e = 0;
+      if constexpr (LEAN) {
+        for (; e + 4 <= cm; e += 4) {
+          uint32_t tg[4];
+          double vl[4], gr[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            tg[r] = g.lds.tgt[(e + r) * nd + idc];
+            vl[r] = g.lds.val[(e + r) * nd + idc];
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) gr[r] = row[tg[r]];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc = fma(vl[r], gr[r], acc);
+        }
+      }
+      for (; e < cm; ++e) acc = fma(g.lds.val[e * nd + idc], row[g.lds.tgt[e * nd + idc]], acc);
       const double xk = row[idc];
       acc = fma(ddn + dgu + ((stu & sdn & 1u) ? p.u : 0.0), xk, acc);
       const double wk = sj * acc - bsp * xp;
@@ -457,33 +503,42 @@ __device__ double hv_rows_block(const Group<true>& g, const ModelParams& p, cons
 // Block-mode sweep 1 of lanczos_step with wave-private rows, for dn sectors of
 // at most 64 C states: w = sj H u - bj sp uprev, returning this thread's part
 // of <w, u>.  Wave wv takes rows wv, wv + waves, ...; lane l holds columns
-// l + 64 c (c < C).  Per row: the row of u and the up hops (contiguous pieces
-// of other rows, HB hops at a time) are loaded together, the row is staged in
-// the wave's own LDS buffer for the dn gathers through the cached lists, and
-// no workgroup barrier is taken, so every wave streams its rows at its own
-// pace.
+// l + 64 c (c < C), clamped to the last column so every load is in bounds
+// (a clamped lane computes the last column's value again and stores it to the
+// same address; only its dot term is masked).  Per row: the row's up-hop
+// list is read once (one entry per lane) and broadcast hop by hop, the row of
+// u and HB hops' sources (contiguous pieces of other rows, addressed from a
+// wave-uniform row base) are loaded together, the row is staged in the
+// wave's own LDS buffer, and the dn hops are gathered slot by slot with the C
+// columns' table reads and gathers in flight together.  No workgroup
+// barrier is taken: every wave streams its rows at its own pace.
 template <int C>
 __device__ double hv_rows_wave(const Group<true>& g, const ModelParams& p, const SpinList& U, const SpinList& Dn,
                                const double* __restrict__ u, const double* __restrict__ uprev,
                                double* __restrict__ w, double sj, double sp, double bj) {
-  constexpr int HB = C >= 4 ? 2 : 3;  // up hops whose loads (C columns each) are in flight together
+  constexpr int HB = C >= 4 ? 2 : 4;  // up hops whose loads (C columns each) are in flight together
   const int nd = Dn.n, nu = U.n;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = (int)(threadIdx.x & 63);
   const int nwv = (int)(blockDim.x >> 6);
   lds_f64* const row = g.lds.rows + wave * (64 * C);
   const double bsp = bj * sp;
-  const int us = g.lds.umeta[1];  // up slots per state
-  const int nch = (nd + 63) >> 6;
+  const int us = g.lds.umeta[1];      // up slots per state
+  const int ds = rfl(g.lds.meta[1]);  // dn slots per state
+  int idc[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) idc[c] = lane + 64 * c < nd ? lane + 64 * c : nd - 1;
   double dot = 0.0;
   for (int iu = wave; iu < nu; iu += nwv) {
-    const int ro = iu * nd;
+    const int64_t ro = (int64_t)iu * nd;
+    const double* __restrict__ xr = u + ro;
     const int cu = rfl(g.lds.ucnt[iu]);
+    const int lt = lane < cu ? (int)g.lds.utgt[iu * us + lane] : 0;
+    const double lv = lane < cu ? g.lds.uval[iu * us + lane] : 0.0;
     double x[C], acc[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const int id = lane + 64 * c;
-      x[c] = id < nd ? u[ro + id] : 0.0;
+      x[c] = xr[idc[c]];
       acc[c] = 0.0;
     }
     for (int e0 = 0; e0 < cu; e0 += HB) {
@@ -491,14 +546,12 @@ __device__ double hv_rows_wave(const Group<true>& g, const ModelParams& p, const
       double hv[HB];
 #pragma unroll
       for (int e = 0; e < HB; ++e) {
-        const bool on = e0 + e < cu;  // uniform over the wave
-        const int t = on ? rfl((int)g.lds.utgt[iu * us + e0 + e]) : 0;
-        hv[e] = on ? rfl_d(g.lds.uval[iu * us + e0 + e]) : 0.0;
+        const bool on = e0 + e < cu;  // uniform over the wave; an off hop re-reads hop e0 with value 0
+        const int ee = on ? e0 + e : e0;
+        const double* __restrict__ src = u + (int64_t)__builtin_amdgcn_readlane(lt, ee) * nd;
+        hv[e] = on ? readlane_d(lv, ee) : 0.0;
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-          const int id = lane + 64 * c;
-          gv[e][c] = (on && id < nd) ? u[t * nd + id] : 0.0;
-        }
+        for (int c = 0; c < C; ++c) gv[e][c] = src[idc[c]];
       }
 #pragma unroll
       for (int e = 0; e < HB; ++e)
@@ -506,28 +559,34 @@ __device__ double hv_rows_wave(const Group<true>& g, const ModelParams& p, const
         for (int c = 0; c < C; ++c) acc[c] = fma(hv[e], gv[e][c], acc[c]);
     }
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-      const int id = lane + 64 * c;
-      if (id < nd) row[id] = x[c];
-    }
+    for (int c = 0; c < C; ++c) row[idc[c]] = x[c];
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
+    double xp[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) xp[c] = bsp != 0.0 ? uprev[ro + idc[c]] : 0.0;
+    // dn hops slot by slot (slots past a state's count hold target 0 and value 0)
+    for (int e = 0; e < ds; ++e) {
+      uint32_t tg[C];
+      double vl[C], gr[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        tg[c] = g.lds.tgt[e * nd + idc[c]];
+        vl[c] = g.lds.val[e * nd + idc[c]];
+      }
+#pragma unroll
+      for (int c = 0; c < C; ++c) gr[c] = row[tg[c]];
+#pragma unroll
+      for (int c = 0; c < C; ++c) acc[c] = fma(vl[c], gr[c], acc[c]);
+    }
     const double dgu = U.dg[iu];
     const uint32_t stu = (uint32_t)rfl((int)U.st[iu]);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const int id = lane + 64 * c;
-      const int idc = id < nd ? id : nd - 1;
-      const double xp = bsp != 0.0 ? uprev[ro + idc] : 0.0;
-      double a = acc[c];
-      const int cm = c < nch ? rfl(g.lds.meta[2 + c]) : 0;  // largest dn-hop count of the column block
-      for (int e = 0; e < cm; ++e) a = fma(g.lds.val[e * nd + idc], row[g.lds.tgt[e * nd + idc]], a);
-      a = fma(Dn.dg[idc] + dgu + ((stu & Dn.st[idc] & 1u) ? p.u : 0.0), x[c], a);
-      const double wk = sj * a - bsp * xp;
-      if (id < nd) {
-        w[ro + id] = wk;
-        dot += wk * x[c];
-      }
+      const double a = fma(Dn.dg[idc[c]] + dgu + ((stu & Dn.st[idc[c]] & 1u) ? p.u : 0.0), x[c], acc[c]);
+      const double wk = sj * a - bsp * xp[c];
+      w[ro + idc[c]] = wk;
+      if (lane + 64 * c < nd) dot += wk * x[c];
     }
     // every lane's gathers from the row precede the next row's staging
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -540,7 +599,7 @@ __device__ double hv_rows_wave(const Group<true>& g, const ModelParams& p, const
 // sp * uprev.  w = sj H u - bj * sp * uprev; then (if alpha < 0 on entry
 // coefficient unknown) a = <w, v_j>; w -= a v_j; returns |w|^2 through *nrm2.
 // With known coefficients (regeneration passes) a_in is used as given.
-template <bool BLOCK, bool FAST = false>
+template <bool BLOCK, bool FAST = false, bool LEAN = false>
 __device__ void lanczos_step(const Group<BLOCK>& g, const ModelParams& p, const SpinList& U, const SpinList& Dn,
                              const double* __restrict__ u, const double* __restrict__ uprev, double* __restrict__ w,
                              double sj, double sp, double bj, bool known, double a_in, double* a_out, double* nrm2) {
@@ -563,7 +622,7 @@ __device__ void lanczos_step(const Group<BLOCK>& g, const ModelParams& p, const 
         else if (FAST && g.wave_rows && nd <= 256 && nwv * 256 <= rc)
           dot = hv_rows_wave<4>(g, p, U, Dn, u, uprev, w, sj, sp, bj);
         else
-          dot = hv_rows_block(g, p, U, Dn, u, uprev, w, sj, sp, bj);
+          dot = hv_rows_block<LEAN>(g, p, U, Dn, u, uprev, w, sj, sp, bj);
         staged = true;
       }
     }
@@ -742,7 +801,7 @@ __device__ int cf_coeffs(const Group<BLOCK>& g, const ModelParams& p, const Spin
     double aj, nn;
     if (qt)  // keep the normalised Krylov vector v_j = sj u, state-major
       for (int64_t k = g.rank; k < D; k += g.size) qt[k * kcap + j] = sj * u[k];
-    lanczos_step<BLOCK, FAST>(g, p, U, Dn, u, up, w, sj, sp, bj, false, 0.0, &aj, &nn);
+    lanczos_step<BLOCK, FAST, true>(g, p, U, Dn, u, up, w, sj, sp, bj, false, 0.0, &aj, &nn);
     a[j] = aj;
     m = j + 1;
     const double bn = sqrt(nn);
@@ -821,7 +880,7 @@ __device__ void solve_ground(const Group<BLOCK>& g, const Config& cf, int mi, co
   for (int j = 0; j < cap; ++j) {
     double aj, nn;
     const double* u = V + (int64_t)j * D;
-    lanczos_step<BLOCK, FAST>(g, p, U, Dn, u, j > 0 ? u - D : u, V + (int64_t)(j + 1) * D, sj, sp, bj, false, 0.0, &aj, &nn);
+    lanczos_step<BLOCK, FAST, true>(g, p, U, Dn, u, j > 0 ? u - D : u, V + (int64_t)(j + 1) * D, sj, sp, bj, false, 0.0, &aj, &nn);
     m = j + 1;
     const double bn = sqrt(nn);
     if (g.rank == 0) {
@@ -869,7 +928,7 @@ __device__ void solve_ground(const Group<BLOCK>& g, const Config& cf, int mi, co
     double aj, nn;
     // w = H psi - theta psi (unit scale, no previous vector, known coefficient)
     double* wr = sl.vec(2);
-    lanczos_step<BLOCK, FAST>(g, p, U, Dn, psi, sl.vec(0), wr, 1.0, 0.0, 0.0, true, theta, &aj, &nn);
+    lanczos_step<BLOCK, FAST, true>(g, p, U, Dn, psi, sl.vec(0), wr, 1.0, 0.0, 0.0, true, theta, &aj, &nn);
     double r2 = 0.0;
     for (int64_t k = g.rank; k < D; k += g.size) r2 += wr[k] * wr[k];
     res2 = g.sum(r2);

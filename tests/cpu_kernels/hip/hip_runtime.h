@@ -303,6 +303,16 @@ inline T emu_readfirstlane(T v) {
 #define __builtin_amdgcn_readfirstlane(v) emu_readfirstlane(v)
 
 template <class T>
+inline T emu_readlane(T v, int l) {
+  // the lane index must be the same on every lane and name a lane of the wave
+  const int l0 = emu::exchange(l, [](int) { return 0; });
+  if (l0 != l) emu::fail("readlane with a non-uniform lane index", emu::tid());
+  if (l < 0 || l >= 64) emu::fail("readlane lane index out of range", emu::tid());
+  return emu::exchange(v, [=](int) { return l; });
+}
+#define __builtin_amdgcn_readlane(v, l) emu_readlane(v, l)
+
+template <class T>
 inline T __shfl_xor(T v, int mask, int width = 64) {
   return emu::exchange(v, [=](int l) { return (l & ~(width - 1)) | ((l ^ mask) & (width - 1)); });
 }

@@ -131,6 +131,26 @@ def test_block_and_wave_split_against_lehmann(harness, tmp_path):
         assert np.abs(g[i] - ref).max() <= 1e-9 * np.abs(ref).max(), i
 
 
+def test_nine_bath_block_rows_against_wave_mode(harness, tmp_path):
+    """A 9-bath model (252 dn states: four columns per lane in the
+    wave-private row sweep, 210 and 252 up states) solved by one workgroup
+    with the LDS paths and by one wave without them: clean, and E0 and
+    G(i w_n) the same."""
+    models = _models((9,), per=1)
+    wn = (2 * np.arange(16) + 1) * np.pi / 50.0
+    blk, wav = tmp_path / "block.txt", tmp_path / "wave.txt"
+    _write(blk, models, wn, -1, 1, 0)
+    _write(wav, models, wn, 63504, 1, 1)
+    rb, rw = _run(harness, blk), _run(harness, wav)
+    _clean(rb)
+    _clean(rw)
+    e0b, resb, gb, _ = _parse(rb.stdout, 1, len(wn))
+    e0w, resw, gw, _ = _parse(rw.stdout, 1, len(wn))
+    assert resb[0, 1] <= 1e-9 and resw[0, 1] <= 1e-9, (resb, resw)
+    assert abs(e0b[0] - e0w[0]) <= 1e-12 * abs(e0w[0]), (e0b, e0w)
+    assert np.abs(gb - gw).max() <= 1e-9 * np.abs(gw).max()
+
+
 def test_backward_against_finite_differences(harness, tmp_path):
     """keep_basis and the adjoint backward (FP64 matrix-core products
     emulated with the instruction's lane layout): clean, and dL/d(eps_k, V_k)
