@@ -52,7 +52,7 @@ constexpr int kBlock = 1024;      // threads per workgroup of the backward kerne
 // VGPRs of 16 waves: 27-30 ms against 32-40 ms on 8 waves), the fraction phase
 // with 12 waves and wave-private rows (54-57 ms against 70 ms).
 constexpr int kBlockGround = 1024;
-constexpr bool kFastGround = false;
+constexpr bool kFastGround = true;
 constexpr int kBlockFrac = 768;
 constexpr bool kFastFrac = true;
 constexpr int kBlockSolveMax = kBlockGround > kBlockFrac ? kBlockGround : kBlockFrac;
@@ -122,6 +122,7 @@ constexpr int kLdsRowsSolve = 4096;
 constexpr int kLdsDnSolve = 4096;   // solve kernel: 48 KiB
 constexpr int kLdsUpSolve = 4096;   // 48 KiB
 constexpr int kLdsUpRowsSolve = 4096;
+constexpr int kLdsDiagSolve = 256;  // dn-sector diagonal and states: 3 KiB
 constexpr int kLdsDnBwd = 2048;     // backward kernel (its LDS also holds a W chunk): 24 KiB
 constexpr int kLdsUpBwd = 2048;     // 24 KiB
 constexpr int kLdsUpRowsBwd = 2048;
@@ -147,6 +148,11 @@ struct BlockLds {
   int u_cap;     // slots
   int u_rows;    // states
   lds_i32* umeta;
+  // the dn sector's diagonal and states (filled with the dn hop cache when
+  // the sector has at most ddn_cap states; solve kernel only)
+  lds_f64* ddn = nullptr;
+  lds_u32* sdn = nullptr;
+  int ddn_cap = 0;
 };
 
 template <bool BLOCK>
@@ -310,6 +316,11 @@ __device__ __noinline__ void cache_dn(const Group<true>& g, const SpinList& Dn) 
       for (int i = c * 64; i < Dn.n && i < c * 64 + 64; ++i) m = m > Dn.cnt[i] ? m : Dn.cnt[i];
       L.meta[2 + c] = m;
     }
+    if (Dn.n <= L.ddn_cap)
+      for (int i = g.rank; i < Dn.n; i += g.size) {
+        L.ddn[i] = Dn.dg[i];
+        L.sdn[i] = Dn.st[i];
+      }
   }
   if (g.rank == 0) {
     L.meta[0] = fit ? Dn.n : 0;
@@ -581,9 +592,12 @@ __device__ double hv_rows_wave(const Group<true>& g, const ModelParams& p, const
     }
     const double dgu = U.dg[iu];
     const uint32_t stu = (uint32_t)rfl((int)U.st[iu]);
+    const bool dl = nd <= g.lds.ddn_cap;  // the dn diagonal and states cached in LDS
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const double a = fma(Dn.dg[idc[c]] + dgu + ((stu & Dn.st[idc[c]] & 1u) ? p.u : 0.0), x[c], acc[c]);
+      const double ddn = dl ? (double)g.lds.ddn[idc[c]] : Dn.dg[idc[c]];
+      const uint32_t sdn = dl ? (uint32_t)g.lds.sdn[idc[c]] : Dn.st[idc[c]];
+      const double a = fma(ddn + dgu + ((stu & sdn & 1u) ? p.u : 0.0), x[c], acc[c]);
       const double wk = sj * a - bsp * xp[c];
       w[ro + idc[c]] = wk;
       if (lane + 64 * c < nd) dot += wk * x[c];
@@ -683,13 +697,25 @@ __device__ void lanczos_step(const Group<BLOCK>& g, const ModelParams& p, const 
   g.sync();
 }
 
+// lanczos_step as a call of its own: its registers are allocated apart from
+// the caller's (the ground-state task keeps more state live across the step
+// than the 768-thread budget leaves beside the wave-private row sweep).
+template <bool BLOCK, bool FAST, bool LEAN>
+__device__ __noinline__ void lanczos_step_call(const Group<BLOCK>& g, const ModelParams& p, const SpinList& U,
+                                               const SpinList& Dn, const double* __restrict__ u,
+                                               const double* __restrict__ uprev, double* __restrict__ w, double sj,
+                                               double sp, double bj, bool known, double a_in, double* a_out,
+                                               double* nrm2) {
+  lanczos_step<BLOCK, FAST, LEAN>(g, p, U, Dn, u, uprev, w, sj, sp, bj, known, a_in, a_out, nrm2);
+}
+
 // Lowest eigenpair of the m x m tridiagonal (alpha, beta): multisection over
 // the group's lanes on the Sturm count, then inverse iteration at a shift just
 // below the eigenvalue (T - shift is positive definite there, so the Thomas
 // solve needs no pivoting).  Writes the unit eigenvector to sv (tc is scratch)
 // and returns theta and |last component| through theta / slast.
 template <bool BLOCK>
-__device__ void tri_lowest(const Group<BLOCK>& g, const double* alpha, const double* beta, int m, double* sv,
+__device__ __noinline__ void tri_lowest(const Group<BLOCK>& g, const double* alpha, const double* beta, int m, double* sv,
                            double* tc, double* scal, double* theta, double* slast) {
   double lo = 1e300, hi = -1e300;
   for (int i = g.rank; i < m; i += g.size) {
@@ -771,10 +797,23 @@ __device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
 }
 
 // s * norm2 / (s z + e0 - a0 - b0^2 / (s z + e0 - a1 - ...)), z = i w.
+// The coefficients are loaded eight steps ahead of the recursion (the same
+// steps in the same order), so their latency is paid once per eight steps.
 __device__ cplx cf_value(const double* a, const double* b, int m, double norm2, double e0, double s, double w) {
   const cplx z = {e0, s * w};
   cplx t = {0.0, 0.0};
-  for (int j = m - 1; j >= 1; --j) t = cdiv({b[j - 1] * b[j - 1], 0.0}, {z.re - a[j] - t.re, z.im - t.im});
+  int j = m - 1;
+  for (; j >= 8; j -= 8) {
+    double aa[8], bb[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      aa[r] = a[j - r];
+      bb[r] = b[j - r - 1];
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) t = cdiv({bb[r] * bb[r], 0.0}, {z.re - aa[r] - t.re, z.im - t.im});
+  }
+  for (; j >= 1; --j) t = cdiv({b[j - 1] * b[j - 1], 0.0}, {z.re - a[j] - t.re, z.im - t.im});
   const cplx r = cdiv({s * norm2, 0.0}, {z.re - a[0] - t.re, z.im - t.im});
   return r;
 }
@@ -810,8 +849,24 @@ __device__ int cf_coeffs(const Group<BLOCK>& g, const ModelParams& p, const Spin
     b[j] = bn;
     if (m % 8 == 0) {
       g.sync();
+      // block mode: the check reads the coefficients from the LDS row
+      // buffer (idle between sweeps) instead of global memory
+      const double* ca = a;
+      const double* cb = b;
+      if constexpr (BLOCK) {
+        if (g.has_lds && 2 * m <= g.lds.rows_cap) {
+          double* const st = (double*)g.lds.rows;
+          for (int i = g.rank; i < m; i += g.size) {
+            st[i] = a[i];
+            st[m + i] = b[i];
+          }
+          g.sync();
+          ca = st;
+          cb = st + m;
+        }
+      }
       if (g.rank == 0) {
-        const cplx c = cf_value(a, b, m, n2, e0, sgn, wmin);
+        const cplx c = cf_value(ca, cb, m, n2, e0, sgn, wmin);
         // change measured against the part's spectral weight n2 as well, so a
         // small |G(i wmin)| does not make the test unreachable
         const double d = hypot(c.re - prev.re, c.im - prev.im);
@@ -842,6 +897,30 @@ __device__ __forceinline__ double start_value(uint64_t seed, int64_t k) {
 // every vector kept in the model's store, psi = sum_j sv_j v_j written over the
 // store's first vector, its explicit residual, E0 (also into the model's
 // coefficient record) and the step count.
+// psi[k] = sum_j c_j V[j D + k] over V_0 (each state's entries of every V_j
+// are read by the one member that then overwrites V_0 there), eight vectors'
+// loads in flight at a time, summed in step order.  A call of its own, so its
+// registers are allocated apart from the ground-state loop's.
+template <bool BLOCK>
+__device__ __noinline__ void combine_psi(const Group<BLOCK>& g, double* V, const double* c, int m, int64_t D) {
+  for (int64_t k = g.rank; k < D; k += g.size) {
+    double acc = 0.0;
+    int j = 0;
+    for (; j + 8 <= m; j += 8) {
+      double cv[8], vv[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        cv[r] = c[j + r];
+        vv[r] = V[(int64_t)(j + r) * D + k];
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r) acc += cv[r] * vv[r];
+    }
+    for (; j < m; ++j) acc += c[j] * V[(int64_t)j * D + k];
+    V[k] = acc;
+  }
+}
+
 template <bool BLOCK, bool FAST>
 __device__ void solve_ground(const Group<BLOCK>& g, const Config& cf, int mi, const SlotRef& sl) {
   const ModelParams& p = cf.params[mi];
@@ -880,7 +959,7 @@ __device__ void solve_ground(const Group<BLOCK>& g, const Config& cf, int mi, co
   for (int j = 0; j < cap; ++j) {
     double aj, nn;
     const double* u = V + (int64_t)j * D;
-    lanczos_step<BLOCK, FAST, true>(g, p, U, Dn, u, j > 0 ? u - D : u, V + (int64_t)(j + 1) * D, sj, sp, bj, false, 0.0, &aj, &nn);
+    lanczos_step_call<BLOCK, FAST, true>(g, p, U, Dn, u, j > 0 ? u - D : u, V + (int64_t)(j + 1) * D, sj, sp, bj, false, 0.0, &aj, &nn);
     m = j + 1;
     const double bn = sqrt(nn);
     if (g.rank == 0) {
@@ -891,7 +970,24 @@ __device__ void solve_ground(const Group<BLOCK>& g, const Config& cf, int mi, co
     const bool breakdown = !(bn > 1e-12 * (fabs(aj) + bj));
     const bool check = breakdown || m == cap || (m >= cf.check_every && m % cf.check_every == 0);
     if (check) {
-      tri_lowest<BLOCK>(g, alpha, beta, m, sl.sv(), sl.tc(), sl.scal(), &theta, &slast);
+      bool done = false;
+      if constexpr (BLOCK) {
+        // block mode: the tridiagonal solve runs on copies in the LDS row
+        // buffer (idle between sweeps); the eigenvector goes back to sv
+        if (g.has_lds && 4 * m + 8 <= g.lds.rows_cap) {
+          double* const st = (double*)g.lds.rows;
+          for (int i = g.rank; i < m; i += g.size) {
+            st[i] = alpha[i];
+            st[m + i] = beta[i];
+          }
+          g.sync();
+          tri_lowest<BLOCK>(g, st, st + m, m, st + 2 * m, st + 3 * m, st + 4 * m, &theta, &slast);
+          for (int i = g.rank; i < m; i += g.size) sl.sv()[i] = st[2 * m + i];
+          g.sync();
+          done = true;
+        }
+      }
+      if (!done) tri_lowest<BLOCK>(g, alpha, beta, m, sl.sv(), sl.tc(), sl.scal(), &theta, &slast);
       ritz = breakdown ? 0.0 : bn * slast;
       if (ritz <= cf.tol || breakdown) break;
     }
@@ -906,11 +1002,7 @@ __device__ void solve_ground(const Group<BLOCK>& g, const Config& cf, int mi, co
     double* const c = sl.tc();
     for (int j = g.rank; j < m; j += g.size) c[j] = sl.sv()[j] * (j == 0 ? s0 : 1.0 / beta[j - 1]);
     g.sync();
-    for (int64_t k = g.rank; k < D; k += g.size) {
-      double acc = 0.0;
-      for (int j = 0; j < m; ++j) acc += c[j] * V[(int64_t)j * D + k];
-      psi[k] = acc;
-    }
+    combine_psi<BLOCK>(g, V, c, m, D);
   }
   g.sync();
   // normalise psi and take the explicit residual |H psi - theta psi|
@@ -928,7 +1020,7 @@ __device__ void solve_ground(const Group<BLOCK>& g, const Config& cf, int mi, co
     double aj, nn;
     // w = H psi - theta psi (unit scale, no previous vector, known coefficient)
     double* wr = sl.vec(2);
-    lanczos_step<BLOCK, FAST, true>(g, p, U, Dn, psi, sl.vec(0), wr, 1.0, 0.0, 0.0, true, theta, &aj, &nn);
+    lanczos_step_call<BLOCK, FAST, true>(g, p, U, Dn, psi, sl.vec(0), wr, 1.0, 0.0, 0.0, true, theta, &aj, &nn);
     double r2 = 0.0;
     for (int64_t k = g.rank; k < D; k += g.size) r2 += wr[k] * wr[k];
     res2 = g.sum(r2);
@@ -1046,10 +1138,13 @@ __global__ __launch_bounds__(PHASE ? kBlockFrac : kBlockGround) void k_batch_sol
     __shared__ double s_uval[kLdsUpSolve];
     __shared__ int32_t s_ucnt[kLdsUpRowsSolve];
     __shared__ int32_t s_umeta[4];
+    __shared__ double s_ddn[kLdsDiagSolve];
+    __shared__ uint32_t s_sdn[kLdsDiagSolve];
     Group<true> g{(int)threadIdx.x, (int)blockDim.x, red};
     g.lds = BlockLds{(lds_f64*)s_rows, kLdsRowsSolve,     (lds_u32*)s_tgt,  (lds_f64*)s_val,
                      kLdsDnSolve,      (lds_i32*)s_meta,  (lds_u32*)s_utgt, (lds_f64*)s_uval,
-                     (lds_i32*)s_ucnt, kLdsUpSolve,       kLdsUpRowsSolve,  (lds_i32*)s_umeta};
+                     (lds_i32*)s_ucnt, kLdsUpSolve,       kLdsUpRowsSolve,  (lds_i32*)s_umeta,
+                     (lds_f64*)s_ddn,  (lds_u32*)s_sdn,   kLdsDiagSolve};
     if (threadIdx.x == 0) s_umeta[0] = 0;
     g.has_lds = cf.staged != 0;
     g.wave_rows = cf.staged == 1;
