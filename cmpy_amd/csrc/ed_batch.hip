@@ -1209,6 +1209,25 @@ __global__ void k_batch_g(Config cf) {
 // One workgroup per model; no orthogonality of Q is assumed (everything is
 // evaluated through K in the sector's state space).
 // ===========================================================================
+// A continued-fraction part whose K products run device-wide (its Krylov
+// basis is long): P = Qt Re W by the grouped matrix-core GEMM, then diag(K)
+// and the bond sums of K over 64-row blocks.
+struct BwdJob {
+  int32_t model, part, m, dt;  // dt: states of the part's sector
+  int64_t q_off;               // Krylov basis in the basis store (rows of stride kcap)
+  int64_t w_off;               // Re W (m x m, row-major) in the job store
+  int64_t p_off;               // P (dt x m, row-major) in the job store
+  int32_t blk0, nblk;          // its 64-row blocks in the block list
+};
+struct BwdTile {  // a 64 x 64 tile of one job's P
+  int32_t job, r0, c0, pad;
+};
+struct BwdBlock {  // 64 rows of one job's sector
+  int32_t job, r0;
+};
+constexpr int kRecPart = 1 + 2 * kMaxBath;           // per part: tr K, S_eps[k], S_v[k] (before the part's sign)
+constexpr int kRec = 2 * kRecPart + 4 * kMaxBath;    // + dE0/deps_k, psi dH/deps_k zeta, dE0/dv_k, psi dH/dv_k zeta
+
 struct BwdConfig {
   const ModelParams* params;
   int32_t num_models;
@@ -1232,6 +1251,13 @@ struct BwdConfig {
   double* grad_eps;   // [num_models][max_bath]
   double* grad_v;     // [num_models][max_bath]
   int32_t* cg_iters;  // [num_models]
+  // parts whose K products run device-wide (job_of[2 model + part] >= 0)
+  const int32_t* job_of;
+  const BwdJob* jobs;
+  const BwdBlock* blocks;
+  double* jstore;
+  double* rec;        // [num_models][kRec]
+  double* partial;    // [blocks][kRecPart]
 };
 
 // Backward slot: the four spin lists (same layout as SlotRef), then
@@ -1289,6 +1315,7 @@ __device__ __forceinline__ double bond_hops(const SpinList& U, const SpinList& D
 constexpr int kPc = 16;             // W columns per LDS chunk in the backward
 constexpr int kBwdLds = 512 * kPc;  // doubles of the backward's LDS staging buffer (64 KiB)
 constexpr int kBwdMaxSteps = 1024;  // continued-fraction steps a kept basis may hold
+constexpr int kBwdSplitM = 128;     // parts with this many steps run their K products device-wide
 
 __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi, const BwdSlot& bs, double* wl) {
   const ModelParams& p = bc.params[mi];
@@ -1311,14 +1338,13 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
   const double* co = bc.store_coef + (int64_t)mi * (4 * kcap + 8);
   const double e0 = co[4 * kcap];
   const double* psi = bc.store_psi + bc.off_psi[mi];
-  double* scal = bs.scal(nw);
-  // scal layout: [0..15] misc, [16 .. 16+nb) S_eps, [32 .. 32+nb) S_v
-  if (g.rank < 64) scal[g.rank] = 0.0;
+  double* const rec = bc.rec + (int64_t)mi * kRec;
+  for (int t = g.rank; t < kRec; t += g.size) rec[t] = 0.0;
   g.sync();
-  double trK0 = 0.0, trK1 = 0.0;  // trace of K of the particle and hole parts
   for (int part = 0; part < 2; ++part) {
     const int m = (int)co[4 * kcap + 3 + part];
     if (m <= 0) continue;
+    const int job = bc.job_of[2 * mi + part];
     const double sigma = part == 0 ? 1.0 : -1.0;
     const double* a = co + (part == 0 ? 0 : 2 * kcap);
     const double* b = co + (part == 0 ? kcap : 3 * kcap);
@@ -1467,6 +1493,16 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
       Xr[r] = xr;
       Xi[r] = xi;
     }
+    if (job >= 0) {
+      // long basis: Re W to the job store; P and the K sums run device-wide
+      double* const wj = bc.jstore + bc.jobs[job].w_off;
+      for (int64_t t = g.rank; t < (int64_t)m * m; t += g.size) {
+        const int i = (int)(t / m), j = (int)(t - (int64_t)i * m);
+        wj[t] = W[(int64_t)i * kcap + j];
+      }
+      g.sync();
+      continue;
+    }
     // P = Qt Re W with FP64 matrix cores (16 x 16 x 4), in groups of at most
     // one 32-column block per wave: within a group starting at column c0,
     // wave (wr, wc) accumulates the 32 x 32 block of P at rows
@@ -1550,10 +1586,7 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
       tr += acc;
     }
     tr = g.sum(tr);
-    if (part == 0)
-      trK0 = tr;
-    else
-      trK1 = tr;
+    if (g.rank == 0) rec[part * kRecPart] = tr;
     g.sync();
     for (int k = 0; k < nb; ++k) {
       double se = 0.0, sv = 0.0;
@@ -1569,8 +1602,8 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
       se = g.sum(se);
       sv = g.sum(sv);
       if (g.rank == 0) {
-        scal[16 + k] += sigma * se;
-        scal[32 + k] += sigma * sv;
+        rec[part * kRecPart + 1 + k] = se;
+        rec[part * kRecPart + 1 + kMaxBath + k] = sv;
       }
       g.sync();
     }
@@ -1650,9 +1683,11 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
     dv = g.sum(dv);
     tv = g.sum(tv);
     if (g.rank == 0) {
-      const double ba = trK1 - trK0;
-      bc.grad_eps[(int64_t)mi * bc.max_bath + k] = scal[16 + k] + ba * de - 2.0 * te;
-      bc.grad_v[(int64_t)mi * bc.max_bath + k] = scal[32 + k] + ba * dv - 2.0 * tv;
+      double* const rt = rec + 2 * kRecPart;
+      rt[k] = de;
+      rt[kMaxBath + k] = te;
+      rt[2 * kMaxBath + k] = dv;
+      rt[3 * kMaxBath + k] = tv;
     }
   }
   if (g.rank == 0) bc.cg_iters[mi] = it;
@@ -1686,6 +1721,180 @@ __global__ __launch_bounds__(kBlock) void k_batch_backward(BwdConfig bc) {
     __syncthreads();
     if (q >= bc.num_models) break;
     backward_model(g, bc, q, bs, wl);
+  }
+}
+
+// P = Qt Re W (Re W symmetric, so Qt Re W^T) of every device-wide job: one
+// 64 x 64 tile of one job's P per block from the tile list, each of the four
+// waves a 32 x 32 quarter as 2 x 2 v_mfma_f64_16x16x4 tiles, 16-wide K slabs
+// staged in LDS (zero past the job's rows, columns and m).  Operand lanes:
+// lane l gives A[l % 16][l / 16] and B^T[l / 16][l % 16] of a 16x4 / 4x16
+// step; result element r of lane l is C[l / 16 + 4 r][l % 16].
+__global__ __launch_bounds__(256) void k_bwd_p(const BwdJob* __restrict__ jobs, const BwdTile* __restrict__ tiles,
+                                               const double* __restrict__ store_q, int kcap,
+                                               double* __restrict__ jstore) {
+  __shared__ double As[64][16 + 1];
+  __shared__ double Bs[64][16 + 1];
+  typedef double f64x4 __attribute__((ext_vector_type(4)));
+  const BwdTile tl = tiles[blockIdx.x];
+  const BwdJob jb = jobs[tl.job];
+  const double* __restrict__ A = store_q + jb.q_off;  // dt x m, row stride kcap
+  const double* __restrict__ B = jstore + jb.w_off;   // m x m, row stride m
+  double* __restrict__ Cm = jstore + jb.p_off;        // dt x m, row stride m
+  const int64_t M = jb.dt;
+  const int N = jb.m, K = jb.m;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  const int64_t m0 = tl.r0;
+  const int n0 = tl.c0;
+  f64x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int i = threadIdx.x; i < 64 * 16; i += 256) {
+      const int row = i >> 4, kk = i & 15;
+      const int kg = k0 + kk;
+      As[row][kk] = (kg < K && m0 + row < M) ? A[(m0 + row) * kcap + kg] : 0.0;
+      Bs[row][kk] = (kg < K && n0 + row < N) ? B[(int64_t)(n0 + row) * K + kg] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 16; ks += 4) {
+      const double a0 = As[wm + li][ks + lk], a1 = As[wm + 16 + li][ks + lk];
+      const double b0 = Bs[wn + li][ks + lk], b1 = Bs[wn + 16 + li][ks + lk];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm + ti * 16 + lk + 4 * r;
+        const int col = n0 + wn + tj * 16 + li;
+        if (row < M && col < N) Cm[row * N + col] = acc[ti][tj][r];
+      }
+}
+
+// <x, y> over n entries, lanes striding the entries, the same value on every
+// lane (fixed order).
+__device__ __forceinline__ double wave_dot(const double* __restrict__ x, const double* __restrict__ y, int n) {
+  double s = 0.0;
+  for (int j = (int)(threadIdx.x & 63); j < n; j += 64) s += x[j] * y[j];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  return s;
+}
+
+// diag(K) = rows of P . rows of Qt, its trace, S_eps[k] = sum occ_k diag(K)
+// and S_v[k] = sum over the hops of bond k of K[a, t] (with the hop signs)
+// over one 64-row block of one job: a wave per row in turn, states and hops
+// regenerated from the row index (the same hops as the spin lists hold);
+// the block's sums, wave by wave in order, go to its partial record.
+__global__ __launch_bounds__(256) void k_bwd_sums(BwdConfig bc) {
+  __shared__ double red[4][kRecPart];
+  const BwdBlock bl = bc.blocks[blockIdx.x];
+  const BwdJob jb = bc.jobs[bl.job];
+  const ModelParams& p = bc.params[jb.model];
+  const int ns = p.nb + 1, nb = p.nb;
+  const int n_t = p.n_up + (jb.part == 0 ? 1 : -1);
+  const int nd = (int)binom_u(ns, p.n_dn);
+  const int m = jb.m, kcap = bc.kcap;
+  const double* __restrict__ P = bc.jstore + jb.p_off;
+  const double* __restrict__ Q = bc.store_q + jb.q_off;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double acc[kRecPart];  // uniform over the wave
+#pragma unroll
+  for (int t = 0; t < kRecPart; ++t) acc[t] = 0.0;
+  for (int rr = wave; rr < 64; rr += 4) {
+    const int64_t r = (int64_t)bl.r0 + rr;
+    if (r >= jb.dt) break;
+    const int it = (int)(r / nd), id = (int)(r - (int64_t)it * nd);
+    const uint32_t st = comb_unrank(it, ns, n_t), sd = comb_unrank(id, ns, p.n_dn);
+    const double* pr = P + r * m;
+    const double dg = wave_dot(pr, Q + r * kcap, m);
+    acc[0] += dg;
+    for (int k = 0; k < nb; ++k) {
+      acc[1 + k] += occ_k(st, sd, k) * dg;
+      const uint32_t mk = 1u | (2u << k);
+      double sv = 0.0;
+      const uint32_t bu = st & mk;
+      if (bu != 0 && bu != mk) {
+        const double sg = hop_sign(st, bc.sign_width, 0, k + 1) < 0 ? -1.0 : 1.0;
+        sv += sg * wave_dot(pr, Q + ((int64_t)comb_rank(st ^ mk) * nd + id) * kcap, m);
+      }
+      const uint32_t bd = sd & mk;
+      if (bd != 0 && bd != mk) {
+        const double sg = hop_sign(sd, bc.sign_width, 0, k + 1) < 0 ? -1.0 : 1.0;
+        sv += sg * wave_dot(pr, Q + ((int64_t)it * nd + comb_rank(sd ^ mk)) * kcap, m);
+      }
+      acc[1 + kMaxBath + k] += sv;
+    }
+  }
+  if (lane == 0)
+    for (int t = 0; t < kRecPart; ++t) red[wave][t] = acc[t];
+  __syncthreads();
+  for (int t = threadIdx.x; t < kRecPart; t += blockDim.x)
+    bc.partial[(int64_t)blockIdx.x * kRecPart + t] = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+}
+
+// Gradient of every model from its record and, for its device-wide parts,
+// the blocks' partial sums (in block order):
+//   dL/deps_k = sum_parts s_part S_eps[k] + (trK_hole - trK_particle) dE0/deps_k - 2 psi dH/deps_k zeta,
+// the same for v_k with S_v.
+__global__ void k_bwd_combine(BwdConfig bc) {
+  const int mi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mi >= bc.num_models) return;
+  const ModelParams& p = bc.params[mi];
+  const double* rec = bc.rec + (int64_t)mi * kRec;
+  const double* rt = rec + 2 * kRecPart;
+  const double* co = bc.store_coef + (int64_t)mi * (4 * bc.kcap + 8);
+  double tr[2] = {0.0, 0.0};
+  for (int part = 0; part < 2; ++part) {
+    if ((int)co[4 * bc.kcap + 3 + part] <= 0) continue;
+    const int job = bc.job_of[2 * mi + part];
+    if (job < 0) {
+      tr[part] = rec[part * kRecPart];
+    } else {
+      const BwdJob jb = bc.jobs[job];
+      double t = 0.0;
+      for (int b = 0; b < jb.nblk; ++b) t += bc.partial[(int64_t)(jb.blk0 + b) * kRecPart];
+      tr[part] = t;
+    }
+  }
+  for (int k = 0; k < p.nb; ++k) {
+    double se = 0.0, sv = 0.0;
+    for (int part = 0; part < 2; ++part) {
+      if ((int)co[4 * bc.kcap + 3 + part] <= 0) continue;
+      const double sigma = part == 0 ? 1.0 : -1.0;
+      const int job = bc.job_of[2 * mi + part];
+      double e, v;
+      if (job < 0) {
+        e = rec[part * kRecPart + 1 + k];
+        v = rec[part * kRecPart + 1 + kMaxBath + k];
+      } else {
+        const BwdJob jb = bc.jobs[job];
+        e = 0.0;
+        v = 0.0;
+        for (int b = 0; b < jb.nblk; ++b) {
+          e += bc.partial[(int64_t)(jb.blk0 + b) * kRecPart + 1 + k];
+          v += bc.partial[(int64_t)(jb.blk0 + b) * kRecPart + 1 + kMaxBath + k];
+        }
+      }
+      se += sigma * e;
+      sv += sigma * v;
+    }
+    const double ba = tr[1] - tr[0];
+    bc.grad_eps[(int64_t)mi * bc.max_bath + k] = se + ba * rt[k] - 2.0 * rt[kMaxBath + k];
+    bc.grad_v[(int64_t)mi * bc.max_bath + k] = sv + ba * rt[2 * kMaxBath + k] - 2.0 * rt[3 * kMaxBath + k];
   }
 }
 
@@ -1739,6 +1948,14 @@ struct ed_batch {
   double *d_gre = nullptr, *d_gim = nullptr, *d_grad_eps = nullptr, *d_grad_v = nullptr;
   int32_t* d_cg = nullptr;
   int32_t* d_bcounter = nullptr;
+  // backward: parts whose K products run device-wide (tables and stores)
+  std::vector<int64_t> h_off_q;
+  int32_t* d_job_of = nullptr;
+  BwdJob* d_jobs = nullptr;
+  BwdTile* d_tiles = nullptr;
+  BwdBlock* d_blocks = nullptr;
+  double *d_jstore = nullptr, *d_rec = nullptr, *d_partial = nullptr;
+  int64_t cap_jobs = 0, cap_tiles = 0, cap_blocks = 0, cap_jstore = 0;
   char* d_bws = nullptr;
   int64_t bws_bytes = 0;
   int32_t solved_nw = -1;  // frequencies of the last solve (the backward uses them)
@@ -1768,7 +1985,9 @@ static void free_batch(ed_batch* b) {
                   (void*)b->d_resid, (void*)b->d_wn, (void*)b->d_iters, (void*)b->d_ws, (void*)b->d_off_psi,
                   (void*)b->d_off_q, (void*)b->d_store_psi, (void*)b->d_store_q, (void*)b->d_store_coef,
                   (void*)b->d_gre, (void*)b->d_gim, (void*)b->d_grad_eps, (void*)b->d_grad_v, (void*)b->d_cg,
-                  (void*)b->d_bcounter, (void*)b->d_bws, (void*)b->d_gs_store, (void*)b->d_off_gs})
+                  (void*)b->d_bcounter, (void*)b->d_bws, (void*)b->d_gs_store, (void*)b->d_off_gs, (void*)b->d_job_of,
+                  (void*)b->d_jobs, (void*)b->d_tiles, (void*)b->d_blocks, (void*)b->d_jstore, (void*)b->d_rec,
+                  (void*)b->d_partial})
     if (p) (void)hipFree(p);
   delete b;
 }
@@ -2080,10 +2299,12 @@ extern "C" int ed_batch_create(ed_batch** out, const ed_batch_desc* d, void* str
     if ((rc = balloc(&b->d_off_psi, M)) != ED_OK || (rc = balloc(&b->d_off_q, 2 * M)) != ED_OK ||
         (rc = balloc(&b->d_store_psi, np)) != ED_OK || (rc = balloc(&b->d_store_q, nq)) != ED_OK ||
         (rc = balloc(&b->d_grad_eps, M * d->max_bath)) != ED_OK || (rc = balloc(&b->d_grad_v, M * d->max_bath)) != ED_OK ||
-        (rc = balloc(&b->d_cg, M)) != ED_OK || (rc = balloc(&b->d_bcounter, 1)) != ED_OK) {
+        (rc = balloc(&b->d_cg, M)) != ED_OK || (rc = balloc(&b->d_bcounter, 1)) != ED_OK ||
+        (rc = balloc(&b->d_rec, M * kRec)) != ED_OK || (rc = balloc(&b->d_job_of, 2 * M)) != ED_OK) {
       free_batch(b);
       return rc;
     }
+    b->h_off_q = off_q;
     if (hipMemcpyAsync(b->d_off_psi, off_psi.data(), sizeof(int64_t) * M, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(b->d_off_q, off_q.data(), sizeof(int64_t) * 2 * M, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
@@ -2302,6 +2523,69 @@ extern "C" int ed_batch_backward(ed_batch* b, const double* dl_dre, const double
   }
   ED_HIP(hipMemcpyAsync(b->d_gre, dl_dre, sizeof(double) * M * nw, hipMemcpyHostToDevice, s));
   ED_HIP(hipMemcpyAsync(b->d_gim, dl_dim, sizeof(double) * M * nw, hipMemcpyHostToDevice, s));
+  // parts with at least split_m continued-fraction steps run their K
+  // products device-wide (CMPY_AMD_BWD_SPLIT_M; the per-model workgroup
+  // would spend O(D m^2) on them alone)
+  const int split_m = env_flag("CMPY_AMD_BWD_SPLIT_M", kBwdSplitM);
+  std::vector<int32_t> iters(3 * M);
+  ED_HIP(hipMemcpy(iters.data(), b->d_iters, sizeof(int32_t) * 3 * M, hipMemcpyDeviceToHost));
+  std::vector<int32_t> job_of(2 * M, -1);
+  std::vector<BwdJob> jobs;
+  std::vector<BwdTile> tiles;
+  std::vector<BwdBlock> blocks;
+  int64_t jst = 0;
+  for (int64_t i = 0; i < M; ++i) {
+    const ModelParams& p = b->params[i];
+    const int ns = p.nb + 1;
+    for (int part = 0; part < 2; ++part) {
+      const int m = iters[3 * i + 1 + part];
+      const int n_t = p.n_up + (part == 0 ? 1 : -1);
+      if (m <= 0 || m < split_m || n_t < 0 || n_t > ns) continue;
+      BwdJob jb{};
+      jb.model = (int32_t)i;
+      jb.part = part;
+      jb.m = m;
+      jb.dt = (int32_t)sector_dim(ns, n_t, p.n_dn);
+      jb.q_off = b->h_off_q[2 * i + part];
+      jb.w_off = jst;
+      jst += (int64_t)m * m;
+      jb.p_off = jst;
+      jst += (int64_t)jb.dt * m;
+      jb.blk0 = (int32_t)blocks.size();
+      jb.nblk = (jb.dt + 63) / 64;
+      const int32_t j = (int32_t)jobs.size();
+      for (int32_t r0 = 0; r0 < jb.dt; r0 += 64) {
+        blocks.push_back(BwdBlock{j, r0});
+        for (int32_t c0 = 0; c0 < m; c0 += 64) tiles.push_back(BwdTile{j, r0, c0, 0});
+      }
+      job_of[2 * i + part] = j;
+      jobs.push_back(jb);
+    }
+  }
+  auto grow = [](auto** ptr, int64_t* cap, int64_t n) -> int {
+    if (n <= *cap) return ED_OK;
+    if (*ptr) (void)hipFree(*ptr);
+    *ptr = nullptr;
+    *cap = 0;
+    ED_TRY(balloc(ptr, n));
+    *cap = n;
+    return ED_OK;
+  };
+  ED_TRY(grow(&b->d_jobs, &b->cap_jobs, (int64_t)jobs.size()));
+  ED_TRY(grow(&b->d_tiles, &b->cap_tiles, (int64_t)tiles.size()));
+  if ((int64_t)blocks.size() > b->cap_blocks) {
+    if (b->d_partial) (void)hipFree(b->d_partial);
+    b->d_partial = nullptr;
+    ED_TRY(grow(&b->d_blocks, &b->cap_blocks, (int64_t)blocks.size()));
+    ED_TRY(balloc(&b->d_partial, b->cap_blocks * kRecPart));
+  }
+  ED_TRY(grow(&b->d_jstore, &b->cap_jstore, jst));
+  ED_HIP(hipMemcpy(b->d_job_of, job_of.data(), sizeof(int32_t) * 2 * M, hipMemcpyHostToDevice));
+  if (!jobs.empty()) {
+    ED_HIP(hipMemcpy(b->d_jobs, jobs.data(), sizeof(BwdJob) * jobs.size(), hipMemcpyHostToDevice));
+    ED_HIP(hipMemcpy(b->d_tiles, tiles.data(), sizeof(BwdTile) * tiles.size(), hipMemcpyHostToDevice));
+    ED_HIP(hipMemcpy(b->d_blocks, blocks.data(), sizeof(BwdBlock) * blocks.size(), hipMemcpyHostToDevice));
+  }
   BwdConfig bc{};
   bc.params = b->d_params;
   bc.num_models = (int32_t)M;
@@ -2328,12 +2612,24 @@ extern "C" int ed_batch_backward(ed_batch* b, const double* dl_dre, const double
   bc.grad_eps = b->d_grad_eps;
   bc.grad_v = b->d_grad_v;
   bc.cg_iters = b->d_cg;
+  bc.job_of = b->d_job_of;
+  bc.jobs = b->d_jobs;
+  bc.blocks = b->d_blocks;
+  bc.jstore = b->d_jstore;
+  bc.rec = b->d_rec;
+  bc.partial = b->d_partial;
   hipEvent_t e0, e1;
   ED_HIP(hipEventCreate(&e0));
   ED_HIP(hipEventCreate(&e1));
   (void)hipEventRecord(e0, s);
   hipLaunchKernelGGL(k_reset_one, dim3(1), dim3(64), 0, s, b->d_bcounter);
   hipLaunchKernelGGL(k_batch_backward, dim3((unsigned)grid), dim3(kBlock), 0, s, bc);
+  if (!jobs.empty()) {
+    hipLaunchKernelGGL(k_bwd_p, dim3((unsigned)tiles.size()), dim3(256), 0, s, (const BwdJob*)b->d_jobs,
+                       (const BwdTile*)b->d_tiles, (const double*)b->d_store_q, (int)b->kcap, b->d_jstore);
+    hipLaunchKernelGGL(k_bwd_sums, dim3((unsigned)blocks.size()), dim3(256), 0, s, bc);
+  }
+  hipLaunchKernelGGL(k_bwd_combine, dim3((unsigned)((M + 63) / 64)), dim3(64), 0, s, bc);
   const hipError_t el = hipGetLastError();
   (void)hipEventRecord(e1, s);
   const hipError_t es = hipEventSynchronize(e1);

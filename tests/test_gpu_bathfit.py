@@ -114,7 +114,7 @@ def _fd_grads(b, wn, target, eb, v, h=1e-5):
     return fd
 
 
-def test_backward_long_continued_fractions(gpu):
+def test_backward_long_continued_fractions(gpu, monkeypatch):
     """Two models of the bath-fit workload's draw (seed 101, indices 130 and
     242) whose particle fractions need 590 and 660 steps at beta = 100: past
     the 512 columns one wave group of the backward's Krylov product covers,
@@ -139,6 +139,13 @@ def test_backward_long_continued_fractions(gpu):
     assert res.cf_steps.max() > 512, res.cf_steps  # two wave groups of the Krylov product
     assert not res.cf_capped.any(), res.cf_steps
     ge, gv, cg, _ = b.backward(gr, gi)
+    # these parts' K products run device-wide by default (>= 128 steps); the
+    # per-model workgroup path gives the same gradient
+    monkeypatch.setenv("CMPY_AMD_BWD_SPLIT_M", "100000")
+    ge1, gv1, _, _ = b.backward(gr, gi)
+    monkeypatch.delenv("CMPY_AMD_BWD_SPLIT_M")
+    ref = np.abs(np.concatenate([ge1, gv1], axis=1)).max()
+    assert np.abs(np.concatenate([ge - ge1, gv - gv1], axis=1)).max() <= 1e-10 * ref
     _check(np.concatenate([ge, gv], axis=1), _fd_grads(b, wn, target, eb, v), "fd")
 
 
