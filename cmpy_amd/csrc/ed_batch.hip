@@ -1215,11 +1215,18 @@ __global__ void k_batch_g(Config cf) {
 struct BwdJob {
   int32_t model, part, m, dt;  // dt: states of the part's sector
   int64_t q_off;               // Krylov basis in the basis store (rows of stride kcap)
-  int64_t w_off;               // Re W (m x m, row-major) in the job store
+  int64_t a_off, b_off;        // A = (Re V, -Im V) and B = (Re U, Im U) interleaved, m x 2 nw each
+  int64_t w_off;               // Re W = A B^T (m x m, row-major) in the job store
   int64_t p_off;               // P (dt x m, row-major) in the job store
   int32_t blk0, nblk;          // its 64-row blocks in the block list
 };
-struct BwdTile {  // a 64 x 64 tile of one job's P
+// One product C = A B^T of the backward's device-wide stage (row-major,
+// C with row stride N); A from the basis store (a_in_q) or the job store.
+struct BwdGemm {
+  int64_t a_off, b_off, c_off;
+  int32_t lda, ldb, M, N, K, a_in_q;
+};
+struct BwdTile {  // a 64 x 64 tile of one product
   int32_t job, r0, c0, pad;
 };
 struct BwdBlock {  // 64 rows of one job's sector
@@ -1317,6 +1324,25 @@ constexpr int kBwdLds = 512 * kPc;  // doubles of the backward's LDS staging buf
 constexpr int kBwdMaxSteps = 1024;  // continued-fraction steps a kept basis may hold
 constexpr int kBwdSplitM = 128;     // parts with this many steps run their K products device-wide
 
+// sum_j x[j] y[j] over j < n in order, eight entries' loads in flight at a
+// time (the same sum as one term at a time).
+__device__ __forceinline__ double row_dot(const double* __restrict__ x, const double* __restrict__ y, int n) {
+  double acc = 0.0;
+  int j = 0;
+  for (; j + 8 <= n; j += 8) {
+    double a[8], b[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      a[r] = x[j + r];
+      b[r] = y[j + r];
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) acc += a[r] * b[r];
+  }
+  for (; j < n; ++j) acc += x[j] * y[j];
+  return acc;
+}
+
 __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi, const BwdSlot& bs, double* wl) {
   const ModelParams& p = bc.params[mi];
   const int ns = p.nb + 1;
@@ -1397,7 +1423,22 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
     double* ub = bs.ubar(nw);
     const double* gre = bc.gre + (int64_t)mi * nw;
     const double* gim = bc.gim + (int64_t)mi * nw;
-    {
+    if (job >= 0) {
+      // long basis: A and B of Re W = A B^T to the job store (the product
+      // and everything after it run device-wide)
+      const BwdJob& jb = bc.jobs[job];
+      double* const Aj = bc.jstore + jb.a_off;
+      double* const Bj = bc.jstore + jb.b_off;
+      for (int64_t t = g.rank; t < (int64_t)m * nw; t += g.size) {
+        const int i = (int)(t / nw), n = (int)(t - (int64_t)i * nw);
+        const double ur = Uc[ux(n, i)], ui = Uc[ux(n, i) + 1];
+        const double vr = gre[n] * ur + gim[n] * ui, vi = gre[n] * ui - gim[n] * ur;
+        Aj[2 * t] = vr;
+        Aj[2 * t + 1] = -vi;
+        Bj[2 * t] = ur;
+        Bj[2 * t + 1] = ui;
+      }
+    } else {
       const int nbk = (m + 3) / 4;
       const int ntile = nbk * (nbk + 1) / 2;
       const int ncap = kBwdLds / (4 * m);
@@ -1484,9 +1525,25 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
     double* Xr = bs.vec(part == 0 ? 4 : 6);
     double* Xi = bs.vec(part == 0 ? 5 : 7);
     for (int64_t r = g.rank; r < Dt; r += g.size) {
+      // eight entries' loads in flight at a time, summed in order
       const double* q = qt + r * kcap;
       double xr = 0.0, xi = 0.0;
-      for (int i = 0; i < m; ++i) {
+      int i = 0;
+      for (; i + 8 <= m; i += 8) {
+        double qv[8], ur[8], ui[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          qv[t] = q[i + t];
+          ur[t] = ub[2 * (i + t)];
+          ui[t] = ub[2 * (i + t) + 1];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          xr += qv[t] * ur[t];
+          xi += qv[t] * ui[t];
+        }
+      }
+      for (; i < m; ++i) {
         xr += q[i] * ub[2 * i];
         xi += q[i] * ub[2 * i + 1];
       }
@@ -1494,12 +1551,6 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
       Xi[r] = xi;
     }
     if (job >= 0) {
-      // long basis: Re W to the job store; P and the K sums run device-wide
-      double* const wj = bc.jstore + bc.jobs[job].w_off;
-      for (int64_t t = g.rank; t < (int64_t)m * m; t += g.size) {
-        const int i = (int)(t / m), j = (int)(t - (int64_t)i * m);
-        wj[t] = W[(int64_t)i * kcap + j];
-      }
       g.sync();
       continue;
     }
@@ -1580,8 +1631,7 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
     double* dg = bs.vec(8);
     double tr = 0.0;
     for (int64_t r = g.rank; r < Dt; r += g.size) {
-      double acc = 0.0;
-      for (int j = 0; j < m; ++j) acc += P[r * kcap + j] * qt[r * kcap + j];
+      const double acc = row_dot(P + r * kcap, qt + r * kcap, m);
       dg[r] = acc;
       tr += acc;
     }
@@ -1593,11 +1643,8 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
       for (int64_t r = g.rank; r < Dt; r += g.size) {
         const int iu = (int)(r / nd), id = (int)(r - (int64_t)iu * nd);
         se += occ_k(T.st[iu], Dn.st[id], k) * dg[r];
-        sv += bond_hops(T, Dn, r, k, [&](int64_t aa, int64_t bb) {
-          double acc = 0.0;
-          for (int j = 0; j < m; ++j) acc += P[aa * kcap + j] * qt[bb * kcap + j];
-          return acc;
-        });
+        sv += bond_hops(T, Dn, r, k,
+                        [&](int64_t aa, int64_t bb) { return row_dot(P + aa * kcap, qt + bb * kcap, m); });
       }
       se = g.sum(se);
       sv = g.sum(sv);
@@ -1724,25 +1771,25 @@ __global__ __launch_bounds__(kBlock) void k_batch_backward(BwdConfig bc) {
   }
 }
 
-// P = Qt Re W (Re W symmetric, so Qt Re W^T) of every device-wide job: one
-// 64 x 64 tile of one job's P per block from the tile list, each of the four
+// The backward's device-wide products C = A B^T (Re W = A B^T, then
+// P = Qt Re W = Qt Re W^T, Re W being symmetric): one 64 x 64 tile of one
+// product per block from the tile list, each of the four
 // waves a 32 x 32 quarter as 2 x 2 v_mfma_f64_16x16x4 tiles, 16-wide K slabs
 // staged in LDS (zero past the job's rows, columns and m).  Operand lanes:
 // lane l gives A[l % 16][l / 16] and B^T[l / 16][l % 16] of a 16x4 / 4x16
 // step; result element r of lane l is C[l / 16 + 4 r][l % 16].
-__global__ __launch_bounds__(256) void k_bwd_p(const BwdJob* __restrict__ jobs, const BwdTile* __restrict__ tiles,
-                                               const double* __restrict__ store_q, int kcap,
-                                               double* __restrict__ jstore) {
+__global__ __launch_bounds__(256) void k_bwd_gemm(const BwdGemm* __restrict__ gemms, const BwdTile* __restrict__ tiles,
+                                                  const double* __restrict__ store_q, double* __restrict__ jstore) {
   __shared__ double As[64][16 + 1];
   __shared__ double Bs[64][16 + 1];
   typedef double f64x4 __attribute__((ext_vector_type(4)));
   const BwdTile tl = tiles[blockIdx.x];
-  const BwdJob jb = jobs[tl.job];
-  const double* __restrict__ A = store_q + jb.q_off;  // dt x m, row stride kcap
-  const double* __restrict__ B = jstore + jb.w_off;   // m x m, row stride m
-  double* __restrict__ Cm = jstore + jb.p_off;        // dt x m, row stride m
-  const int64_t M = jb.dt;
-  const int N = jb.m, K = jb.m;
+  const BwdGemm gm = gemms[tl.job];
+  const double* __restrict__ A = (gm.a_in_q ? store_q : jstore) + gm.a_off;
+  const double* __restrict__ B = jstore + gm.b_off;
+  double* __restrict__ Cm = jstore + gm.c_off;
+  const int64_t M = gm.M;
+  const int N = gm.N, K = gm.K, lda = gm.lda, ldb = gm.ldb;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
@@ -1757,8 +1804,8 @@ __global__ __launch_bounds__(256) void k_bwd_p(const BwdJob* __restrict__ jobs, 
     for (int i = threadIdx.x; i < 64 * 16; i += 256) {
       const int row = i >> 4, kk = i & 15;
       const int kg = k0 + kk;
-      As[row][kk] = (kg < K && m0 + row < M) ? A[(m0 + row) * kcap + kg] : 0.0;
-      Bs[row][kk] = (kg < K && n0 + row < N) ? B[(int64_t)(n0 + row) * K + kg] : 0.0;
+      As[row][kk] = (kg < K && m0 + row < M) ? A[(m0 + row) * lda + kg] : 0.0;
+      Bs[row][kk] = (kg < K && n0 + row < N) ? B[(int64_t)(n0 + row) * ldb + kg] : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -1952,10 +1999,11 @@ struct ed_batch {
   std::vector<int64_t> h_off_q;
   int32_t* d_job_of = nullptr;
   BwdJob* d_jobs = nullptr;
+  BwdGemm* d_gemms = nullptr;
   BwdTile* d_tiles = nullptr;
   BwdBlock* d_blocks = nullptr;
   double *d_jstore = nullptr, *d_rec = nullptr, *d_partial = nullptr;
-  int64_t cap_jobs = 0, cap_tiles = 0, cap_blocks = 0, cap_jstore = 0;
+  int64_t cap_jobs = 0, cap_gemms = 0, cap_tiles = 0, cap_blocks = 0, cap_jstore = 0;
   char* d_bws = nullptr;
   int64_t bws_bytes = 0;
   int32_t solved_nw = -1;  // frequencies of the last solve (the backward uses them)
@@ -1986,7 +2034,7 @@ static void free_batch(ed_batch* b) {
                   (void*)b->d_off_q, (void*)b->d_store_psi, (void*)b->d_store_q, (void*)b->d_store_coef,
                   (void*)b->d_gre, (void*)b->d_gim, (void*)b->d_grad_eps, (void*)b->d_grad_v, (void*)b->d_cg,
                   (void*)b->d_bcounter, (void*)b->d_bws, (void*)b->d_gs_store, (void*)b->d_off_gs, (void*)b->d_job_of,
-                  (void*)b->d_jobs, (void*)b->d_tiles, (void*)b->d_blocks, (void*)b->d_jstore, (void*)b->d_rec,
+                  (void*)b->d_jobs, (void*)b->d_gemms, (void*)b->d_tiles, (void*)b->d_blocks, (void*)b->d_jstore, (void*)b->d_rec,
                   (void*)b->d_partial})
     if (p) (void)hipFree(p);
   delete b;
@@ -2531,7 +2579,8 @@ extern "C" int ed_batch_backward(ed_batch* b, const double* dl_dre, const double
   ED_HIP(hipMemcpy(iters.data(), b->d_iters, sizeof(int32_t) * 3 * M, hipMemcpyDeviceToHost));
   std::vector<int32_t> job_of(2 * M, -1);
   std::vector<BwdJob> jobs;
-  std::vector<BwdTile> tiles;
+  std::vector<BwdGemm> gemms;                 // Re W products, then P products
+  std::vector<BwdTile> tiles_w, tiles;        // their tiles
   std::vector<BwdBlock> blocks;
   int64_t jst = 0;
   for (int64_t i = 0; i < M; ++i) {
@@ -2547,6 +2596,10 @@ extern "C" int ed_batch_backward(ed_batch* b, const double* dl_dre, const double
       jb.m = m;
       jb.dt = (int32_t)sector_dim(ns, n_t, p.n_dn);
       jb.q_off = b->h_off_q[2 * i + part];
+      jb.a_off = jst;
+      jst += 2ll * m * nw;
+      jb.b_off = jst;
+      jst += 2ll * m * nw;
       jb.w_off = jst;
       jst += (int64_t)m * m;
       jb.p_off = jst;
@@ -2554,13 +2607,24 @@ extern "C" int ed_batch_backward(ed_batch* b, const double* dl_dre, const double
       jb.blk0 = (int32_t)blocks.size();
       jb.nblk = (jb.dt + 63) / 64;
       const int32_t j = (int32_t)jobs.size();
-      for (int32_t r0 = 0; r0 < jb.dt; r0 += 64) {
-        blocks.push_back(BwdBlock{j, r0});
-        for (int32_t c0 = 0; c0 < m; c0 += 64) tiles.push_back(BwdTile{j, r0, c0, 0});
-      }
+      for (int32_t r0 = 0; r0 < jb.dt; r0 += 64) blocks.push_back(BwdBlock{j, r0});
       job_of[2 * i + part] = j;
       jobs.push_back(jb);
     }
+  }
+  // Re W = A B^T (m x m, K = 2 nw) for every job, then P = Qt Re W^T
+  const int32_t nj = (int32_t)jobs.size();
+  for (int32_t j = 0; j < nj; ++j) {
+    const BwdJob& jb = jobs[j];
+    gemms.push_back(BwdGemm{jb.a_off, jb.b_off, jb.w_off, 2 * nw, 2 * nw, jb.m, jb.m, 2 * nw, 0});
+    for (int32_t r0 = 0; r0 < jb.m; r0 += 64)
+      for (int32_t c0 = 0; c0 < jb.m; c0 += 64) tiles_w.push_back(BwdTile{j, r0, c0, 0});
+  }
+  for (int32_t j = 0; j < nj; ++j) {
+    const BwdJob& jb = jobs[j];
+    gemms.push_back(BwdGemm{jb.q_off, jb.w_off, jb.p_off, b->kcap, jb.m, jb.dt, jb.m, jb.m, 1});
+    for (int32_t r0 = 0; r0 < jb.dt; r0 += 64)
+      for (int32_t c0 = 0; c0 < jb.m; c0 += 64) tiles.push_back(BwdTile{nj + j, r0, c0, 0});
   }
   auto grow = [](auto** ptr, int64_t* cap, int64_t n) -> int {
     if (n <= *cap) return ED_OK;
@@ -2572,7 +2636,8 @@ extern "C" int ed_batch_backward(ed_batch* b, const double* dl_dre, const double
     return ED_OK;
   };
   ED_TRY(grow(&b->d_jobs, &b->cap_jobs, (int64_t)jobs.size()));
-  ED_TRY(grow(&b->d_tiles, &b->cap_tiles, (int64_t)tiles.size()));
+  ED_TRY(grow(&b->d_gemms, &b->cap_gemms, (int64_t)gemms.size()));
+  ED_TRY(grow(&b->d_tiles, &b->cap_tiles, (int64_t)(tiles_w.size() + tiles.size())));
   if ((int64_t)blocks.size() > b->cap_blocks) {
     if (b->d_partial) (void)hipFree(b->d_partial);
     b->d_partial = nullptr;
@@ -2583,7 +2648,10 @@ extern "C" int ed_batch_backward(ed_batch* b, const double* dl_dre, const double
   ED_HIP(hipMemcpy(b->d_job_of, job_of.data(), sizeof(int32_t) * 2 * M, hipMemcpyHostToDevice));
   if (!jobs.empty()) {
     ED_HIP(hipMemcpy(b->d_jobs, jobs.data(), sizeof(BwdJob) * jobs.size(), hipMemcpyHostToDevice));
-    ED_HIP(hipMemcpy(b->d_tiles, tiles.data(), sizeof(BwdTile) * tiles.size(), hipMemcpyHostToDevice));
+    ED_HIP(hipMemcpy(b->d_gemms, gemms.data(), sizeof(BwdGemm) * gemms.size(), hipMemcpyHostToDevice));
+    ED_HIP(hipMemcpy(b->d_tiles, tiles_w.data(), sizeof(BwdTile) * tiles_w.size(), hipMemcpyHostToDevice));
+    ED_HIP(hipMemcpy(b->d_tiles + tiles_w.size(), tiles.data(), sizeof(BwdTile) * tiles.size(),
+                     hipMemcpyHostToDevice));
     ED_HIP(hipMemcpy(b->d_blocks, blocks.data(), sizeof(BwdBlock) * blocks.size(), hipMemcpyHostToDevice));
   }
   BwdConfig bc{};
@@ -2625,8 +2693,10 @@ extern "C" int ed_batch_backward(ed_batch* b, const double* dl_dre, const double
   hipLaunchKernelGGL(k_reset_one, dim3(1), dim3(64), 0, s, b->d_bcounter);
   hipLaunchKernelGGL(k_batch_backward, dim3((unsigned)grid), dim3(kBlock), 0, s, bc);
   if (!jobs.empty()) {
-    hipLaunchKernelGGL(k_bwd_p, dim3((unsigned)tiles.size()), dim3(256), 0, s, (const BwdJob*)b->d_jobs,
-                       (const BwdTile*)b->d_tiles, (const double*)b->d_store_q, (int)b->kcap, b->d_jstore);
+    hipLaunchKernelGGL(k_bwd_gemm, dim3((unsigned)tiles_w.size()), dim3(256), 0, s, (const BwdGemm*)b->d_gemms,
+                       (const BwdTile*)b->d_tiles, (const double*)b->d_store_q, b->d_jstore);
+    hipLaunchKernelGGL(k_bwd_gemm, dim3((unsigned)tiles.size()), dim3(256), 0, s, (const BwdGemm*)b->d_gemms,
+                       (const BwdTile*)(b->d_tiles + tiles_w.size()), (const double*)b->d_store_q, b->d_jstore);
     hipLaunchKernelGGL(k_bwd_sums, dim3((unsigned)blocks.size()), dim3(256), 0, s, bc);
   }
   hipLaunchKernelGGL(k_bwd_combine, dim3((unsigned)((M + 63) / 64)), dim3(64), 0, s, bc);
