@@ -831,10 +831,17 @@ __device__ int cf_coeffs(const Group<BLOCK>& g, const ModelParams& p, const Spin
   double* u = sl.vec(1);
   double* up = sl.vec(0);
   double* w = sl.vec(2);
-  double* scal = sl.scal();
   double sj = 1.0 / sqrt(n2), sp = 0.0, bj = 0.0;
   int m = 0, calm = 0;
   cplx prev = {0.0, 0.0};
+  // Continuants of the fraction at z = i wmin, carried step by step:
+  // y_m = d_0 - b_0^2 / (d_1 - b_1^2 / (... d_{m-1})) = A_m / B_m with
+  // d_k = (e0 - a_k) + i sgn wmin, A_n = d_n A_{n-1} - b_{n-1}^2 A_{n-2}
+  // (A_{-1} = 1, A_0 = d_0; B_{-1} = 0, B_0 = 1), and f_m = sgn n2 B_m / A_m
+  // is the value cf_value(a, b, m, n2, e0, sgn, wmin) takes.  Every member
+  // holds the same values (rescaled by the size of A_m each step), so a check
+  // costs O(1) and no barrier.
+  cplx A = {1.0, 0.0}, Ap = {0.0, 0.0}, B = {0.0, 0.0}, Bp = {1.0, 0.0};
   const int64_t D = (int64_t)U.n * Dn.n;
   for (int j = 0; j < cap; ++j) {
     double aj, nn;
@@ -847,35 +854,39 @@ __device__ int cf_coeffs(const Group<BLOCK>& g, const ModelParams& p, const Spin
     if (j == cap - 1) break;
     if (!(bn > 1e-12 * (fabs(aj) + bj))) break;  // invariant subspace
     b[j] = bn;
+    {
+      const cplx d = {e0 - aj, sgn * wmin};
+      const double c = -bj * bj;  // -b_{j-1}^2 (unused at j = 0)
+      cplx An, Bn;
+      if (j == 0) {
+        An = d;
+        Bn = {1.0, 0.0};
+        Ap = {1.0, 0.0};
+        Bp = {0.0, 0.0};
+      } else {
+        An = {d.re * A.re - d.im * A.im + c * Ap.re, d.re * A.im + d.im * A.re + c * Ap.im};
+        Bn = {d.re * B.re - d.im * B.im + c * Bp.re, d.re * B.im + d.im * B.re + c * Bp.im};
+        Ap = A;
+        Bp = B;
+      }
+      A = An;
+      B = Bn;
+      const double sz = fmax(fabs(A.re), fabs(A.im));
+      if (sz > 0.0) {
+        const double r = 1.0 / sz;
+        A = {A.re * r, A.im * r};
+        B = {B.re * r, B.im * r};
+        Ap = {Ap.re * r, Ap.im * r};
+        Bp = {Bp.re * r, Bp.im * r};
+      }
+    }
     if (m % 8 == 0) {
-      g.sync();
-      // block mode: the check reads the coefficients from the LDS row
-      // buffer (idle between sweeps) instead of global memory
-      const double* ca = a;
-      const double* cb = b;
-      if constexpr (BLOCK) {
-        if (g.has_lds && 2 * m <= g.lds.rows_cap) {
-          double* const st = (double*)g.lds.rows;
-          for (int i = g.rank; i < m; i += g.size) {
-            st[i] = a[i];
-            st[m + i] = b[i];
-          }
-          g.sync();
-          ca = st;
-          cb = st + m;
-        }
-      }
-      if (g.rank == 0) {
-        const cplx c = cf_value(ca, cb, m, n2, e0, sgn, wmin);
-        // change measured against the part's spectral weight n2 as well, so a
-        // small |G(i wmin)| does not make the test unreachable
-        const double d = hypot(c.re - prev.re, c.im - prev.im);
-        scal[4] = d <= cf_tol * fmax(hypot(c.re, c.im), n2) ? 1.0 : 0.0;
-        prev = c;
-      }
-      g.sync();
-      calm = scal[4] != 0.0 ? calm + 1 : 0;
-      g.sync();
+      const cplx f = cdiv({sgn * n2 * B.re, sgn * n2 * B.im}, A);
+      // change measured against the part's spectral weight n2 as well, so a
+      // small |G(i wmin)| does not make the test unreachable
+      const double dd = hypot(f.re - prev.re, f.im - prev.im);
+      calm = dd <= cf_tol * fmax(hypot(f.re, f.im), n2) ? calm + 1 : 0;
+      prev = f;
       if (calm >= 2) break;
     }
     double* t = up;
