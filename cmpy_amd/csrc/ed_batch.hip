@@ -1354,6 +1354,38 @@ __device__ __forceinline__ double row_dot(const double* __restrict__ x, const do
   return acc;
 }
 
+// <p, y1> and <p, y2> (y2 may be null: its sum stays 0) in order, p's loads
+// shared, eight entries of each in flight at a time.
+__device__ __forceinline__ void row_dot2(const double* __restrict__ x, const double* __restrict__ y1,
+                                         const double* __restrict__ y2, int n, double* s1, double* s2) {
+  double a1 = 0.0, a2 = 0.0;
+  int j = 0;
+  if (y2) {
+    for (; j + 8 <= n; j += 8) {
+      double xv[8], v1[8], v2[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        xv[r] = x[j + r];
+        v1[r] = y1[j + r];
+        v2[r] = y2[j + r];
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        a1 += xv[r] * v1[r];
+        a2 += xv[r] * v2[r];
+      }
+    }
+    for (; j < n; ++j) {
+      a1 += x[j] * y1[j];
+      a2 += x[j] * y2[j];
+    }
+  } else {
+    a1 = row_dot(x, y1, n);
+  }
+  *s1 = a1;
+  *s2 = a2;
+}
+
 __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi, const BwdSlot& bs, double* wl) {
   const ModelParams& p = bc.params[mi];
   const int ns = p.nb + 1;
@@ -1654,8 +1686,34 @@ __device__ void backward_model(const Group<true>& g, const BwdConfig& bc, int mi
       for (int64_t r = g.rank; r < Dt; r += g.size) {
         const int iu = (int)(r / nd), id = (int)(r - (int64_t)iu * nd);
         se += occ_k(T.st[iu], Dn.st[id], k) * dg[r];
-        sv += bond_hops(T, Dn, r, k,
-                        [&](int64_t aa, int64_t bb) { return row_dot(P + aa * kcap, qt + bb * kcap, m); });
+        // the bond's up hop and dn hop from r (at most one each), their dots
+        // with P's row r together; summed as one hop after the other
+        int64_t t0 = -1, t1 = -1;
+        double g0 = 0.0, g1 = 0.0;
+        for (int e = 0; e < T.cnt[iu]; ++e) {
+          const uint32_t t = (uint32_t)T.tgt[iu * kMaxBath + e];
+          if ((int)((t >> 16) & 0xff) != k) continue;
+          g0 = (t & 0x80000000u) ? -1.0 : 1.0;
+          t0 = (int64_t)(t & 0xffff) * nd + id;
+        }
+        for (int e = 0; e < Dn.cnt[id]; ++e) {
+          const uint32_t t = (uint32_t)Dn.tgt[id * kMaxBath + e];
+          if ((int)((t >> 16) & 0xff) != k) continue;
+          g1 = (t & 0x80000000u) ? -1.0 : 1.0;
+          t1 = (int64_t)iu * nd + (t & 0xffff);
+        }
+        if (t0 < 0) {  // dn hop only: it takes the first slot
+          t0 = t1;
+          g0 = g1;
+          t1 = -1;
+        }
+        if (t0 >= 0) {
+          double d0, d1;
+          row_dot2(P + r * kcap, qt + t0 * kcap, t1 >= 0 ? qt + t1 * kcap : nullptr, m, &d0, &d1);
+          double sb = g0 * d0;
+          if (t1 >= 0) sb += g1 * d1;
+          sv += sb;
+        }
       }
       se = g.sum(se);
       sv = g.sum(sv);
