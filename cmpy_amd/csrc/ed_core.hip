@@ -348,8 +348,6 @@ static int dev_alloc(T** p, int64_t count) {
 }
 
 
-// Builds the half-width table (see Tables::h16) on the host from the packed
-// column-major table and uploads it.
 // Pair table for the column pass when every row's pair list fits the unrolled
 // maximum; otherwise none is built and the column pass uses the plain lists.
 static int build_pair_table(Tables& tab, int64_t count, hipStream_t s) {
@@ -382,6 +380,8 @@ static int build_pair_table(Tables& tab, int64_t count, hipStream_t s) {
   return ED_OK;
 }
 
+// Builds the half-width table (see Tables::h16) on the host from the packed
+// column-major table and uploads it.
 static int build_half_tables(Tables& tab, int64_t count, int width, hipStream_t s) {
   constexpr int L = kColourLanes;
   std::vector<uint32_t> pk((size_t)count * width);

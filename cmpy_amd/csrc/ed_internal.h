@@ -248,10 +248,16 @@ HvArgs make_hv_args(const ed_op* op);
 
 // Lanczos epilogue: y = s_j * H x - c_j * xprev, partial <y, x>.
 struct HvEpilogue {
-  const double* xprev;  // may be null
-  const double* sx;     // device scalar: scale of x (null = 1)
-  const double* cprev;  // device scalar: coefficient of xprev (null = 0)
-  double* partial;      // per-block partial dots (null = none)
+  const double* xprev = nullptr;  // may be null
+  const double* sx = nullptr;     // device scalar: scale of x (null = 1)
+  const double* cprev = nullptr;  // device scalar: coefficient of xprev (null = 0)
+  double* partial = nullptr;      // per-block partial dots (null = none)
+  // Recurrence with known coefficients (the eigenvector's second pass):
+  // y = H (sx x) - cself sx x - cprev xprev, and psi += psi_y sx x over the
+  // sector entries (row padding untouched).  Applied by the row pass.
+  const double* cself = nullptr;  // device scalar (null = 0)
+  double* psi = nullptr;          // null = none
+  double psi_y = 0.0;
 };
 
 // Launch helpers implemented in ed_matvec.hip.  Vectors are stored row by

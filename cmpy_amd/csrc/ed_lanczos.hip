@@ -624,7 +624,8 @@ struct EventTimer {
 // One step of the three-vector recurrence: H.v with the epilogue (partials
 // of <w, u_j> into part), the update with alpha_j formed in it (|w|^2
 // partials into part2), beta_{j+1}.  Without reduce (the eigenvector's
-// second pass) alpha_j and beta_{j+1} are already known.
+// second pass) alpha_j and beta_{j+1} are already known, and the H.v's
+// epilogue applies alpha_j and accumulates psi itself: no update pass.
 static int run_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, double* part2, double* u_prev,
                     double* u_cur, double* w, int j, hipStream_t s, EventTimer* tm, double* psi, double yj,
                     bool reduce) {
@@ -633,6 +634,11 @@ static int run_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, double
   e.sx = c.scale + j;
   e.cprev = c.cprev + j;
   e.partial = reduce ? part : nullptr;
+  if (!reduce) {
+    e.cself = c.alpha + j;
+    e.psi = psi;
+    e.psi_y = yj;
+  }
   if (tm && tm->on) {
     hipEvent_t a = tm->next();
     if (a) ED_HIP(hipEventRecord(a, s));
@@ -643,10 +649,11 @@ static int run_step(ed_op* op, const LzLayout& L, LzCoef c, double* part, double
     hipEvent_t b = tm->next();
     if (b) ED_HIP(hipEventRecord(b, s));
   }
-  const int ub = update_blocks(L.D);
-  launch_update(ub, s, w, u_cur, L.D, c.alpha + j, c.scale + j, psi, yj, reduce ? part2 : nullptr,
-                reduce ? part : nullptr, np);
-  if (reduce) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part2, ub, (int)RED_BETA, c, j, nullptr);
+  if (reduce) {
+    const int ub = update_blocks(L.D);
+    launch_update(ub, s, w, u_cur, L.D, c.alpha + j, c.scale + j, psi, yj, part2, part, np);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedThreads), 0, s, part2, ub, (int)RED_BETA, c, j, nullptr);
+  }
   ED_HIP(hipGetLastError());
   return ED_OK;
 }

@@ -100,14 +100,17 @@ __global__ __launch_bounds__(256) void k_hv(HvArgs a, const double* __restrict__
     if (LZ) {
       const double sx = e.sx ? *e.sx : 1.0;
       const double cp = e.cprev ? *e.cprev : 0.0;
+      const double cs = e.cself ? *e.cself : 0.0;
+      const double ps = e.psi_y * sx;
 #pragma unroll
       for (int r = 0; r < RB; ++r) {
         const int64_t iu = r0 + r;
         if (iu < a.num_up) {
           const int64_t k = iu * ld + id;
-          double v = sx * acc[r];
+          double v = sx * (acc[r] - cs * xo[r]);
           if (e.xprev) v -= cp * e.xprev[k];
           y[k] = v;
+          if (e.psi) e.psi[k] += ps * xo[r];
           dot += v * xo[r];
         }
       }
@@ -284,6 +287,8 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows(Hv2Args a, int rows_per
   __syncthreads();
   const double sx = e.sx ? *e.sx : 1.0;
   const double cp = e.cprev ? *e.cprev : 0.0;
+  const double cs = e.cself ? *e.cself : 0.0;
+  const double ps = e.psi_y * sx;
   const double t = a.nb > 0 ? hv[0] : 0.0;
   double dot = 0.0;
   const int64_t nd_pad = (nd + 63) & ~int64_t(63);
@@ -326,8 +331,9 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows(Hv2Args a, int rows_per
         for (int q = 0; q < kRowBatch; ++q) acc += vl[q] * g[q];
       }
       if (act) {
-        const double v = sx * acc - cp * xp;
+        const double v = sx * (acc - cs * xo) - cp * xp;
         __builtin_nontemporal_store(v, y + gi);
+        if (e.psi) e.psi[gi] += ps * xo;
         dot += v * xo;
       }
     }
@@ -384,6 +390,8 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_off(Hv2Args a, int rows
   __syncthreads();
   const double sx = e.sx ? *e.sx : 1.0;
   const double cp = e.cprev ? *e.cprev : 0.0;
+  const double cs = e.cself ? *e.cself : 0.0;
+  const double ps = e.psi_y * sx;
   const double t = a.hop[0];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -414,9 +422,10 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_off(Hv2Args a, int rows
         if (act && rg + r < R) {
           const int64_t iu = r0 + rg + r;
           const double xo = lds[(int64_t)(rg + r) * stride + id];
-          const double dg = a.d_up[iu] + ddn + u_term2(a, a.st_up[iu] & sdn);
+          const double dg = a.d_up[iu] + ddn + u_term2(a, a.st_up[iu] & sdn) - cs;
           const double v = sx * (dg * xo + t * (accp[r] - accn[r])) + yu[r] - cp * xp[r];
           __builtin_nontemporal_store(v, y + iu * ld + id);
+          if (e.psi) e.psi[iu * ld + id] += ps * xo;
           dot += v * xo;
         }
       }
@@ -437,6 +446,12 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_off(Hv2Args a, int rows
 // dynamic LDS therefore starts at 0).
 using lds_f64 = __attribute__((address_space(3))) double;
 
+// One word's two hops of sign sg into a0 (low entry) and a1 (high entry).
+__device__ __forceinline__ void add_word(uint32_t x, double sg, const lds_f64* row, double& a0, double& a1) {
+  a0 = fma(sg, row[x & 0xffffu], a0);
+  a1 = fma(sg, row[x >> 16], a1);
+}
+
 template <int NW>
 __device__ __forceinline__ double sum_words(const char* tb, uint32_t nd4, uint32_t lane4, int nwp,
                                             const lds_f64* row) {
@@ -445,11 +460,7 @@ __device__ __forceinline__ double sum_words(const char* tb, uint32_t nd4, uint32
   for (int k = 0; k < NW; ++k) w[k] = *(const uint32_t*)(tb + (size_t)k * nd4 + lane4);
   double a0 = 0.0, a1 = 0.0;
 #pragma unroll
-  for (int k = 0; k < NW; ++k) {
-    const double sg = k < nwp ? 1.0 : -1.0;
-    a0 = fma(sg, row[w[k] & 0xffffu], a0);
-    a1 = fma(sg, row[w[k] >> 16], a1);
-  }
+  for (int k = 0; k < NW; ++k) add_word(w[k], k < nwp ? 1.0 : -1.0, row, a0, a1);
   return a0 + a1;
 }
 
@@ -489,8 +500,10 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_half(Hv2Args a, int row
   __syncthreads();
   const double sx = e.sx ? *e.sx : 1.0;
   const double cp = e.cprev ? *e.cprev : 0.0;
+  const double cs = e.cself ? *e.cself : 0.0;
+  const double ps = e.psi_y * sx;
   const double t = a.hop[0];
-  const double dup = a.d_up[iu];
+  const double dup = a.d_up[iu] - cs;
   const uint32_t sup = a.st_up[iu];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -513,6 +526,7 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_half(Hv2Args a, int row
       const double dg = dup + ddn + u_term2(a, sup & sdn);
       const double v = sx * (dg * xo + t * pn) + yu - cp * xp;
       __builtin_nontemporal_store(v, y + gi);
+      if (e.psi) e.psi[gi] += ps * xo;
       dot += v * xo;
     }
   }
@@ -536,11 +550,7 @@ __device__ __forceinline__ double sum_words_q(const u32x4* tq, int nwp, const ld
 #pragma unroll
   for (int g = 0; g < NG; ++g)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const double sg = 4 * g + j < nwp ? 1.0 : -1.0;
-      a0 = fma(sg, row[w[g][j] & 0xffffu], a0);
-      a1 = fma(sg, row[w[g][j] >> 16], a1);
-    }
+    for (int j = 0; j < 4; ++j) add_word(w[g][j], 4 * g + j < nwp ? 1.0 : -1.0, row, a0, a1);
   return a0 + a1;
 }
 
@@ -598,8 +608,9 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_held(Hv2Args a, int row
   for (int k = 0; k < kRowHeld; ++k) held[k] = fma(-cp, ph[k], held[k]);
   __syncthreads();
   const double sx = e.sx ? *e.sx : 1.0;
+  const double ps = e.psi_y * sx;
   const double t = a.hop[0];
-  const double dup = a.d_up[iu];
+  const double dup = a.d_up[iu] - (e.cself ? *e.cself : 0.0);
   const uint32_t sup = a.st_up[iu];
   double dot = 0.0;
   for (int k = 0; k < kRowHeld; ++k) {
@@ -622,6 +633,7 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_held(Hv2Args a, int row
       const double dg = dup + ddn + u_term2(a, sup & sdn);
       const double v = sx * (dg * xo + t * pn) + held[k];
       __builtin_nontemporal_store(v, y + gi);
+      if (e.psi) e.psi[gi] += ps * xo;
       dot += v * xo;
     }
   }
@@ -1432,7 +1444,7 @@ int launch_matvec(const ed_op* op, const double* x, double* y, int64_t ld, const
   int64_t nrb, nct;
   grid_shape(op, rb, ld, &nrb, &nct);
   ED_REQUIRE(nrb * nct < (1ll << 31), "grid too large");
-  const bool lz = epi.xprev || epi.sx || epi.cprev || epi.partial;
+  const bool lz = epi.xprev || epi.sx || epi.cprev || epi.partial || epi.cself || epi.psi;
   if (lz)
     launch_rb<true>(rb, a, x, y, epi, nrb, nct, s);
   else

@@ -10,10 +10,78 @@
 // Output per model: "model m D num_up num_dn", then "dense v..." (row-major)
 // where D <= 400, and "hop sigma k cnt col val col val ..." per state of each
 // spin species with at most 10,000 states.
+#include <algorithm>
 #include <fstream>
 #include <string>
 
 #include "cmpy_amd/csrc/ed_core.cpp"
+
+// Decodes a species' half-width row tables back into each state's hop list
+// and compares it, as a multiset of (target, sign), with the packed table;
+// checks that every slot of a half-wave reads 32 distinct bank pairs and that
+// the lane-contiguous copy holds the same words.  Prints "half sigma words"
+// and returns 0, or reports and returns 1.
+static int check_half(const ed::Tables& t, int64_t count, int sigma) {
+  if (!t.h16) return 0;
+  constexpr int L = ed::kColourLanes;
+  const int64_t nchunk = (count + 63) / 64;
+  const int nw = t.hwords;
+  for (int64_t i = 0; i < count; ++i) {
+    const int64_t c = i / 64;
+    const int nwp = t.hwords_chunk[2 * c], nwc = t.hwords_chunk[2 * c + 1];
+    std::vector<std::pair<int32_t, int>> got, want;
+    for (int w = 0; w < nw; ++w) {
+      const uint32_t x = t.h16[(size_t)w * count + i];
+      const uint32_t q = t.h16q[((size_t)(c * t.hgroups + w / 4) * 64 + i % 64) * 4 + w % 4];
+      if (q != x) {
+        std::fprintf(stderr, "sigma %d state %lld word %d: lane-contiguous copy differs\n", sigma, (long long)i, w);
+        return 1;
+      }
+      for (int j = 0; j < 2; ++j) {
+        const int32_t p = (int32_t)((x >> (16 * j)) & 0xffffu);
+        const int sg = w < nwp ? 1 : -1;
+        if (p >= count + L) {
+          std::fprintf(stderr, "sigma %d state %lld: entry %d past the pads\n", sigma, (long long)i, p);
+          return 1;
+        }
+        if (p >= count) continue;
+        if (w >= nwc) {
+          std::fprintf(stderr, "sigma %d state %lld: hop past the chunk's words\n", sigma, (long long)i);
+          return 1;
+        }
+        got.emplace_back(p, sg);
+      }
+    }
+    for (int k = 0; k < t.cnt[i]; ++k) {
+      const uint32_t e = t.pk_cm[(size_t)k * count + i];
+      want.emplace_back(ed::hop_target(e), (e & 0x80000000u) ? -1 : 1);
+    }
+    std::sort(got.begin(), got.end());
+    std::sort(want.begin(), want.end());
+    if (got != want) {
+      std::fprintf(stderr, "sigma %d state %lld: decoded hops differ from the packed table\n", sigma, (long long)i);
+      return 1;
+    }
+  }
+  for (int64_t c = 0; c < nchunk; ++c)
+    for (int h = 0; h < 2; ++h)
+      for (int w = 0; w < nw; ++w)
+        for (int j = 0; j < 2; ++j) {
+          std::vector<char> used(L, 0);
+          for (int l = 0; l < L; ++l) {
+            const int64_t i = c * 64 + h * L + l;
+            if (i >= count) break;
+            const int32_t p = (int32_t)((t.h16[(size_t)w * count + i] >> (16 * j)) & 0xffffu);
+            if (used[p % L]++) {
+              std::fprintf(stderr, "sigma %d chunk %lld half %d slot %d: bank pair %d twice\n", sigma,
+                           (long long)c, h, 2 * w + j, p % L);
+              return 1;
+            }
+          }
+        }
+  std::printf("half %d %d\n", sigma, nw);
+  return 0;
+}
 
 int main(int argc, char** argv) {
   if (argc < 2) return 2;
@@ -49,6 +117,7 @@ int main(int argc, char** argv) {
     ed_op_shape(op, &a, &b);
     const int64_t D = a * b;
     std::printf("model %d %lld %lld %lld\n", m, (long long)D, (long long)a, (long long)b);
+    if (check_half(op->up, a, 1) || check_half(op->dn, b, 2)) return 1;
     if (D <= 400) {
       std::vector<double> h((size_t)(D * D));
       double* hd = nullptr;

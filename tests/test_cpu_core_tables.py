@@ -42,15 +42,18 @@ def _models():
     ]
 
 
-def test_operator_tables_clean_and_equal_to_oracle(tmp_path):
+def _core_check():
     import importlib.util
 
     spec = importlib.util.spec_from_file_location("build_harness", HERE / "cpu_kernels" / "build_harness.py")
     bh = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(bh)
-    exe = bh.build(name="core_check", target="core")
-    models = _models()
-    path = tmp_path / "models.txt"
+    return bh.build(name="core_check", target="core")
+
+
+def _run_models(exe, path, models, env=None):
+    import os
+
     with open(path, "w") as f:
         f.write(f"{len(models)}\n")
         for ns, nu, nd, w, bonds, hops, eps, u in models:
@@ -59,9 +62,41 @@ def test_operator_tables_clean_and_equal_to_oracle(tmp_path):
             f.write(" ".join(f"{x:.17g}" for x in hops) + "\n")
             f.write(" ".join(f"{x:.17g}" for x in eps) + "\n")
             f.write(" ".join(f"{x:.17g}" for x in u) + "\n")
-    r = subprocess.run([str(exe), str(path)], capture_output=True, text=True, timeout=900)
+    r = subprocess.run([str(exe), str(path)], capture_output=True, text=True, timeout=900,
+                       env={**os.environ, **(env or {})})
     assert r.returncode == 0, r.stderr[-4000:]
     assert "AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-4000:]
+    return r
+
+
+def _half_lines(out):
+    """{(model, sigma): words} from the harness's half-table checks."""
+    got, m = {}, -1
+    for line in out.splitlines():
+        t = line.split()
+        if t[0] == "model":
+            m = int(t[1])
+        elif t[0] == "half":
+            got[(m, int(t[1]))] = int(t[2])
+    return got
+
+
+def test_half_tables_decode_to_the_hop_lists(tmp_path):
+    """The half-width row tables of the 4x4 torus sectors (12,870 and 8,008
+    states): every state's decoded entries equal its packed hop list as a
+    multiset of (target, sign), each slot of a half-wave reads 32 distinct
+    bank pairs, and the lane-contiguous copy holds the same words (checked
+    inside the harness)."""
+    exe = _core_check()
+    got = _half_lines(_run_models(exe, tmp_path / "half.txt", _models()[:2]).stdout)
+    assert set(got) == {(0, 1), (0, 2), (1, 2)}, got
+    assert all(0 < w <= 24 for w in got.values()), got
+
+
+def test_operator_tables_clean_and_equal_to_oracle(tmp_path):
+    exe = _core_check()
+    models = _models()
+    r = _run_models(exe, tmp_path / "models.txt", models)
     blocks, cur = [], None
     for line in r.stdout.splitlines():
         t = line.split()
