@@ -516,6 +516,8 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_half(Hv2Args a, int row
     const int nwp = uniform_load(a.dn_hwords + 2 * chunk);
     const int nw = uniform_load(a.dn_hwords + 2 * chunk + 1);
     const int64_t gi = iu * ld + ids;
+    // psi read before the gathers, so its HBM latency overlaps them
+    const double pso = e.psi && act ? __builtin_nontemporal_load(e.psi + gi) : 0.0;
     const double xp = e.xprev ? e.xprev[gi] : 0.0;
     const double yu = ADDUP ? __builtin_nontemporal_load(y + gi) : 0.0;
     const uint32_t sdn = a.st_dn[ids];
@@ -526,7 +528,7 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_half(Hv2Args a, int row
       const double dg = dup + ddn + u_term2(a, sup & sdn);
       const double v = sx * (dg * xo + t * pn) + yu - cp * xp;
       __builtin_nontemporal_store(v, y + gi);
-      if (e.psi) e.psi[gi] += ps * xo;
+      if (e.psi) __builtin_nontemporal_store(pso + ps * xo, e.psi + gi);
       dot += v * xo;
     }
   }
@@ -623,6 +625,8 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_held(Hv2Args a, int row
     const int nwp = uniform_load(a.dn_hwords + 2 * chunk);
     const int nw = uniform_load(a.dn_hwords + 2 * chunk + 1);
     const int64_t gi = iu * ld + ids;
+    // psi read before the gathers, so its HBM latency overlaps them
+    const double pso = e.psi && act ? __builtin_nontemporal_load(e.psi + gi) : 0.0;
     const uint32_t sdn = a.st_dn[ids];
     const double ddn = a.d_dn[ids];
     const double pn =
@@ -633,7 +637,7 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_held(Hv2Args a, int row
       const double dg = dup + ddn + u_term2(a, sup & sdn);
       const double v = sx * (dg * xo + t * pn) + held[k];
       __builtin_nontemporal_store(v, y + gi);
-      if (e.psi) e.psi[gi] += ps * xo;
+      if (e.psi) __builtin_nontemporal_store(pso + ps * xo, e.psi + gi);
       dot += v * xo;
     }
   }
@@ -1122,6 +1126,127 @@ __global__ __launch_bounds__(256) void k_hv_cols_quad(Hv2Args a, const double* _
   if (wr && pend_at) __builtin_nontemporal_store(pend, pend_at);
 }
 
+// Rows per block of the oct column pass: their quad lists are staged in LDS.
+constexpr int kOctRows = 16;
+
+// Word k of staged row r's quad list (header at k = 0).
+__device__ __forceinline__ uint32_t oct_word(const uint32_t* lists, int r, int stride, int k) {
+  return lists[r * stride + k];
+}
+
+// Two rows' up hops over a 16-column tile, two quads (an oct) per load:
+// eighth ei of the wave (lanes 8 ei .. 8 ei + 7) takes entry ei of the oct,
+// the half ei & 1 of its word ei >> 1, and each lane reads two adjacent
+// columns, so one load serves eight hops.  Entries of quad 2m + (ei >> 2)
+// have sign +1 below qp quads; quads from nq on (the second half of a row's
+// last oct when nq is odd, and every oct past the row's own when the other
+// row has more) read the pad row, which the buffer range turns into zeros.
+// psel picks entry ei & 3 of a word pair.  NO octs per row, all loads of
+// both rows in flight together.
+template <int NO>
+__device__ __forceinline__ void sum_octs2(const uint32_t* lists, int stride, int ra, int rb, const int (&qp)[2],
+                                          const int (&nq)[2], uint32_t pad, int ei, uint32_t psel, uint32_t nd8,
+                                          uint32_t laneoff, __amdgpu_buffer_rsrc_t rx, double (&s)[2][2]) {
+  f64x2 g[2][NO];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int r = h ? rb : ra;
+#pragma unroll
+    for (int m = 0; m < NO; ++m) {
+      const uint32_t w0 = oct_word(lists, r, stride, 1 + 4 * m), w1 = oct_word(lists, r, stride, 2 + 4 * m);
+      const uint32_t w2 = oct_word(lists, r, stride, 3 + 4 * m), w3 = oct_word(lists, r, stride, 4 + 4 * m);
+      const uint32_t lo = __builtin_amdgcn_perm(w1, w0, psel);
+      const uint32_t hi = __builtin_amdgcn_perm(w3, w2, psel);
+      const uint32_t ent = 2 * m + (ei >> 2) < nq[h] ? (ei >= 4 ? hi : lo) : pad;
+      const auto v = __builtin_amdgcn_raw_buffer_load_b128(rx, __umul24(ent, nd8) + laneoff, 0, 0);
+      g[h][m] = __builtin_bit_cast(f64x2, v);
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int m = 0; m < NO; ++m) {
+      const double sg = 2 * m + (ei >> 2) < qp[h] ? 1.0 : -1.0;
+      a0 = fma(sg, g[h][m].x, a0);
+      a1 = fma(sg, g[h][m].y, a1);
+    }
+    s[h][0] = a0;
+    s[h][1] = a1;
+  }
+}
+
+// Column pass over 16-column tiles (one 128-byte line per row of x) from the
+// quad lists, run first like k_hv_cols_quad: a tile's slab of x is half as
+// large (1.65 MB for 12,870 rows), so more of it stays in the XCD's L2 while
+// the tile's rows gather from it.  The block's rows' lists are staged in LDS
+// and each wave takes two rows at a time.  Requires rch <= kOctRows, an even
+// num_dn and (num_up + 1) * num_dn * 8 < 2^32.
+__global__ __launch_bounds__(256) void k_hv_cols_oct(Hv2Args a, const double* __restrict__ x,
+                                                     double* __restrict__ y, HvEpilogue e, int64_t ntiles,
+                                                     int64_t nrc, int rch, double*) {
+  __shared__ uint32_t lists[kOctRows * kQuadWords];
+  const int64_t b = blockIdx.x;
+  const int64_t j = b >> 3;
+  const int64_t tile = (j / nrc) * 8 + (b & 7);
+  const int64_t row0 = (j % nrc) * rch;
+  if (tile >= ntiles) return;  // the whole block
+  const int64_t rend = row0 + rch < a.nu ? row0 + rch : a.nu;
+  const int nrows = (int)(rend - row0);
+  const int stride = a.up_quad_stride;
+  for (int t = threadIdx.x; t < nrows * stride; t += blockDim.x) lists[t] = a.up_quad[row0 * stride + t];
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int ei = lane >> 3;
+  const int q = ei & 3;
+  const uint32_t psel = 0x0c0c0000u | ((uint32_t)(2 * q + 1) << 8) | (uint32_t)(2 * q);
+  const int64_t nd = a.nd;
+  const int64_t col = tile * 16 + 2 * (lane & 7);
+  const bool act = col < nd;
+  const int64_t colc = act ? col : nd - 2;
+  const uint32_t laneoff = (uint32_t)colc * 8u;
+  const bool wr = act && ei == 0;
+  const int64_t ld = a.ld;
+  const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(x, (uint32_t)(a.nu * ld * 8));
+  const double st = (e.sx ? *e.sx : 1.0) * a.hop[0];
+  const uint32_t nd8 = (uint32_t)ld * 8u;
+  const uint32_t pad = (uint32_t)a.nu;
+  for (int ra = wave; ra < nrows; ra += 8) {
+    const int rb = ra + 4;
+    const bool hb = rb < nrows;  // uniform
+    const uint32_t h0 = oct_word(lists, ra, stride, 0);
+    const uint32_t h1 = hb ? oct_word(lists, rb, stride, 0) : 0u;
+    const int qp[2] = {(int)(h0 & 0xffu), (int)(h1 & 0xffu)};
+    const int nq[2] = {(int)((h0 >> 8) & 0xffu), (int)((h1 >> 8) & 0xffu)};
+    const int no = ((nq[0] > nq[1] ? nq[0] : nq[1]) + 1) >> 1;
+    double sm[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    const int rbc = hb ? rb : ra;  // a missing second row reads the first row's words, all masked
+    switch (no) {
+#define ED_OCT_CASE(n) \
+  case n: sum_octs2<n>(lists, stride, ra, rbc, qp, nq, pad, ei, psel, nd8, laneoff, rx, sm); break;
+      ED_OCT_CASE(1) ED_OCT_CASE(2) ED_OCT_CASE(3) ED_OCT_CASE(4) ED_OCT_CASE(5) ED_OCT_CASE(6)
+#undef ED_OCT_CASE
+      default: break;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        double v = sm[h][c];
+        v += __shfl_xor(v, 8, 64);
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        sm[h][c] = v;
+      }
+    if (wr) {
+      __builtin_nontemporal_store(f64x2{st * sm[0][0], st * sm[0][1]}, (f64x2*)(y + (row0 + ra) * ld + colc));
+      if (hb)
+        __builtin_nontemporal_store(f64x2{st * sm[1][0], st * sm[1][1]}, (f64x2*)(y + (row0 + rb) * ld + colc));
+    }
+  }
+}
+
 static int env_int(const char* name, int dflt) {
   const char* v = getenv(name);
   return v && *v ? atoi(v) : dflt;
@@ -1343,9 +1468,12 @@ static TwoPassPlan plan_two_pass(const ed_op* op, int64_t ld) {
       // turns them off)
       p.col_first = true;
       if (op->up.quad16 && op->num_dn % 2 == 0 && ld % 2 == 0 && env_int("CMPY_AMD_HV_QUAD", 1) != 0) {
-        p.tw = 32;
-        p.ntiles = (op->num_dn + 31) / 32;
-        p.ck = k_hv_cols_quad;
+        // CMPY_AMD_HV_OCT=1: 16-column tiles from the same lists
+        const bool oct = env_int("CMPY_AMD_HV_OCT", 0) != 0;
+        if (oct) p.rch = std::min(p.rch, kOctRows);
+        p.tw = oct ? 16 : 32;
+        p.ntiles = (op->num_dn + p.tw - 1) / p.tw;
+        p.ck = oct ? k_hv_cols_oct : k_hv_cols_quad;
       } else {
         p.ck = p.tw == 64 ? k_hv_cols_first<64> : k_hv_cols_first<32>;
       }
