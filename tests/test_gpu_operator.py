@@ -220,9 +220,10 @@ def test_large_sector_matvec_both_kernels(gpu, geom, monkeypatch):
     # column passes (32- and 64-column tiles): run first and written into y
     # (the default; quad lists with 16-byte loads, or pair lists), or after the
     # row pass
-    base = dict(MODE="1", SPLIT="1", TW="32", HALF="1", PAIR="1", PTW="32", FIRST="1", QUAD="1", HELD="1", OCT="0")
+    base = dict(MODE="1", SPLIT="1", TW="32", HALF="1", PAIR="1", PTW="32", FIRST="1", QUAD="1", HELD="1", OCT="0",
+                PRCH="16")
     combos = (dict(MODE="0"), dict(PAIR="0", TW="16"), dict(PAIR="0"), dict(PAIR="0", TW="64"), {}, dict(HELD="0"),
-              dict(OCT="1"), dict(QUAD="0"), dict(QUAD="0", PTW="64"),
+              dict(OCT="1"), dict(OCT="1", PRCH="8"), dict(QUAD="0"), dict(QUAD="0", PTW="64"),
               dict(HALF="0"), dict(FIRST="0"), dict(FIRST="0", PTW="64"), dict(SPLIT="0"),
               dict(SPLIT="0", TW="16"))
     for over in combos:
