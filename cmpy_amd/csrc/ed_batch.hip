@@ -1333,7 +1333,7 @@ __device__ __forceinline__ double bond_hops(const SpinList& U, const SpinList& D
 constexpr int kPc = 16;             // W columns per LDS chunk in the backward
 constexpr int kBwdLds = 512 * kPc;  // doubles of the backward's LDS staging buffer (64 KiB)
 constexpr int kBwdMaxSteps = 1024;  // continued-fraction steps a kept basis may hold
-constexpr int kBwdSplitM = 128;     // parts with this many steps run their K products device-wide
+constexpr int kBwdSplitM = 96;      // parts with this many steps run their K products device-wide
 
 // sum_j x[j] y[j] over j < n in order, eight entries' loads in flight at a
 // time (the same sum as one term at a time).

@@ -151,14 +151,14 @@ def test_nine_bath_block_rows_against_wave_mode(harness, tmp_path):
     assert np.abs(gb - gw).max() <= 1e-9 * np.abs(gw).max()
 
 
-@pytest.mark.parametrize("split_m", ["128", "0"])
+@pytest.mark.parametrize("split_m", ["96", "0"])
 def test_backward_against_finite_differences(harness, tmp_path, split_m):
     """keep_basis and the adjoint backward (FP64 matrix-core products
     emulated with the instruction's lane layout): clean, and dL/d(eps_k, V_k)
     for L = sum_n Re G + Im G equal to central differences of the oracle's
     Lehmann G.  split_m "0" sends every part's K products through the
     device-wide kernels (grouped GEMM, row-block sums, per-model combine),
-    the default only parts of at least 128 steps."""
+    the default only parts of at least 96 steps."""
     models = _models((5, 6), per=1)
     wn = (2 * np.arange(8) + 1) * np.pi / 50.0
     path = tmp_path / "keep.txt"

@@ -139,7 +139,7 @@ def test_backward_long_continued_fractions(gpu, monkeypatch):
     assert res.cf_steps.max() > 512, res.cf_steps  # two wave groups of the Krylov product
     assert not res.cf_capped.any(), res.cf_steps
     ge, gv, cg, _ = b.backward(gr, gi)
-    # these parts' K products run device-wide by default (>= 128 steps); the
+    # these parts' K products run device-wide by default (>= 96 steps); the
     # per-model workgroup path gives the same gradient
     monkeypatch.setenv("CMPY_AMD_BWD_SPLIT_M", "100000")
     ge1, gv1, _, _ = b.backward(gr, gi)
