@@ -101,8 +101,9 @@ def test_groundstate_12site_chain_partial_reorth(gpu, u):
 
 
 def test_groundstate_4x4_one_up_six_dn(gpu):
-    """4x4 torus, sector (1, 6), D = 128,128: the Lanczos epilogue through the
-    two-pass sweep with the half-width row tables (8008 dn states)."""
+    """4x4 torus, sector (1, 6), D = 128,128 (8008 dn states): the Lanczos
+    epilogue through the default H.v of this size, the fused single pass
+    (the two-pass paths: test_eigenvector_pass_every_row_kernel)."""
     from cmpy_amd import HubbardModel, groundstate, square_neighbors
     import scipy.sparse.linalg as sla
 
@@ -115,6 +116,32 @@ def test_groundstate_4x4_one_up_six_dn(gpu):
     assert np.linalg.norm(r) <= 2e-10
     w, _ = sla.eigsh(ref.csr(), k=1, which="SA", tol=1e-12, v0=psi)
     assert abs(gs.energy - w[0]) <= E0_RTOL * abs(w[0])
+
+
+@pytest.mark.parametrize("env", [{}, {"HELD": "0"}, {"HALF": "0"}, {"FIRST": "0"}, {"MODE": "0"}, {"SPLIT": "0"},
+                                 {"OCT": "1"}])
+def test_eigenvector_pass_every_row_kernel(gpu, monkeypatch, env):
+    """The eigenvector's second pass applies alpha_j and accumulates psi in the
+    row pass's epilogue; every H.v path carries it (held and per-chunk y_up
+    rows, 32-bit offset tables, the row pass first, the fused single pass,
+    general tables, the 16-column column pass).  4x4 torus, sector (1, 6),
+    D = 128,128: the regenerated psi has an explicit residual under the
+    oracle's operator at the solver's tolerance, and the energy of the default
+    path.  The sector is below the two-pass threshold, so the two passes are
+    forced (CMPY_AMD_HV_MODE=1) except in the single-pass case."""
+    from cmpy_amd import HubbardModel, groundstate, square_neighbors
+
+    ref = O.hubbard_operator(16, 1, 6, O.square_bonds(4, 4), 3.0, 0.0, 1.0)
+    m = HubbardModel(16, square_neighbors(4, 4), inter=3.0, eps=0.0, hop=1.0)
+    e_default = groundstate(m, 1, 6, tol=1e-10, reorth=False, seed=3).energy
+    for k, v in {"MODE": "1", **env}.items():
+        monkeypatch.setenv("CMPY_AMD_HV_" + k, v)
+    m.clear_operator_cache()
+    gs, st = groundstate(m, 1, 6, tol=1e-10, reorth=False, return_stats=True, seed=3)
+    assert st.converged and st.residual <= 1e-10, (env, st)
+    psi = gs.state / np.linalg.norm(gs.state)
+    assert np.linalg.norm(ref.matvec(psi) - gs.energy * psi) <= 2e-10, env
+    assert abs(gs.energy - e_default) <= 1e-12 * abs(e_default), env
 
 
 def test_groundstate_4x4_half_filling(gpu):
