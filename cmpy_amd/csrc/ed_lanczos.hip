@@ -14,7 +14,10 @@
 //   w     -= alpha_j s_j u_j,  beta_{j+1} = |w|          (one vector pass)
 //   u_{j+1} = w,  s_{j+1} = 1/beta_{j+1}
 // All reductions are two-stage with a fixed summation order, so repeated runs
-// (and the eigenvector's second pass) reproduce every vector bit for bit.
+// reproduce every vector bit for bit.  The eigenvector's second pass
+// regenerates the vectors from the stored coefficients with alpha_j folded
+// into the H.v (no separate update), so its vectors equal the first pass's
+// to rounding.
 #include "ed_internal.h"
 
 #include <algorithm>
@@ -198,8 +201,12 @@ __global__ __launch_bounds__(kVecThreads) void k_update(double* __restrict__ w, 
   }
 }
 
+typedef double v2d __attribute__((ext_vector_type(2)));
+
 // k_update on pairs of doubles (16-byte loads and stores; all pointers
-// 16-byte aligned, D2 = D / 2 with D even)
+// 16-byte aligned, D2 = D / 2 with D even).  u and w are streamed with
+// nontemporal hints: neither stays cache-resident at these sizes, and the
+// next step reads both from memory anyway.
 __global__ __launch_bounds__(kVecThreads) void k_update2(double2* __restrict__ w, const double2* __restrict__ u,
                                                          int64_t D2, double* alpha, const double* scale,
                                                          double2* __restrict__ psi, double yj, double* partial,
@@ -210,11 +217,11 @@ __global__ __launch_bounds__(kVecThreads) void k_update2(double2* __restrict__ w
   double acc = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < D2; k += stride) {
-    const double2 uk = u[k];
-    double2 wk = w[k];
+    const v2d uk = __builtin_nontemporal_load((const v2d*)(u + k));
+    v2d wk = __builtin_nontemporal_load((const v2d*)(w + k));
     wk.x -= a * uk.x;
     wk.y -= a * uk.y;
-    w[k] = wk;
+    __builtin_nontemporal_store(wk, (v2d*)(w + k));
     acc += wk.x * wk.x + wk.y * wk.y;
     if (psi) {
       double2 pk = psi[k];
