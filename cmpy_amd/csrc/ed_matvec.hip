@@ -518,7 +518,7 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_half(Hv2Args a, int row
     const int64_t gi = iu * ld + ids;
     // psi read before the gathers, so its HBM latency overlaps them
     const double pso = e.psi && act ? __builtin_nontemporal_load(e.psi + gi) : 0.0;
-    const double xp = e.xprev ? e.xprev[gi] : 0.0;
+    const double xp = e.xprev ? __builtin_nontemporal_load(e.xprev + gi) : 0.0;
     const double yu = ADDUP ? __builtin_nontemporal_load(y + gi) : 0.0;
     const uint32_t sdn = a.st_dn[ids];
     const double ddn = a.d_dn[ids];
@@ -600,7 +600,7 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_held(Hv2Args a, int row
       const int64_t id = c0 + lane;
       const int64_t gi = iu * ld + (id < nd ? id : c0);
       if (ADDUP) held[k] = __builtin_nontemporal_load(y + gi);
-      if (e.xprev) ph[k] = e.xprev[gi];
+      if (e.xprev) ph[k] = __builtin_nontemporal_load(e.xprev + gi);
     }
   }
   stage_rows<kRowThreads, 16>(row, x + iu * ld, nd);
