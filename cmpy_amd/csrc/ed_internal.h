@@ -183,6 +183,10 @@ constexpr int kQuadWords = 1 + 2 * kQuadMaxQuads;
 // pads D .. D+31 must fit 16 bits, and the staged row with its pads must fit
 // the LDS of one workgroup (checked when the kernel is chosen).
 constexpr int64_t kHalfMaxStates = 65535 - 32;
+// Vectors of at least this many states (128 MiB) are streamed with
+// nontemporal hints where a kernel reads or writes them once per step; below
+// it they stay cache-resident between the launches of a step.
+constexpr int64_t kStreamStates = int64_t(1) << 24;
 constexpr int kHalfMaxWords = 24;  // longest per-chunk word list the row pass unrolls
 
 __host__ __device__ inline uint32_t pack_hop(int32_t target, int bond, int sign) {

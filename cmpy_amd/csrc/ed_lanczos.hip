@@ -204,9 +204,10 @@ __global__ __launch_bounds__(kVecThreads) void k_update(double* __restrict__ w, 
 typedef double v2d __attribute__((ext_vector_type(2)));
 
 // k_update on pairs of doubles (16-byte loads and stores; all pointers
-// 16-byte aligned, D2 = D / 2 with D even).  u and w are streamed with
-// nontemporal hints: neither stays cache-resident at these sizes, and the
-// next step reads both from memory anyway.
+// 16-byte aligned, D2 = D / 2 with D even).  NT: u and w streamed with
+// nontemporal hints, for vectors far larger than the caches (the next step
+// reads both from memory anyway); smaller ones keep them cache-resident.
+template <bool NT>
 __global__ __launch_bounds__(kVecThreads) void k_update2(double2* __restrict__ w, const double2* __restrict__ u,
                                                          int64_t D2, double* alpha, const double* scale,
                                                          double2* __restrict__ psi, double yj, double* partial,
@@ -217,11 +218,14 @@ __global__ __launch_bounds__(kVecThreads) void k_update2(double2* __restrict__ w
   double acc = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < D2; k += stride) {
-    const v2d uk = __builtin_nontemporal_load((const v2d*)(u + k));
-    v2d wk = __builtin_nontemporal_load((const v2d*)(w + k));
+    const v2d uk = NT ? __builtin_nontemporal_load((const v2d*)(u + k)) : *(const v2d*)(u + k);
+    v2d wk = NT ? __builtin_nontemporal_load((const v2d*)(w + k)) : *(const v2d*)(w + k);
     wk.x -= a * uk.x;
     wk.y -= a * uk.y;
-    __builtin_nontemporal_store(wk, (v2d*)(w + k));
+    if (NT)
+      __builtin_nontemporal_store(wk, (v2d*)(w + k));
+    else
+      *(v2d*)(w + k) = wk;
     acc += wk.x * wk.x + wk.y * wk.y;
     if (psi) {
       double2 pk = psi[k];
@@ -406,8 +410,8 @@ static void launch_update(int vb, hipStream_t s, double* w, const double* u, int
                           const double* hv_part = nullptr, int np = 0) {
   const uintptr_t al = (uintptr_t)w | (uintptr_t)u | (uintptr_t)psi;
   if (D % 2 == 0 && (al & 15) == 0)
-    hipLaunchKernelGGL(k_update2, dim3(vb), dim3(kVecThreads), 0, s, (double2*)w, (const double2*)u, D / 2, alpha,
-                       scale, (double2*)psi, yj, partial, hv_part, np);
+    hipLaunchKernelGGL(D >= kStreamStates ? k_update2<true> : k_update2<false>, dim3(vb), dim3(kVecThreads), 0, s,
+                       (double2*)w, (const double2*)u, D / 2, alpha, scale, (double2*)psi, yj, partial, hv_part, np);
   else
     hipLaunchKernelGGL(k_update, dim3(vb), dim3(kVecThreads), 0, s, w, u, D, alpha, scale, psi, yj, partial,
                        hv_part, np);

@@ -518,7 +518,7 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_half(Hv2Args a, int row
     const int64_t gi = iu * ld + ids;
     // psi read before the gathers, so its HBM latency overlaps them
     const double pso = e.psi && act ? __builtin_nontemporal_load(e.psi + gi) : 0.0;
-    const double xp = e.xprev ? __builtin_nontemporal_load(e.xprev + gi) : 0.0;
+    const double xp = e.xprev ? e.xprev[gi] : 0.0;
     const double yu = ADDUP ? __builtin_nontemporal_load(y + gi) : 0.0;
     const uint32_t sdn = a.st_dn[ids];
     const double ddn = a.d_dn[ids];
@@ -572,10 +572,11 @@ __device__ __forceinline__ double sum_words_q_n(int ng, const u32x4* tq, int nwp
 // together with the staging of x, held in registers (one per column chunk of
 // the thread, indexed by the uniform chunk counter), so no chunk waits for
 // HBM: only table words (L2) and LDS gathers remain on each chunk's path.
-// For num_dn <= kRowHeld * kRowThreads.
+// For num_dn <= kRowHeld * kRowThreads.  NT: x_prev read with a nontemporal
+// hint (vectors of at least kStreamStates states, read once per step).
 constexpr int kRowHeld = 16;
 
-template <bool ADDUP>
+template <bool ADDUP, bool NT = false>
 __global__ __launch_bounds__(kRowThreads) void k_hv_rows_held(Hv2Args a, int rows_per_block,
                                                               const double* __restrict__ x,
                                                               double* __restrict__ y, HvEpilogue e) {
@@ -600,7 +601,7 @@ __global__ __launch_bounds__(kRowThreads) void k_hv_rows_held(Hv2Args a, int row
       const int64_t id = c0 + lane;
       const int64_t gi = iu * ld + (id < nd ? id : c0);
       if (ADDUP) held[k] = __builtin_nontemporal_load(y + gi);
-      if (e.xprev) ph[k] = __builtin_nontemporal_load(e.xprev + gi);
+      if (e.xprev) ph[k] = NT ? __builtin_nontemporal_load(e.xprev + gi) : e.xprev[gi];
     }
   }
   stage_rows<kRowThreads, 16>(row, x + iu * ld, nd);
@@ -1417,7 +1418,8 @@ static TwoPassPlan plan_two_pass(const ed_op* op, int64_t ld) {
   const bool half = split && R <= 2 && op->dn.h16 &&
                     sizeof(double) * (size_t)(op->num_dn + 32 + kRowThreads / 64) <= budget &&
                     env_int("CMPY_AMD_HV_HALF", 1) != 0 && lds_starts_at_zero(k_hv_rows_half<false>) &&
-                    lds_starts_at_zero(k_hv_rows_half<true>) && lds_starts_at_zero(k_hv_rows_held<true>);
+                    lds_starts_at_zero(k_hv_rows_half<true>) && lds_starts_at_zero(k_hv_rows_held<true>) &&
+                    lds_starts_at_zero(k_hv_rows_held<true, true>);
   if (half) {
     p.rows_per_block = R = 1;
     p.lds_bytes = sizeof(double) * (size_t)(op->num_dn + 32 + kRowThreads / 64);
@@ -1480,7 +1482,7 @@ static TwoPassPlan plan_two_pass(const ed_op* op, int64_t ld) {
       p.nrc = (op->num_up + p.rch - 1) / p.rch;
       if (p.rk == k_hv_rows_half<false>)
         p.rk = op->num_dn <= (int64_t)kRowHeld * kRowThreads && env_int("CMPY_AMD_HV_HELD", 1) != 0
-                   ? k_hv_rows_held<true>
+                   ? (op->num_up * ld >= kStreamStates ? k_hv_rows_held<true, true> : k_hv_rows_held<true>)
                    : k_hv_rows_half<true>;
       else if (p.rk == k_hv_rows_off<1, false>)
         p.rk = k_hv_rows_off<1, true>;
